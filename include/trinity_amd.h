@@ -1,0 +1,176 @@
+/*
+ * trinity_amd.h -- C ABI of the MI355X (gfx950) implementation of
+ * hobywan/trinity's metric-and-smoothing path.
+ *
+ * The reference is a C++ library whose path is driven through
+ *   trinity::Metrics(Mesh*, double target, int Lp, double h_min, double h_max)
+ *       .run(Stats*)                                   include/trinity/metric.h:38-44
+ *   trinity::Partit(Mesh const*, int max_part).extractColoring(const Mesh*)
+ *                                                      include/trinity/partition.h:46-49
+ *   trinity::Smooth(Mesh*, Partit*, int level).run(Stats*)
+ *                                                      include/trinity/smoothing.h:39-42
+ * on the arrays of trinity::Mesh (include/trinity/mesh.h:108-149).  The
+ * entry points below replace those passes one for one; the drop-in sources
+ * in trinity_amd/dropin/ re-implement the three classes on top of them.
+ *
+ * Conventions
+ *  - Every function returns TRN_OK (0) or a negative TRN_ERR_* code and
+ *    never throws.  trn_last_error() gives a message for the calling thread.
+ *  - Pointers suffixed _d are DEVICE pointers (caller-owned).  Arrays use
+ *    the reference layout: xy[2nv] interleaved (x,y) = Mesh::geom.points,
+ *    tensor[3nv] = (m11, m12, m22) = Mesh::geom.tensor, tri[3nt] 0-based
+ *    = Mesh::topo.elems, tags[nv] = Mesh::sync.tags (bit 2 = boundary),
+ *    qualit[nt] = Mesh::geom.qualit, solut[nv] = Mesh::geom.solut.
+ *  - All work is ordered on the context's stream (trn_ctx_set_stream);
+ *    functions that return host values synchronise that stream first.
+ *  - Everything is FP64; results follow the reference's operation order
+ *    and are built without floating-point contraction.
+ */
+#ifndef TRINITY_AMD_H
+#define TRINITY_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TRN_OK            0
+#define TRN_ERR_ARG      -1   /* invalid argument (null pointer, size < 0, ...) */
+#define TRN_ERR_HIP      -2   /* a HIP runtime call failed */
+#define TRN_ERR_CAPACITY -3   /* stencil larger than the kernels support */
+#define TRN_ERR_PALETTE  -4   /* colouring needs more colours than allowed */
+#define TRN_ERR_STATE    -5   /* internal inconsistency */
+
+typedef struct trn_ctx trn_ctx;    /* device, stream, scratch, timers */
+typedef struct trn_mesh trn_mesh;  /* device-resident topology (CSR) */
+
+/* Library version string. */
+const char* trn_version(void);
+/* Message of the last failing call on this thread ("" if none). */
+const char* trn_last_error(void);
+
+/* ---- context ----------------------------------------------------------- */
+/* Creates a context on `device` with its own default-priority stream. */
+int trn_ctx_create(int device, trn_ctx** out);
+int trn_ctx_destroy(trn_ctx* ctx);
+/* Routes all work to `hip_stream` (a hipStream_t; NULL = the null stream),
+ * e.g. the stream a host framework allocates on. */
+int trn_ctx_set_stream(trn_ctx* ctx, void* hip_stream);
+/* Routes all work back to the context's own stream (the default). */
+int trn_ctx_use_own_stream(trn_ctx* ctx);
+int trn_ctx_synchronize(trn_ctx* ctx);
+/* Per-kernel timing with hipEvents around every launch (off by default). */
+int trn_ctx_set_timing(trn_ctx* ctx, int enable);
+int trn_ctx_reset_timing(trn_ctx* ctx);
+/* Number of timed kernels; then, per index, name, total ms, launches and
+ * algorithmic bytes credited (synchronises the stream). */
+int trn_ctx_timing_count(trn_ctx* ctx, int32_t* count);
+int trn_ctx_timing_get(trn_ctx* ctx, int32_t index, const char** name,
+                       double* total_ms, int64_t* launches, double* bytes);
+
+/* ---- topology: Mesh::load + Mesh::rebuildTopology ------------------------
+ * Replaces the topology part of Mesh::load (src/io.cpp:97-99, :128-129) and
+ * Mesh::rebuildTopology (src/mesh.cpp:191-249): builds, on the device,
+ *   v2t  incident triangles per vertex, ascending  (Mesh::topo.stenc/sync.deg)
+ *   v2v  sorted-unique 1-ring                      (Mesh::topo.vicin)
+ * tri (3nt, 0-based; a negative first index marks a deleted triangle as in
+ * mesh.cpp:200) and tags (nv) may be host or device pointers; they are
+ * copied into the mesh.  Also measures the largest 2-ring element patch
+ * (Mesh::getVicinity(i,2), src/mesh.cpp:390-434) the recovery will gather. */
+int trn_mesh_create(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
+                    const uint8_t* tags, trn_mesh** out);
+int trn_mesh_destroy(trn_mesh* mesh);
+/* Sizes: nnz of v2v, max triangles per vertex, max 1-ring, max 2-ring patch. */
+int trn_mesh_info(const trn_mesh* mesh, int64_t* nnz_v2v, int32_t* max_deg_t,
+                  int32_t* max_deg_v, int32_t* max_patch);
+/* Device pointers of the mesh-owned CSR arrays (valid until destroy). */
+int trn_mesh_csr(const trn_mesh* mesh, const int32_t** v2t_off_d,
+                 const int32_t** v2t_d, const int32_t** v2v_off_d,
+                 const int32_t** v2v_d);
+
+/* ---- metric: Metrics::run and its stages ----------------------------------
+ * trn_metric_run replaces Metrics::run (src/metric.cpp:59-70) with
+ * Metrics(mesh, target_factor, lp_norm, h_min, h_max): L2-projection
+ * gradient (src/hessian.cpp:24-83) and Hessian (:86-153) over the 2-ring
+ * patch, local L^p normalisation (src/metric.cpp:100-150), complexity
+ * (:153-185) and global normalisation with eigenvalue clamps (:188-238).
+ * Writes the metric into tensor_d [3nv].  complexity (host) may be NULL; if
+ * given the call synchronises.  gradient_d [2nv] and hessian_d [3nv]
+ * (symmetrised raw Hessian) are optional stage outputs (NULL = not kept). */
+int trn_metric_run(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d,
+                   const double* solut_d, double target_factor,
+                   int32_t lp_norm, double h_min, double h_max,
+                   double* tensor_d, double* complexity,
+                   double* gradient_d, double* hessian_d);
+/* Metrics::computeGradient for every vertex (src/hessian.cpp:24-83). */
+int trn_gradient(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d,
+                 const double* solut_d, double* gradient_d);
+/* Metrics::computeHessian for every vertex (src/hessian.cpp:86-153). */
+int trn_hessian(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d,
+                const double* gradient_d, double* tensor_d);
+/* Metrics::normalizeLocally (src/metric.cpp:100-150), in place. */
+int trn_normalize_local(trn_ctx* ctx, int32_t nv, double* tensor_d,
+                        int32_t lp_norm);
+/* Metrics::computeComplexity (src/metric.cpp:153-185); deterministic
+ * two-level tree sum.  Synchronises. */
+int trn_complexity(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d,
+                   const double* tensor_d, double* complexity);
+/* Metrics::normalizeGlobally (src/metric.cpp:188-238) with
+ * scale_fact = target / complexity, in place. */
+int trn_normalize_global(trn_ctx* ctx, int32_t nv, double* tensor_d,
+                         double scale_fact, double h_min, double h_max);
+
+/* ---- colouring: Partit::extractColoring ------------------------------------
+ * Replaces Partit::extractColoring (src/partition.cpp:79-197): a seeded,
+ * deterministic speculative-greedy colouring with conflict repair.  Every
+ * vertex with a non-empty 1-ring gets a colour in 1..max_colors such that no
+ * edge joins two equal colours (the reference uses 1..max_part-1 with
+ * Partit(mesh, 8), i.e. at most 7).  Returns TRN_ERR_PALETTE if a vertex
+ * cannot be coloured within max_colors (<= 64).  Also emits the colour
+ * classes (Partit::task.subset / cardin / nb.parts): order_d [nv] lists the
+ * coloured vertices grouped by colour, ascending within a colour;
+ * class_off (host, max_colors+2 entries) receives the offsets of colours
+ * 1..ncolors in order_d.  Synchronises. */
+int trn_color(trn_ctx* ctx, const trn_mesh* mesh, uint64_t seed,
+              int32_t max_colors, int32_t* color_d, int32_t* order_d,
+              int32_t* class_off, int32_t* ncolors);
+/* Colour classes of an externally supplied colouring (colour 0 = not
+ * coloured), e.g. the reference's own.  class_off as above.  Synchronises. */
+int trn_color_classes(trn_ctx* ctx, int32_t nv, const int32_t* color_d,
+                      int32_t ncolors, int32_t* order_d, int32_t* class_off);
+/* Counts edges whose end points share a colour and non-isolated vertices
+ * left uncoloured or outside 1..ncolors (0 and 0 = valid).  Synchronises. */
+int trn_color_check(trn_ctx* ctx, const trn_mesh* mesh, const int32_t* color_d,
+                    int32_t ncolors, int64_t* bad_edges, int64_t* bad_vertices);
+
+/* ---- smoothing: Smooth::run ---------------------------------------------
+ * Smooth::cacheQuality (src/smoothing.cpp:145-155): qualit_d[t] =
+ * Mesh::computeQuality(t) (src/mesh.cpp:626-644, numeric.cpp:251-306). */
+int trn_quality_cache(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d,
+                      const double* tensor_d, double* qualit_d);
+/* Replaces Smooth::run (src/smoothing.cpp:274-302) after its colouring:
+ * `sweeps` rounds (Smooth's `level`) of Smooth::movePoints (:158-180), each
+ * visiting colours 1..ncolors in ascending order and applying
+ * moveSmartLaplacian (:40-126) to every non-boundary vertex of the class.
+ * order_d/class_off as produced by trn_color or trn_color_classes.  If
+ * recompute_qualit != 0 the quality cache is rebuilt first (the reference
+ * rebuilds it once per Smooth::run, :290).  commits/tasks (host, [sweeps],
+ * may be NULL) receive the per-sweep accepted-move and task counts; when
+ * either is given the call synchronises. */
+int trn_smooth_run(trn_ctx* ctx, const trn_mesh* mesh, double* xy_d,
+                   double* tensor_d, double* qualit_d, const int32_t* order_d,
+                   const int32_t* class_off, int32_t ncolors, int32_t sweeps,
+                   int32_t recompute_qualit, int32_t* commits, int32_t* tasks);
+
+/* ---- device math probe ----------------------------------------------------
+ * Evaluates the device transcendental used by the path on n inputs
+ * (op: 0 log, 1 exp, 2 pow(x, y), 3 pow(x, 3), 4 sqrt) -- lets tests compare
+ * the device library against the host libm that the reference links. */
+int trn_debug_math(trn_ctx* ctx, int32_t op, int64_t n, const double* x_d,
+                   const double* y_d, double* out_d);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TRINITY_AMD_H */

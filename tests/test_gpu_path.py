@@ -1,0 +1,247 @@
+"""GPU parity tests of the HIP path, called through the C ABI, against the
+reference's golden fixtures and the oracle on the same seeded inputs.
+
+Tolerances (FP64):
+  gradient, raw Hessian                    bit-exact (same operation order,
+                                           no contraction, IEEE div/sqrt)
+  locally normalised metric                1e-14 per-vertex norm-relative
+                                           (device pow vs glibc pow: 1 ulp)
+  complexity                               1e-12 relative (tree-ordered sum)
+  final metric                             1e-10 per-vertex norm-relative
+  element quality                          1e-10 relative
+  smoothed coordinates (same colouring)    1e-12 per-vertex norm-relative,
+                                           commit and task counts exact
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import golden_cases, load_golden, normrel
+from trinity_amd import meshgen as mg
+
+pytestmark = pytest.mark.gpu
+
+CASES = golden_cases()
+
+
+def T():
+    import trinity_amd
+    return trinity_amd
+
+
+def dev(a, dtype=None):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.cuda()
+
+
+def bits_equal(a, b):
+    a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+@pytest.mark.parametrize("wl", ["uniform16:tanh", "jitter33:fronts", "jitter64:layer"])
+def test_topology_matches_rebuild_topology(ctx, wl):
+    m = mg.workload(wl)
+    dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags)
+    got = dm.csr()
+    topo = oracle.Topology(m.nv, m.tri)
+    for k in ("v2t_off", "v2t", "v2v_off", "v2v"):
+        ref = getattr(topo, k)
+        assert np.array_equal(got[k], ref[:got[k].size]), k
+    assert dm.max_patch == max(len(topo.patch(i)) for i in range(m.nv))
+
+
+def _metric_on_gpu(ctx, g):
+    import torch
+    tf, p, hmin, hmax, sweeps = g["params"]
+    nv = g["xy_in"].shape[0]
+    dm = T().DeviceMesh(ctx, nv, g["tri"], g["tags"])
+    xy = dev(g["xy_in"])
+    f = dev(g["solut"])
+    grad = torch.empty((nv, 2), dtype=torch.float64, device="cuda")
+    hess = torch.empty((nv, 3), dtype=torch.float64, device="cuda")
+    ten, c = T().metric_run(ctx, dm, xy, f, tf, int(p), hmin, hmax, gradient=grad, hessian=hess)
+    mloc = T().normalize_local(ctx, hess.clone(), int(p))
+    return dm, xy, {"gradient": grad.cpu().numpy(), "hessian": hess.cpu().numpy(),
+                    "metric_local": mloc.cpu().numpy(), "metric": ten.cpu().numpy(),
+                    "complexity": c}
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_metric_matches_reference_golden(ctx, name):
+    g = load_golden(name)
+    dm, xy, r = _metric_on_gpu(ctx, g)
+    assert abs(r["complexity"] - g["complexity"][0]) <= 1e-12 * abs(g["complexity"][0])
+    if "sample_v" in g:
+        iv = g["sample_v"]
+        assert bits_equal(r["gradient"][iv], g["s_gradient"])
+        assert bits_equal(r["hessian"][iv], g["s_hessian"])
+        assert normrel(r["metric_local"][iv], g["s_metric_local"]).max() <= 1e-14
+        assert normrel(r["metric"][iv], g["s_metric"]).max() <= 1e-10
+    else:
+        assert bits_equal(r["gradient"], g["gradient"])
+        assert bits_equal(r["hessian"], g["hessian"])
+        assert normrel(r["metric_local"], g["metric_local"]).max() <= 1e-14
+        assert normrel(r["metric"], g["metric"]).max() <= 1e-10
+
+
+def test_stage_entry_points_compose_to_metric_run(ctx):
+    import torch
+    m = mg.workload("jitter48:fronts")
+    dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags)
+    xy, f = dev(m.xy), dev(m.solut)
+    full, c = T().metric_run(ctx, dm, xy, f, 0.8, 2, 1e-9, 0.4)
+    g = T().gradient(ctx, dm, xy, f)
+    h = T().hessian(ctx, dm, xy, g)
+    T().normalize_local(ctx, h, 2)
+    c2 = T().complexity(ctx, dm, xy, h)
+    assert c2 == c
+    T().normalize_global(ctx, h, int(np.floor(m.nv * 0.8)) / c2, 1e-9, 0.4)
+    assert torch.equal(h, full)
+
+
+def test_metric_full_mesh_matches_oracle(ctx):
+    m = mg.workload("jitter300:layer")
+    st = oracle.metric_stages(m.xy, m.tri, m.solut, 0.8, 4, 1e-9, 0.4)
+    g = {"params": np.array([0.8, 4, 1e-9, 0.4, 1]), "xy_in": m.xy, "tri": m.tri,
+         "tags": m.tags, "solut": m.solut}
+    dm, xy, r = _metric_on_gpu(ctx, g)
+    assert bits_equal(r["gradient"], st["gradient"])
+    assert bits_equal(r["hessian"], st["hessian"])
+    assert abs(r["complexity"] - st["complexity"]) <= 1e-12 * st["complexity"]
+    assert normrel(r["metric"], st["metric"]).max() <= 1e-10
+
+
+@pytest.mark.parametrize("wl", ["uniform40:tanh", "jitter100:fronts"])
+def test_colouring_valid_deterministic_seeded(ctx, wl):
+    m = mg.workload(wl)
+    dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags)
+    topo = oracle.Topology(m.nv, m.tri)
+    a = T().color(ctx, dm, seed=3, max_colors=7)
+    b = T().color(ctx, dm, seed=3, max_colors=7)
+    c = T().color(ctx, dm, seed=4, max_colors=7)
+    ca, cb, cc = a.color.cpu().numpy(), b.color.cpu().numpy(), c.color.cpu().numpy()
+    assert np.array_equal(ca, cb)
+    assert not np.array_equal(ca, cc)
+    for col in (a, c):
+        assert T().color_check(ctx, dm, col.color, col.ncolors) == (0, 0)
+        assert oracle.color_valid(topo, col.color.cpu().numpy(), col.ncolors)
+        assert 1 <= col.ncolors <= 7
+        # classes: colour c lists its vertices in ascending order
+        order = col.order.cpu().numpy()
+        cv = col.color.cpu().numpy()
+        off = col.class_off
+        assert off[0] == 0 and off[-1] == m.nv
+        for k in range(col.ncolors):
+            seg = order[off[k]:off[k + 1]]
+            assert (cv[seg] == k + 1).all() and (np.diff(seg) > 0).all()
+
+
+def test_colouring_reports_palette_overflow(ctx):
+    m = mg.workload("uniform8:tanh")
+    dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags)
+    with pytest.raises(T().TrinityError) as e:
+        T().color(ctx, dm, seed=1, max_colors=2)
+    assert e.value.code == T().TRN_ERR_PALETTE
+
+
+def test_colour_checker_flags_conflicts(ctx):
+    m = mg.workload("uniform8:tanh")
+    dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags)
+    col = T().color(ctx, dm, seed=1)
+    bad = col.color.clone()
+    bad[10] = bad[11]
+    assert T().color_check(ctx, dm, bad, col.ncolors)[0] >= 1
+
+
+def test_invalid_triangles_are_rejected(ctx):
+    m = mg.workload("uniform4:tanh")
+    tri = m.tri.copy()
+    tri[3, 1] = m.nv + 5
+    with pytest.raises(T().TrinityError) as e:
+        T().DeviceMesh(ctx, m.nv, tri, m.tags)
+    assert e.value.code == T().TRN_ERR_ARG
+
+
+def _smooth_gpu(ctx, dm, xy0, metric, color_np, ncolors, sweeps):
+    import torch
+    col = T().color_classes(ctx, dev(color_np, torch.int32), ncolors)
+    xy = dev(xy0)
+    ten = dev(metric)
+    q = torch.empty(dm.nt, dtype=torch.float64, device="cuda")
+    com, tas = T().smooth_run(ctx, dm, xy, ten, q, col, sweeps)
+    return xy.cpu().numpy(), ten.cpu().numpy(), q.cpu().numpy(), com, tas
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_smoothing_matches_reference_golden(ctx, name):
+    """Fed the reference's colouring and metric, the sweeps reproduce the
+    reference's coordinates, qualities and accepted-move counts."""
+    g = load_golden(name)
+    tf, p, hmin, hmax, sweeps = g["params"]
+    nv = g["xy_in"].shape[0]
+    dm = T().DeviceMesh(ctx, nv, g["tri"], g["tags"])
+    if "sample_v" in g:
+        # metric and colouring from the bit-pinned oracle
+        st = oracle.metric_stages(g["xy_in"], g["tri"], g["solut"], tf, int(p), hmin, hmax)
+        metric = st["metric"]
+        color, nc = oracle.color_firstfit(st["topo"], 8)
+        assert np.array_equal(color[g["sample_v"]], g["s_color"])
+    else:
+        metric, color = g["metric"], g["color"]
+        nc = int(color.max())
+    xy, ten, q, com, tas = _smooth_gpu(ctx, dm, g["xy_in"], metric, color, nc, int(sweeps))
+    assert com == g["commits"].tolist()
+    assert tas == g["tasks"].tolist()
+    if "sample_v" in g:
+        iv, it = g["sample_v"], g["sample_t"]
+        assert normrel(xy[iv], g["s_xy"]).max() <= 1e-12
+        assert normrel(ten[iv], g["s_tensor"]).max() <= 1e-10
+        assert normrel(q[it], g["s_qualit"]).max() <= 1e-10
+    else:
+        assert normrel(xy, g["xy"]).max() <= 1e-12
+        assert normrel(ten, g["tensor"]).max() <= 1e-10
+        assert normrel(q, g["qualit"]).max() <= 1e-10
+
+
+def test_quality_cache_matches_oracle(ctx):
+    import torch
+    m = mg.workload("jitter80:fronts")
+    st = oracle.metric_stages(m.xy, m.tri, m.solut)
+    dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags)
+    q = T().quality_cache(ctx, dm, dev(m.xy), dev(st["metric"]))
+    q0 = oracle.quality_cache(m.xy, m.tri, st["metric"])
+    assert normrel(q.cpu().numpy(), q0).max() <= 1e-10
+
+
+@pytest.mark.parametrize("wl,p,sweeps", [("jitter200:fronts", 2, 3), ("uniform256:tanh", 2, 1),
+                                         ("jitter160:layer", 4, 2)])
+def test_smoothing_with_gpu_colouring_matches_oracle(ctx, wl, p, sweeps):
+    m = mg.workload(wl)
+    st = oracle.metric_stages(m.xy, m.tri, m.solut, 0.8, p)
+    dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags)
+    col = T().color(ctx, dm, seed=11, max_colors=8)
+    cnp = col.color.cpu().numpy()
+    xo, to, qo, co, tao = oracle.smooth_run(m.xy, m.tri, m.tags, st["metric"], cnp, col.ncolors,
+                                            sweeps, topo=st["topo"])
+    xy, ten, q, com, tas = _smooth_gpu(ctx, dm, m.xy, st["metric"], cnp, col.ncolors, sweeps)
+    assert tas == tao.tolist()
+    assert com == co.tolist()
+    assert normrel(xy, xo).max() <= 1e-12
+    assert normrel(ten, to).max() <= 1e-10
+
+
+def test_device_transcendentals_within_one_ulp_of_glibc(ctx):
+    rng = np.random.default_rng(0)
+    n = 1 << 18
+    cases = {0: np.exp(rng.uniform(-30, 30, n)), 1: rng.uniform(-30, 30, n),
+             2: np.exp(rng.uniform(-20, 20, n)), 4: np.exp(rng.uniform(-30, 30, n))}
+    y = -1.0 / (2 * rng.integers(1, 5, n) + 2)
+    for op, x in cases.items():
+        ref = oracle.libm(op, x, y if op == 2 else None)
+        out = T().debug_math(ctx, op, dev(x), dev(y) if op == 2 else None).cpu().numpy()
+        ulp = np.abs(out.view(np.int64) - ref.view(np.int64))
+        assert ulp.max() <= (0 if op == 4 else 1), op

@@ -1,0 +1,61 @@
+"""Synthetic mesh generator: sizes of the standard workloads, orientation,
+tags, and the I/O formats."""
+import os
+
+import numpy as np
+import pytest
+
+from trinity_amd import meshgen as mg
+
+
+@pytest.mark.parametrize("n,nv,nt", [(256, 66049, 131072), (1024, 1050625, 2097152),
+                                     (2048, 4198401, 8388608), (4096, 16785409, 33554432)])
+def test_grid_sizes(n, nv, nt):
+    assert (n + 1) ** 2 == nv and 2 * n * n == nt
+    if n <= 256:
+        m = mg.grid_mesh(n)
+        assert m.nv == nv and m.nt == nt
+
+
+def test_triangles_are_ccw_and_jitter_bounded():
+    n = 64
+    m = mg.grid_mesh(n, 0.25, seed=9)
+    a, b, c = m.xy[m.tri[:, 0]], m.xy[m.tri[:, 1]], m.xy[m.tri[:, 2]]
+    cross = (b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (c[:, 0] - a[:, 0]) * (b[:, 1] - a[:, 1])
+    assert (cross > 0).all()
+    u = mg.grid_mesh(n)
+    d = np.abs(m.xy - u.xy)
+    assert d.max() <= 0.25 / n + 1e-15
+    bnd = (m.tags & mg.TAG_BOUND) != 0
+    assert np.array_equal(m.xy[bnd], u.xy[bnd])
+    assert bnd.sum() == 4 * n
+    assert ((m.tags & mg.TAG_CORNER) != 0).sum() == 4
+
+
+def test_jitter_is_seeded():
+    a = mg.grid_mesh(16, 0.25, seed=1)
+    b = mg.grid_mesh(16, 0.25, seed=1)
+    c = mg.grid_mesh(16, 0.25, seed=2)
+    assert np.array_equal(a.xy, b.xy) and not np.array_equal(a.xy, c.xy)
+
+
+def test_batch_sizes_log_uniform():
+    sides = mg.batch_sizes(512)
+    nv = np.array([(s + 1) ** 2 for s in sides])
+    assert len(sides) == 512
+    assert nv.min() >= 1500 and nv.max() <= 2.1e5
+    assert 15e6 <= nv.sum() <= 30e6
+
+
+def test_trnm_and_medit_round_trip(tmp_path):
+    m = mg.workload("jitter8:fronts")
+    p = str(tmp_path / "m.trnm")
+    mg.write_trnm(p, m)
+    r = mg.read_trnm(p)
+    assert np.array_equal(r.xy, m.xy) and np.array_equal(r.tri, m.tri)
+    assert np.array_equal(r.tags, m.tags) and np.array_equal(r.solut, m.solut)
+    mp, bp = str(tmp_path / "m.mesh"), str(tmp_path / "m.bb")
+    mg.write_medit(mp, bp, m)
+    r2 = mg.read_medit(mp, bp)
+    assert np.array_equal(r2.xy, m.xy) and np.array_equal(r2.tri, m.tri)
+    assert np.array_equal(r2.tags, m.tags) and np.array_equal(r2.solut, m.solut)

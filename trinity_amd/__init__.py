@@ -1,0 +1,385 @@
+"""trinity_amd -- MI355X (gfx950) implementation of hobywan/trinity's
+metric-and-smoothing path, driven through the C ABI of libtrinity_amd.so
+(include/trinity_amd.h).
+
+This module is a thin ctypes binding used by the tests and the benchmark;
+PyTorch only supplies device memory and the stream.  There is no CPU
+fallback: every call goes to the HIP kernels, and loading fails loudly when
+the library has not been built (``python -m trinity_amd.build``).
+
+The Python names mirror the reference passes:
+    Metrics.run            -> metric_run      (src/metric.cpp:59-70)
+    Partit.extractColoring -> color           (src/partition.cpp:79-197)
+    Smooth.run             -> smooth_run      (src/smoothing.cpp:274-302)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtrinity_amd.so")
+
+TRN_OK = 0
+TRN_ERR_ARG = -1
+TRN_ERR_HIP = -2
+TRN_ERR_CAPACITY = -3
+TRN_ERR_PALETTE = -4
+TRN_ERR_STATE = -5
+
+# every symbol declared in include/trinity_amd.h
+EXPORTS = [
+    "trn_version", "trn_last_error", "trn_ctx_create", "trn_ctx_destroy",
+    "trn_ctx_set_stream", "trn_ctx_use_own_stream", "trn_ctx_synchronize", "trn_ctx_set_timing",
+    "trn_ctx_reset_timing", "trn_ctx_timing_count", "trn_ctx_timing_get",
+    "trn_mesh_create", "trn_mesh_destroy", "trn_mesh_info", "trn_mesh_csr",
+    "trn_metric_run", "trn_gradient", "trn_hessian", "trn_normalize_local",
+    "trn_complexity", "trn_normalize_global", "trn_color", "trn_color_classes",
+    "trn_color_check", "trn_quality_cache", "trn_smooth_run", "trn_debug_math",
+]
+
+
+class TrinityError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_f64 = ctypes.c_double
+_P = ctypes.POINTER
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m trinity_amd.build` "
+            "(there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "trn_version": ([], ctypes.c_char_p),
+        "trn_last_error": ([], ctypes.c_char_p),
+        "trn_ctx_create": ([ctypes.c_int, _P(_vp)], ctypes.c_int),
+        "trn_ctx_destroy": ([_vp], ctypes.c_int),
+        "trn_ctx_set_stream": ([_vp, _vp], ctypes.c_int),
+        "trn_ctx_use_own_stream": ([_vp], ctypes.c_int),
+        "trn_ctx_synchronize": ([_vp], ctypes.c_int),
+        "trn_ctx_set_timing": ([_vp, ctypes.c_int], ctypes.c_int),
+        "trn_ctx_reset_timing": ([_vp], ctypes.c_int),
+        "trn_ctx_timing_count": ([_vp, _P(_i32)], ctypes.c_int),
+        "trn_ctx_timing_get": ([_vp, _i32, _P(ctypes.c_char_p), _P(_f64), _P(_i64), _P(_f64)],
+                               ctypes.c_int),
+        "trn_mesh_create": ([_vp, _i32, _i32, _vp, _vp, _P(_vp)], ctypes.c_int),
+        "trn_mesh_destroy": ([_vp], ctypes.c_int),
+        "trn_mesh_info": ([_vp, _P(_i64), _P(_i32), _P(_i32), _P(_i32)], ctypes.c_int),
+        "trn_mesh_csr": ([_vp, _P(_vp), _P(_vp), _P(_vp), _P(_vp)], ctypes.c_int),
+        "trn_metric_run": ([_vp, _vp, _vp, _vp, _f64, _i32, _f64, _f64, _vp, _P(_f64), _vp, _vp],
+                           ctypes.c_int),
+        "trn_gradient": ([_vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+        "trn_hessian": ([_vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+        "trn_normalize_local": ([_vp, _i32, _vp, _i32], ctypes.c_int),
+        "trn_complexity": ([_vp, _vp, _vp, _vp, _P(_f64)], ctypes.c_int),
+        "trn_normalize_global": ([_vp, _i32, _vp, _f64, _f64, _f64], ctypes.c_int),
+        "trn_color": ([_vp, _vp, ctypes.c_uint64, _i32, _vp, _vp, _P(_i32), _P(_i32)],
+                      ctypes.c_int),
+        "trn_color_classes": ([_vp, _i32, _vp, _i32, _vp, _P(_i32)], ctypes.c_int),
+        "trn_color_check": ([_vp, _vp, _vp, _i32, _P(_i64), _P(_i64)], ctypes.c_int),
+        "trn_quality_cache": ([_vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+        "trn_smooth_run": ([_vp, _vp, _vp, _vp, _vp, _vp, _P(_i32), _i32, _i32, _i32, _P(_i32),
+                            _P(_i32)], ctypes.c_int),
+        "trn_debug_math": ([_vp, _i32, _i64, _vp, _vp, _vp], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def lib():
+    return _load()
+
+
+def _check(code: int) -> None:
+    if code != TRN_OK:
+        msg = lib().trn_last_error().decode(errors="replace")
+        raise TrinityError(code, msg)
+
+
+def version() -> str:
+    return lib().trn_version().decode()
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _ptr(t) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+def to_device(a, dtype=None, device="cuda"):
+    """numpy array or torch tensor -> contiguous torch tensor on `device`."""
+    torch = _torch()
+    if isinstance(a, torch.Tensor):
+        t = a
+        if dtype is not None:
+            t = t.to(dtype)
+        return t.to(device).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device=device, dtype=dtype).contiguous()
+
+
+class Context:
+    """A device, a stream and the scratch workspace (trn_ctx)."""
+
+    def __init__(self, device: int = 0, use_torch_stream: bool = True):
+        torch = _torch()
+        h = _vp()
+        _check(lib().trn_ctx_create(int(device), ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+        if use_torch_stream:
+            # launch on exactly the stream torch allocates on (its default is
+            # the null stream, pointer 0), so torch reads see finished results
+            s = torch.cuda.current_stream(device)
+            _check(lib().trn_ctx_set_stream(h, _vp(s.cuda_stream) if s.cuda_stream else None))
+
+    def set_stream(self, stream_ptr: int | None) -> None:
+        """Route work to a hipStream_t (None/0 = the null stream)."""
+        _check(lib().trn_ctx_set_stream(self.handle, _vp(stream_ptr) if stream_ptr else None))
+
+    def use_own_stream(self) -> None:
+        _check(lib().trn_ctx_use_own_stream(self.handle))
+
+    def synchronize(self) -> None:
+        _check(lib().trn_ctx_synchronize(self.handle))
+
+    def set_timing(self, on: bool) -> None:
+        _check(lib().trn_ctx_set_timing(self.handle, int(bool(on))))
+
+    def reset_timing(self) -> None:
+        _check(lib().trn_ctx_reset_timing(self.handle))
+
+    def timings(self) -> dict:
+        n = _i32()
+        _check(lib().trn_ctx_timing_count(self.handle, ctypes.byref(n)))
+        out = {}
+        for k in range(n.value):
+            name = ctypes.c_char_p()
+            ms = _f64()
+            cnt = _i64()
+            by = _f64()
+            _check(lib().trn_ctx_timing_get(self.handle, k, ctypes.byref(name), ctypes.byref(ms),
+                                            ctypes.byref(cnt), ctypes.byref(by)))
+            out[name.value.decode()] = {"ms": ms.value, "launches": cnt.value, "bytes": by.value}
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().trn_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceMesh:
+    """Device topology of a triangle mesh (trn_mesh): v2t/v2v CSR built on
+    the GPU with the semantics of Mesh::rebuildTopology."""
+
+    def __init__(self, ctx: Context, nv: int, tri, tags):
+        torch = _torch()
+        self.ctx = ctx
+        self.nv = int(nv)
+        tri_t = to_device(np.asarray(tri, dtype=np.int32).reshape(-1) if not isinstance(tri, torch.Tensor)
+                          else tri.reshape(-1), torch.int32)
+        tags_t = to_device(np.asarray(tags, dtype=np.uint8) if not isinstance(tags, torch.Tensor)
+                           else tags, torch.uint8)
+        self.nt = tri_t.numel() // 3
+        h = _vp()
+        _check(lib().trn_mesh_create(ctx.handle, self.nv, self.nt, _vp(tri_t.data_ptr()),
+                                     _vp(tags_t.data_ptr()), ctypes.byref(h)))
+        self.handle = h
+        nnz, mdt, mdv, mp = _i64(), _i32(), _i32(), _i32()
+        _check(lib().trn_mesh_info(h, ctypes.byref(nnz), ctypes.byref(mdt), ctypes.byref(mdv),
+                                   ctypes.byref(mp)))
+        self.nnz_v2v, self.max_deg_t, self.max_deg_v, self.max_patch = (
+            nnz.value, mdt.value, mdv.value, mp.value)
+
+    def csr(self) -> dict:
+        """Copies of the device CSR arrays as numpy (for tests)."""
+        a, b, c, d = _vp(), _vp(), _vp(), _vp()
+        _check(lib().trn_mesh_csr(self.handle, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                  ctypes.byref(d)))
+        self.ctx.synchronize()
+        v2t_off = _d2h_i32(a.value, self.nv + 1)
+        v2v_off = _d2h_i32(c.value, self.nv + 1)
+        return {"v2t_off": v2t_off, "v2t": _d2h_i32(b.value, int(v2t_off[-1])),
+                "v2v_off": v2v_off, "v2v": _d2h_i32(d.value, int(v2v_off[-1]))}
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().trn_mesh_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_hip = None
+
+
+def _d2h_i32(ptr: int, n: int) -> np.ndarray:
+    """Copies n int32 from library-owned device memory to a numpy array."""
+    global _hip
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.argtypes = [_vp, _vp, ctypes.c_size_t, ctypes.c_int]
+        _hip.hipMemcpy.restype = ctypes.c_int
+    out = np.empty(max(n, 0), dtype=np.int32)
+    if n > 0:
+        rc = _hip.hipMemcpy(out.ctypes.data_as(_vp), _vp(ptr), 4 * n, 2)  # DeviceToHost
+        if rc != 0:
+            raise TrinityError(TRN_ERR_HIP, f"hipMemcpy failed ({rc})")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# passes
+
+def metric_run(ctx: Context, mesh: DeviceMesh, xy, solut, target_factor=0.8, lp_norm=2,
+               h_min=1e-9, h_max=0.4, tensor=None, gradient=None, hessian=None,
+               want_complexity=True):
+    """Metrics::run on the device; returns (tensor, complexity or None)."""
+    torch = _torch()
+    if tensor is None:
+        tensor = torch.empty((mesh.nv, 3), dtype=torch.float64, device=xy.device)
+    c = _f64()
+    _check(lib().trn_metric_run(ctx.handle, mesh.handle, _vp(xy.data_ptr()),
+                                _vp(solut.data_ptr()), float(target_factor), int(lp_norm),
+                                float(h_min), float(h_max), _vp(tensor.data_ptr()),
+                                ctypes.byref(c) if want_complexity else None,
+                                _vp(_ptr(gradient)) if gradient is not None else None,
+                                _vp(_ptr(hessian)) if hessian is not None else None))
+    return tensor, (c.value if want_complexity else None)
+
+
+def gradient(ctx, mesh, xy, solut, out=None):
+    torch = _torch()
+    out = out if out is not None else torch.empty((mesh.nv, 2), dtype=torch.float64, device=xy.device)
+    _check(lib().trn_gradient(ctx.handle, mesh.handle, _vp(xy.data_ptr()), _vp(solut.data_ptr()),
+                              _vp(out.data_ptr())))
+    return out
+
+
+def hessian(ctx, mesh, xy, grad, out=None):
+    torch = _torch()
+    out = out if out is not None else torch.empty((mesh.nv, 3), dtype=torch.float64, device=xy.device)
+    _check(lib().trn_hessian(ctx.handle, mesh.handle, _vp(xy.data_ptr()), _vp(grad.data_ptr()),
+                             _vp(out.data_ptr())))
+    return out
+
+
+def normalize_local(ctx, tensor, lp_norm):
+    _check(lib().trn_normalize_local(ctx.handle, tensor.shape[0], _vp(tensor.data_ptr()),
+                                     int(lp_norm)))
+    return tensor
+
+
+def complexity(ctx, mesh, xy, tensor) -> float:
+    c = _f64()
+    _check(lib().trn_complexity(ctx.handle, mesh.handle, _vp(xy.data_ptr()),
+                                _vp(tensor.data_ptr()), ctypes.byref(c)))
+    return c.value
+
+
+def normalize_global(ctx, tensor, scale_fact, h_min=1e-9, h_max=0.4):
+    _check(lib().trn_normalize_global(ctx.handle, tensor.shape[0], _vp(tensor.data_ptr()),
+                                      float(scale_fact), float(h_min), float(h_max)))
+    return tensor
+
+
+class Coloring:
+    def __init__(self, color, order, class_off, ncolors):
+        self.color = color          # device int32 [nv]
+        self.order = order          # device int32 [nv]
+        self.class_off = class_off  # numpy int32 [ncolors+1]
+        self.ncolors = ncolors
+
+
+def color(ctx, mesh, seed=1, max_colors=64) -> Coloring:
+    """Partit::extractColoring on the device (seeded, deterministic)."""
+    torch = _torch()
+    dev = f"cuda:{ctx.device}"
+    col = torch.empty(mesh.nv, dtype=torch.int32, device=dev)
+    order = torch.empty(mesh.nv, dtype=torch.int32, device=dev)
+    off = (ctypes.c_int32 * 80)()
+    nc = _i32()
+    _check(lib().trn_color(ctx.handle, mesh.handle, int(seed) & ((1 << 64) - 1), int(max_colors),
+                           _vp(col.data_ptr()), _vp(order.data_ptr()), off, ctypes.byref(nc)))
+    return Coloring(col, order, np.array(off[:nc.value + 1], dtype=np.int32), nc.value)
+
+
+def color_classes(ctx, color_t, ncolors) -> Coloring:
+    torch = _torch()
+    order = torch.empty_like(color_t)
+    off = (ctypes.c_int32 * 80)()
+    _check(lib().trn_color_classes(ctx.handle, color_t.numel(), _vp(color_t.data_ptr()),
+                                   int(ncolors), _vp(order.data_ptr()), off))
+    return Coloring(color_t, order, np.array(off[:ncolors + 1], dtype=np.int32), int(ncolors))
+
+
+def color_check(ctx, mesh, color_t, ncolors) -> tuple[int, int]:
+    be, bv = _i64(), _i64()
+    _check(lib().trn_color_check(ctx.handle, mesh.handle, _vp(color_t.data_ptr()), int(ncolors),
+                                 ctypes.byref(be), ctypes.byref(bv)))
+    return be.value, bv.value
+
+
+def quality_cache(ctx, mesh, xy, tensor, out=None):
+    torch = _torch()
+    out = out if out is not None else torch.empty(mesh.nt, dtype=torch.float64, device=xy.device)
+    _check(lib().trn_quality_cache(ctx.handle, mesh.handle, _vp(xy.data_ptr()),
+                                   _vp(tensor.data_ptr()), _vp(out.data_ptr())))
+    return out
+
+
+def smooth_run(ctx, mesh, xy, tensor, qualit, coloring: Coloring, sweeps: int,
+               recompute_qualit=True, counters=True):
+    """Smooth::run (after colouring) on the device, in place on xy/tensor/
+    qualit.  Returns (commits, tasks) per sweep when counters=True."""
+    off = (ctypes.c_int32 * (coloring.ncolors + 1))(*[int(v) for v in coloring.class_off])
+    commits = (ctypes.c_int32 * max(sweeps, 1))()
+    tasks = (ctypes.c_int32 * max(sweeps, 1))()
+    _check(lib().trn_smooth_run(ctx.handle, mesh.handle, _vp(xy.data_ptr()),
+                                _vp(tensor.data_ptr()), _vp(qualit.data_ptr()),
+                                _vp(coloring.order.data_ptr()), off, int(coloring.ncolors),
+                                int(sweeps), int(bool(recompute_qualit)),
+                                commits if counters else None, tasks if counters else None))
+    if counters:
+        return list(commits[:sweeps]), list(tasks[:sweeps])
+    return None
+
+
+def debug_math(ctx, op: int, x, y=None):
+    torch = _torch()
+    out = torch.empty_like(x)
+    _check(lib().trn_debug_math(ctx.handle, int(op), x.numel(), _vp(x.data_ptr()),
+                                _vp(y.data_ptr()) if y is not None else None, _vp(out.data_ptr())))
+    return out

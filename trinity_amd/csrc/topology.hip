@@ -1,0 +1,431 @@
+// topology.hip -- device construction of the mesh adjacency, replacing the
+// topology part of Mesh::load / Mesh::rebuildTopology
+// (src/io.cpp:97-99,128-129; src/mesh.cpp:191-249):
+//   v2t : incident triangles per vertex, ascending (topo.stenc + sync.deg)
+//   v2v : sorted-unique 1-ring                     (topo.vicin)
+// plus the size of the largest 2-ring element patch (Mesh::getVicinity,
+// src/mesh.cpp:390-434), which sizes the LDS sets of the recovery gathers.
+//
+// Also hosts the device-wide exclusive scan and max-reduction helpers.
+#include <string>
+
+#include "trn_api.hpp"
+
+namespace trn {
+
+// ---------------------------------------------------------------------------
+// scan: reduce-then-scan over tiles of 256 threads x 8 items
+
+constexpr int kScanB = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanB * kScanItems;
+
+__device__ __forceinline__ int block_exclusive_scan(int v, int* lds, int& total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) lds[wid] = x;
+  __syncthreads();
+  int woff = 0;
+  total = 0;
+#pragma unroll
+  for (int w = 0; w < kScanB / 64; ++w) {
+    const int s = lds[w];
+    if (w < wid) woff += s;
+    total += s;
+  }
+  __syncthreads();
+  return woff + x - v;
+}
+
+__global__ __launch_bounds__(kScanB) void k_scan_reduce(const int32_t* __restrict__ in,
+                                                        int64_t n, int32_t* __restrict__ bsum) {
+  __shared__ int lds[kScanB / 64];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k)
+    if (base + k < n) s += in[base + k];
+  int total;
+  block_exclusive_scan(s, lds, total);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// single block: exclusive scan of nb block sums in place
+__global__ __launch_bounds__(kScanB) void k_scan_top(int32_t* bsum, int64_t nb) {
+  __shared__ int lds[kScanB / 64];
+  const int64_t per = (nb + kScanB - 1) / kScanB;
+  const int64_t b0 = threadIdx.x * per;
+  int s = 0;
+  for (int64_t k = 0; k < per; ++k)
+    if (b0 + k < nb) s += bsum[b0 + k];
+  int total;
+  int off = block_exclusive_scan(s, lds, total);
+  for (int64_t k = 0; k < per; ++k) {
+    if (b0 + k < nb) {
+      const int v = bsum[b0 + k];
+      bsum[b0 + k] = off;
+      off += v;
+    }
+  }
+  if (threadIdx.x == 0) bsum[nb] = total;
+}
+
+__global__ __launch_bounds__(kScanB) void k_scan_down(const int32_t* in, int32_t* out,
+                                                      int64_t n, const int32_t* __restrict__ bsum,
+                                                      int64_t nb, int with_total) {
+  __shared__ int lds[kScanB / 64];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  int v[kScanItems];
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = (base + k < n) ? in[base + k] : 0;
+    s += v[k];
+  }
+  int total;
+  int off = block_exclusive_scan(s, lds, total) + bsum[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < n) out[base + k] = off;
+    off += v[k];
+  }
+  if (with_total && blockIdx.x == 0 && threadIdx.x == 0) out[n] = bsum[nb];
+}
+
+size_t exclusive_scan_scratch(int64_t n) {
+  const int64_t nb = (n + kScanTile - 1) / kScanTile;
+  return scratch_bytes(sizeof(int32_t) * (size_t)(nb + 1));
+}
+
+void exclusive_scan_i32(trn_ctx* ctx, const int32_t* in, int32_t* out, int64_t n,
+                        bool with_total, int32_t* tmp) {
+  TRN_REQUIRE(tmp != nullptr, TRN_ERR_STATE, "scan scratch not set");
+  if (n <= 0) {
+    if (with_total) memset_async(ctx, out, 0, sizeof(int32_t));
+    return;
+  }
+  const int64_t nb = (n + kScanTile - 1) / kScanTile;
+  launch(ctx, "scan_reduce", 4.0 * n, k_scan_reduce, dim3((unsigned)nb), dim3(kScanB), 0,
+         in, n, tmp);
+  launch(ctx, "scan_top", 8.0 * nb, k_scan_top, dim3(1), dim3(kScanB), 0, tmp, nb);
+  launch(ctx, "scan_down", 8.0 * n, k_scan_down, dim3((unsigned)nb), dim3(kScanB), 0,
+         in, out, n, (const int32_t*)tmp, nb, with_total ? 1 : 0);
+}
+
+// ---------------------------------------------------------------------------
+// reductions
+
+// One atomic per block: many waves hitting a single address serialise in L2.
+__global__ __launch_bounds__(256) void k_reduce_max(const int32_t* __restrict__ in, int64_t n,
+                                                    int32_t* out) {
+  __shared__ int lds[4];
+  int m = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    m = max(m, in[i]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMax(out, max(max(lds[0], lds[1]), max(lds[2], lds[3])));
+}
+
+void reduce_max_i32(trn_ctx* ctx, const int32_t* in, int64_t n, int32_t* out_d) {
+  memset_async(ctx, out_d, 0, sizeof(int32_t));
+  if (n <= 0) return;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 512) blocks = 512;
+  launch(ctx, "reduce_max", 4.0 * n, k_reduce_max, dim3((unsigned)blocks), dim3(256), 0, in, n,
+         out_d);
+}
+
+// ---------------------------------------------------------------------------
+// topology kernels
+
+// Counts triangles with an index outside [0, nv) (deleted ones skipped).
+__global__ void k_validate_tri(int32_t nv, int32_t nt, const int32_t* __restrict__ tri,
+                               int32_t* bad) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const int32_t a = tri[3 * t], b = tri[3 * t + 1], c = tri[3 * t + 2];
+  if (a < 0) return;
+  if (a >= nv || b < 0 || b >= nv || c < 0 || c >= nv || a == b || b == c || a == c)
+    atomicAdd(bad, 1);
+}
+
+__global__ void k_count_v2t(int32_t nt, const int32_t* __restrict__ tri, int32_t* deg) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const int32_t a = tri[3 * t];
+  if (a < 0) return;  // deleted triangle (mesh.cpp:200)
+  atomicAdd(&deg[a], 1);
+  atomicAdd(&deg[tri[3 * t + 1]], 1);
+  atomicAdd(&deg[tri[3 * t + 2]], 1);
+}
+
+__global__ void k_fill_v2t(int32_t nt, const int32_t* __restrict__ tri, int32_t* cursor,
+                           int32_t* __restrict__ v2t) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const int32_t a = tri[3 * t];
+  if (a < 0) return;
+  v2t[atomicAdd(&cursor[a], 1)] = (int32_t)t;
+  v2t[atomicAdd(&cursor[tri[3 * t + 1]], 1)] = (int32_t)t;
+  v2t[atomicAdd(&cursor[tri[3 * t + 2]], 1)] = (int32_t)t;
+}
+
+// Ascending order of each vertex's incident list (mesh.cpp:244).
+__global__ void k_sort_lists(int32_t nv, const int32_t* __restrict__ off, int32_t* list) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const int32_t b = off[i], e = off[i + 1];
+  for (int32_t k = b + 1; k < e; ++k) {
+    const int32_t x = list[k];
+    int32_t j = k - 1;
+    while (j >= b && list[j] > x) {
+      list[j + 1] = list[j];
+      --j;
+    }
+    list[j + 1] = x;
+  }
+}
+
+// Sorted-unique 1-ring (mesh.cpp:214-243): pass 0 counts, pass 1 writes.
+template <int PASS>
+__global__ __launch_bounds__(64) void k_v2v(int32_t nv, const int32_t* __restrict__ tri,
+                                            const int32_t* __restrict__ v2t_off,
+                                            const int32_t* __restrict__ v2t, int cap,
+                                            int32_t* __restrict__ cnt,
+                                            const int32_t* __restrict__ v2v_off,
+                                            int32_t* __restrict__ v2v, int32_t* overflow) {
+  extern __shared__ int lds[];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  int* s = lds + threadIdx.x;
+  const int stride = blockDim.x;
+  int m = 0;
+  bool ok = true;
+  for (int32_t k = v2t_off[i]; k < v2t_off[i + 1]; ++k) {
+    const int32_t t = v2t[k];
+    const int32_t a = tri[3 * t], b = tri[3 * t + 1], c = tri[3 * t + 2];
+    int32_t u, w;
+    if (a == i) { u = b; w = c; }
+    else if (b == i) { u = c; w = a; }
+    else { u = a; w = b; }
+    ok = ok && lds_sorted_insert(s, stride, m, cap, u);
+    ok = ok && lds_sorted_insert(s, stride, m, cap, w);
+  }
+  if (!ok) {
+    atomicAdd(overflow, 1);
+    if (PASS == 0) cnt[i] = 0;
+    return;
+  }
+  if (PASS == 0) {
+    cnt[i] = m;
+  } else {
+    int32_t* out = v2v + v2v_off[i];
+    for (int k = 0; k < m; ++k) out[k] = s[k * stride];
+  }
+}
+
+// Upper bound of the 2-ring candidates: deg(i) + sum_{j in N(i)} deg(j).
+__global__ void k_patch_bound(int32_t nv, const int32_t* __restrict__ v2t_off,
+                              const int32_t* __restrict__ v2v_off,
+                              const int32_t* __restrict__ v2v, int32_t* __restrict__ bound) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  int s = v2t_off[i + 1] - v2t_off[i];
+  for (int32_t a = v2v_off[i]; a < v2v_off[i + 1]; ++a) {
+    const int32_t j = v2v[a];
+    s += v2t_off[j + 1] - v2t_off[j];
+  }
+  bound[i] = s;
+}
+
+// Exact 2-ring element patch size (getVicinity(i,2), mesh.cpp:390-434).
+__global__ __launch_bounds__(64) void k_patch_size(int32_t nv, const int32_t* __restrict__ v2t_off,
+                                                   const int32_t* __restrict__ v2t,
+                                                   const int32_t* __restrict__ v2v_off,
+                                                   const int32_t* __restrict__ v2v, int cap,
+                                                   int32_t* __restrict__ size,
+                                                   int32_t* overflow) {
+  extern __shared__ int lds[];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  int* s = lds + threadIdx.x;
+  const int stride = blockDim.x;
+  int m = 0;
+  bool ok = true;
+  for (int32_t k = v2t_off[i]; k < v2t_off[i + 1]; ++k)
+    ok = ok && lds_sorted_insert(s, stride, m, cap, v2t[k]);
+  for (int32_t a = v2v_off[i]; a < v2v_off[i + 1] && ok; ++a) {
+    const int32_t j = v2v[a];
+    for (int32_t k = v2t_off[j]; k < v2t_off[j + 1]; ++k)
+      ok = ok && lds_sorted_insert(s, stride, m, cap, v2t[k]);
+  }
+  if (!ok) atomicAdd(overflow, 1);
+  size[i] = m;
+}
+
+static unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace trn
+
+using namespace trn;
+
+extern "C" {
+
+int trn_mesh_create(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
+                    const uint8_t* tags, trn_mesh** out) {
+  return guard_impl("trn_mesh_create", [&] {
+    DeviceScope ds(ctx);
+    TRN_REQUIRE(out != nullptr, TRN_ERR_ARG, "null output");
+    *out = nullptr;
+    TRN_REQUIRE(nv > 0 && nt > 0, TRN_ERR_ARG, "nv and nt must be positive");
+    TRN_REQUIRE(tri != nullptr && tags != nullptr, TRN_ERR_ARG, "null tri or tags");
+    TRN_REQUIRE((int64_t)nt * 3 < (int64_t)INT32_MAX, TRN_ERR_ARG, "too many triangles");
+    trn_mesh* m = new trn_mesh();
+    m->ctx = ctx;
+    m->nv = nv;
+    m->nt = nt;
+    auto cleanup = [&]() { trn_mesh_destroy(m); };
+    try {
+      TRN_HIP(hipMalloc(&m->tri, sizeof(int32_t) * 3 * (size_t)nt));
+      TRN_HIP(hipMalloc(&m->tags, (size_t)nv));
+      TRN_HIP(hipMalloc(&m->v2t_off, sizeof(int32_t) * ((size_t)nv + 1)));
+      TRN_HIP(hipMalloc(&m->v2t, sizeof(int32_t) * 3 * (size_t)nt));
+      TRN_HIP(hipMalloc(&m->v2v_off, sizeof(int32_t) * ((size_t)nv + 1)));
+      copy_async(ctx, m->tri, tri, sizeof(int32_t) * 3 * (size_t)nt);
+      copy_async(ctx, m->tags, tags, (size_t)nv);
+
+      Scratch sc(ctx, 4 * scratch_bytes(sizeof(int32_t) * ((size_t)nv + 1)) +
+                          exclusive_scan_scratch(nv) + 256 * 4);
+      int32_t* cnt = sc.carve<int32_t>((size_t)nv + 1);
+      int32_t* cursor = sc.carve<int32_t>((size_t)nv + 1);
+      int32_t* cnt2 = sc.carve<int32_t>((size_t)nv + 1);
+      int32_t* bound = sc.carve<int32_t>((size_t)nv + 1);
+      int32_t* flags = sc.carve<int32_t>(64);
+      int32_t* scan_tmp = sc.carve<int32_t>(exclusive_scan_scratch(nv) / 4);
+      int32_t* h = ctx->pinned_i32;
+
+      // 1. validate indices (an index out of range would make the atomics
+      //    below write outside their arrays)
+      memset_async(ctx, flags, 0, 64 * sizeof(int32_t));
+      launch(ctx, "topo_validate", 12.0 * nt, k_validate_tri, dim3(blocks_for(nt, 256)),
+             dim3(256), 0, nv, nt, (const int32_t*)m->tri, flags);
+      copy_async(ctx, h, flags, sizeof(int32_t));
+      sync(ctx);
+      TRN_REQUIRE(h[0] == 0, TRN_ERR_ARG,
+                  std::to_string(h[0]) + " triangles have an index outside [0, nv) or repeat a vertex");
+
+      // 2. v2t CSR
+      memset_async(ctx, cnt, 0, sizeof(int32_t) * (size_t)nv);
+      launch(ctx, "topo_count_v2t", 24.0 * nt, k_count_v2t, dim3(blocks_for(nt, 256)), dim3(256),
+             0, nt, (const int32_t*)m->tri, cnt);
+      exclusive_scan_i32(ctx, cnt, m->v2t_off, nv, true, scan_tmp);
+      copy_async(ctx, cursor, m->v2t_off, sizeof(int32_t) * (size_t)nv);
+      launch(ctx, "topo_fill_v2t", 36.0 * nt, k_fill_v2t, dim3(blocks_for(nt, 256)), dim3(256), 0,
+             nt, (const int32_t*)m->tri, cursor, m->v2t);
+      launch(ctx, "topo_sort_v2t", 8.0 * nv + 24.0 * nt, k_sort_lists, dim3(blocks_for(nv, 256)),
+             dim3(256), 0, nv, (const int32_t*)m->v2t_off, m->v2t);
+      reduce_max_i32(ctx, cnt, nv, flags + 1);
+      copy_async(ctx, h, flags + 1, sizeof(int32_t));
+      sync(ctx);
+      m->max_deg_t = h[0];
+      TRN_REQUIRE(m->max_deg_t <= kMaxDegree, TRN_ERR_CAPACITY,
+                  "a vertex has " + std::to_string(m->max_deg_t) +
+                      " incident triangles; at most " + std::to_string(kMaxDegree) +
+                      " are supported");
+
+      // 3. v2v CSR (count, scan, fill)
+      const int capv = 2 * (m->max_deg_t > 0 ? m->max_deg_t : 1);
+      const size_t shv = sizeof(int) * 64 * (size_t)capv;
+      memset_async(ctx, flags + 2, 0, sizeof(int32_t));
+      launch(ctx, "topo_count_v2v", 0.0, k_v2v<0>, dim3(blocks_for(nv, 64)), dim3(64), shv, nv,
+             (const int32_t*)m->tri, (const int32_t*)m->v2t_off, (const int32_t*)m->v2t, capv,
+             cnt2, (const int32_t*)nullptr, (int32_t*)nullptr, flags + 2);
+      exclusive_scan_i32(ctx, cnt2, m->v2v_off, nv, true, scan_tmp);
+      reduce_max_i32(ctx, cnt2, nv, flags + 3);
+      copy_async(ctx, h, flags + 2, 2 * sizeof(int32_t));
+      copy_async(ctx, h + 2, m->v2v_off + nv, sizeof(int32_t));
+      sync(ctx);
+      TRN_REQUIRE(h[0] == 0, TRN_ERR_STATE, "1-ring overflow");
+      m->max_deg_v = h[1];
+      m->nnz_v2v = h[2];
+      TRN_HIP(hipMalloc(&m->v2v, sizeof(int32_t) * (size_t)(m->nnz_v2v > 0 ? m->nnz_v2v : 1)));
+      launch(ctx, "topo_fill_v2v", 0.0, k_v2v<1>, dim3(blocks_for(nv, 64)), dim3(64), shv, nv,
+             (const int32_t*)m->tri, (const int32_t*)m->v2t_off, (const int32_t*)m->v2t, capv,
+             (int32_t*)nullptr, (const int32_t*)m->v2v_off, m->v2v, flags + 2);
+
+      // 4. largest 2-ring element patch
+      launch(ctx, "topo_patch_bound", 0.0, k_patch_bound, dim3(blocks_for(nv, 256)), dim3(256),
+             0, nv, (const int32_t*)m->v2t_off, (const int32_t*)m->v2v_off,
+             (const int32_t*)m->v2v, bound);
+      reduce_max_i32(ctx, bound, nv, flags + 4);
+      copy_async(ctx, h, flags + 4, sizeof(int32_t));
+      sync(ctx);
+      int capp = h[0] < kMaxPatchCap ? h[0] : kMaxPatchCap;
+      if (capp < 1) capp = 1;
+      memset_async(ctx, flags + 5, 0, sizeof(int32_t));
+      launch(ctx, "topo_patch_size", 0.0, k_patch_size, dim3(blocks_for(nv, 64)), dim3(64),
+             sizeof(int) * 64 * (size_t)capp, nv, (const int32_t*)m->v2t_off,
+             (const int32_t*)m->v2t, (const int32_t*)m->v2v_off, (const int32_t*)m->v2v, capp,
+             bound, flags + 5);
+      reduce_max_i32(ctx, bound, nv, flags + 6);
+      copy_async(ctx, h, flags + 5, 2 * sizeof(int32_t));
+      sync(ctx);
+      TRN_REQUIRE(h[0] == 0, TRN_ERR_CAPACITY,
+                  "a 2-ring element patch exceeds " + std::to_string(kMaxPatchCap) + " triangles");
+      m->max_patch = h[1];
+      m->patch_cap = m->max_patch > 0 ? m->max_patch : 1;
+    } catch (...) {
+      cleanup();
+      throw;
+    }
+    *out = m;
+  });
+}
+
+int trn_mesh_destroy(trn_mesh* m) {
+  if (!m) return TRN_OK;
+  return guard_impl("trn_mesh_destroy", [&] {
+    DeviceScope ds(m->ctx);
+    (void)hipStreamSynchronize(m->ctx->stream);
+    if (m->tri) (void)hipFree(m->tri);
+    if (m->tags) (void)hipFree(m->tags);
+    if (m->v2t_off) (void)hipFree(m->v2t_off);
+    if (m->v2t) (void)hipFree(m->v2t);
+    if (m->v2v_off) (void)hipFree(m->v2v_off);
+    if (m->v2v) (void)hipFree(m->v2v);
+    delete m;
+  });
+}
+
+int trn_mesh_info(const trn_mesh* m, int64_t* nnz_v2v, int32_t* max_deg_t, int32_t* max_deg_v,
+                  int32_t* max_patch) {
+  return guard_impl("trn_mesh_info", [&] {
+    TRN_REQUIRE(m != nullptr, TRN_ERR_ARG, "null mesh");
+    if (nnz_v2v) *nnz_v2v = m->nnz_v2v;
+    if (max_deg_t) *max_deg_t = m->max_deg_t;
+    if (max_deg_v) *max_deg_v = m->max_deg_v;
+    if (max_patch) *max_patch = m->max_patch;
+  });
+}
+
+int trn_mesh_csr(const trn_mesh* m, const int32_t** v2t_off_d, const int32_t** v2t_d,
+                 const int32_t** v2v_off_d, const int32_t** v2v_d) {
+  return guard_impl("trn_mesh_csr", [&] {
+    TRN_REQUIRE(m != nullptr, TRN_ERR_ARG, "null mesh");
+    if (v2t_off_d) *v2t_off_d = m->v2t_off;
+    if (v2t_d) *v2t_d = m->v2t;
+    if (v2v_off_d) *v2v_off_d = m->v2v_off;
+    if (v2v_d) *v2v_d = m->v2v;
+  });
+}
+
+}  // extern "C"

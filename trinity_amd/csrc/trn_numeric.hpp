@@ -1,0 +1,206 @@
+// trn_numeric.hpp -- FP64 tensor numerics of the metric-and-smoothing path,
+// as __device__ functions for gfx950.
+//
+// Every routine reproduces the operation order of the reference routine it
+// cites, so that, with -ffp-contract=off, results are bit-identical wherever
+// the same libm values are used (sqrt and division are IEEE-exact on both
+// sides; log/exp/pow come from the device math library).
+//
+// std::min(a,b) == (b < a ? b : a) and std::max(a,b) == (a < b ? b : a) are
+// restated literally so that NaN and signed-zero behaviour match.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace trn {
+
+constexpr double kEps = 1.e-9;   // include/trinity/header.h:50
+
+__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
+
+// numeric::eigenDecompose, src/numeric.cpp:25-95.  vec[0..1] is vec1 and
+// vec[2..3] is vec2 (rows of the eigenvector matrix; column k belongs to
+// val[k]).
+__device__ __forceinline__ void eigen(const double m0, const double m1,
+                                      const double m2, const double m3,
+                                      double val[2], double vec[4]) {
+  if (m1 * m2 < kEps) {                    // numeric.cpp:30-38 (absolute)
+    val[0] = m3;
+    val[1] = m0;
+    vec[0] = 0; vec[2] = 1;
+    vec[1] = 1; vec[3] = 0;
+    return;
+  }
+  const double trace = m0 + m3;            // numeric.cpp:44-50
+  const double det = m0 * m3 - m1 * m2;
+  double term = trace * 0.5;
+  const double sqrt_delta = __builtin_sqrt(term * term - det);
+  val[0] = term - sqrt_delta;
+  val[1] = term + sqrt_delta;
+  if (!__builtin_isnormal(val[0]) || !__builtin_isnormal(val[1]))
+    val[0] = val[1];                       // numeric.cpp:52-64
+  const double dd = m0 - m3;               // numeric.cpp:69-71, pow(.,2)=x*x
+  term = __builtin_sqrt(dmax(0.25 * (dd * dd) + m1 * m2, 0.));
+  if (m0 - m3 >= 0) {                      // numeric.cpp:72-82
+    vec[0] = +0.5 * (m0 - m3) - term;
+    vec[1] = m1;
+    vec[2] = m2;
+    vec[3] = -0.5 * (m0 - m3) + term;
+  } else {
+    vec[0] = m2;
+    vec[1] = +0.5 * (m0 - m3) + term;
+    vec[2] = -0.5 * (m0 - m3) - term;
+    vec[3] = m1;
+  }
+  const double n0 = __builtin_sqrt(vec[0] * vec[0] + vec[1] * vec[1]);
+  const double n1 = __builtin_sqrt(vec[2] * vec[2] + vec[3] * vec[3]);
+  vec[0] /= n0; vec[2] /= n1;              // numeric.cpp:91-94
+  vec[1] /= n0; vec[3] /= n1;
+}
+
+// P.diag(val).P^T with the products of metric.cpp:134-142 (also
+// metric.cpp:222-231 and numeric.cpp:146-154, :203-211).
+__device__ __forceinline__ void rebuild(const double val[2], const double vec[4],
+                                        double h[4]) {
+  const double c0 = val[0] * vec[0];
+  const double c1 = val[0] * vec[2];
+  const double c2 = val[1] * vec[1];
+  const double c3 = val[1] * vec[3];
+  h[0] = vec[0] * c0 + vec[1] * c2;
+  h[1] = vec[0] * c1 + vec[1] * c3;
+  h[2] = vec[2] * c0 + vec[3] * c2;
+  h[3] = vec[2] * c1 + vec[3] * c3;
+}
+
+// Matrix logarithm term of numeric::interpolateTensor for one tensor
+// (numeric.cpp:115-154): eigen, log of both eigenvalues, rebuild.  The four
+// entries are kept separately because cache[1] and cache[2] round
+// differently and both feed the later eigen decomposition.
+__device__ __forceinline__ void tensor_log(const double t0, const double t1,
+                                           const double t2, double out[4]) {
+  double val[2], vec[4];
+  eigen(t0, t1, t1, t2, val, vec);
+  val[0] = log(val[0]);
+  val[1] = log(val[1]);
+  rebuild(val, vec, out);
+}
+
+// Exponential step of numeric::interpolateTensor (numeric.cpp:192-216):
+// divide the accumulated log by nb, eigen, exp, rebuild.
+__device__ __forceinline__ void tensor_exp_mean(double lm[4], const int nb,
+                                                double r[3]) {
+  lm[0] /= nb; lm[1] /= nb; lm[2] /= nb; lm[3] /= nb;
+  double val[2], vec[4], c[4];
+  eigen(lm[0], lm[1], lm[2], lm[3], val, vec);
+  val[0] = exp(val[0]);
+  val[1] = exp(val[1]);
+  rebuild(val, vec, c);
+  r[0] = c[0];
+  r[1] = c[1];
+  r[2] = c[3];
+}
+
+// numeric::interpolateTensor of three tensors (numeric.cpp:98-248).
+__device__ __forceinline__ void interpolate3(const double* ta, const double* tb,
+                                             const double* tc, double r[3]) {
+  double lm[4] = {0., 0., 0., 0.}, c[4];
+  tensor_log(ta[0], ta[1], ta[2], c);
+  lm[0] += c[0]; lm[1] += c[1]; lm[2] += c[2]; lm[3] += c[3];
+  tensor_log(tb[0], tb[1], tb[2], c);
+  lm[0] += c[0]; lm[1] += c[1]; lm[2] += c[2]; lm[3] += c[3];
+  tensor_log(tc[0], tc[1], tc[2], c);
+  lm[0] += c[0]; lm[1] += c[1]; lm[2] += c[2]; lm[3] += c[3];
+  tensor_exp_mean(lm, 3, r);
+}
+
+// Metric-space quality from the averaged tensor j (numeric.cpp:275-305).
+__device__ __forceinline__ double quality_from_mean(const double* pa,
+                                                    const double* pb,
+                                                    const double* pc,
+                                                    const double j[3]) {
+  const double det = j[0] * j[2] - j[1] * j[1];
+  double coef[6];
+  coef[0] = pb[0] - pa[0];
+  coef[1] = pb[1] - pa[1];
+  coef[2] = pc[0] - pb[0];
+  coef[3] = pc[1] - pb[1];
+  coef[4] = pa[0] - pc[0];
+  coef[5] = pa[1] - pc[1];
+  double perim = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int i = k * 2, l = i + 1;
+    const double v0 = j[0] * coef[i] + j[1] * coef[l];
+    const double v1 = j[1] * coef[i] + j[2] * coef[l];
+    perim += __builtin_sqrt(coef[i] * v0 + coef[l] * v1);
+  }
+  const double a0 = pb[0] - pa[0];           // numeric.cpp:296-300
+  const double a1 = pc[0] - pa[0];
+  const double a2 = pb[1] - pa[1];
+  const double a3 = pc[1] - pa[1];
+  const double area = 0.5 * fabs(a0 * a3 - a1 * a2);
+  const double mean = dmin(perim / 3, 3 / perim);
+  const double norm = (12 * 1.7320508075688772) * pow(mean * (2. - mean), 3.);
+  return area * norm * __builtin_sqrt(det) / (perim * perim);
+}
+
+// numeric::computeQuality, src/numeric.cpp:251-306.
+__device__ __forceinline__ double quality(const double* pa, const double* pb,
+                                          const double* pc, const double* ta,
+                                          const double* tb, const double* tc) {
+  double j[3];
+  interpolate3(ta, tb, tc, j);
+  return quality_from_mean(pa, pb, pc, j);
+}
+
+// numeric::approxRiemannDist (4-argument form), src/numeric.cpp:469-497.
+__device__ __forceinline__ double riemann_dist(const double* pa, const double* pb,
+                                               const double* ta, const double* tb) {
+  const double v0 = pb[0] - pa[0];
+  const double v1 = pb[1] - pa[1];
+  const double m0 = ta[0] * v0 + ta[1] * v1;
+  const double m1 = ta[1] * v0 + ta[2] * v1;
+  const double m2 = tb[0] * v0 + tb[1] * v1;
+  const double m3 = tb[1] * v0 + tb[2] * v1;
+  const double l0 = __builtin_sqrt(v0 * m0 + v1 * m1);
+  const double l1 = __builtin_sqrt(v0 * m2 + v1 * m3);
+  if (fabs(l0 - l1) < kEps) return l0;
+  const double lmin = dmin(l0, l1);
+  const double lmax = dmax(l0, l1);
+  const double ratio = lmax / lmin;
+  return lmax * (ratio - 1) / (ratio * log(ratio));
+}
+
+// Mesh::isCounterclockwise, src/mesh.cpp:700-713 (absolute threshold).
+__device__ __forceinline__ bool ccw(const double* pa, const double* pb,
+                                    const double* pc) {
+  const double c0 = pb[0] - pa[0];
+  const double c1 = pc[0] - pa[0];
+  const double c2 = pb[1] - pa[1];
+  const double c3 = pc[1] - pa[1];
+  return 0.5 * (c0 * c3 - c1 * c2) > kEps;
+}
+
+// Per-triangle geometry of the L2 projection, src/hessian.cpp:42-61 (the
+// same code appears at :108-128): area and the six basis-function
+// derivatives with norm = 2/area.
+__device__ __forceinline__ void tri_geometry(const double2 p0, const double2 p1,
+                                             const double2 p2, double& area,
+                                             double d[6]) {
+  const double c0 = p1.x - p0.x;
+  const double c1 = p2.x - p0.x;
+  const double c2 = p1.y - p0.y;
+  const double c3 = p2.y - p0.y;
+  area = 0.5 * (c0 * c3 - c1 * c2);
+  const double norm = 2 / area;
+  d[0] = norm * (p1.y - p2.y);
+  d[1] = norm * (p2.x - p1.x);
+  d[2] = norm * (p2.y - p0.y);
+  d[3] = norm * (p0.x - p2.x);
+  d[4] = norm * (p0.y - p1.y);
+  d[5] = norm * (p1.x - p0.x);
+}
+
+}  // namespace trn

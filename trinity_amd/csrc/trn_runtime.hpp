@@ -1,0 +1,152 @@
+// trn_runtime.hpp -- context, error handling, launch timing and scratch
+// management shared by every translation unit of libtrinity_amd.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/trinity_amd.h"
+
+namespace trn {
+
+// Internal errors travel as exceptions and are turned into status codes at
+// the C boundary (trn_guard); nothing throws across the ABI.
+struct Error {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& msg);
+void set_last_error(const std::string& msg);
+
+#define TRN_HIP(expr)                                                        \
+  do {                                                                       \
+    hipError_t _e = (expr);                                                  \
+    if (_e != hipSuccess)                                                    \
+      ::trn::fail(TRN_ERR_HIP, std::string(#expr) + ": " +                   \
+                                   hipGetErrorString(_e));                   \
+  } while (0)
+
+#define TRN_REQUIRE(cond, code, msg)                                         \
+  do {                                                                       \
+    if (!(cond)) ::trn::fail((code), (msg));                                 \
+  } while (0)
+
+// Per-kernel timing (opt-in): hipEvents bracket each launch on the stream
+// the kernel runs on; durations are folded into per-name totals when the
+// caller asks for them.
+struct KernelStat {
+  std::string name;
+  double total_ms = 0.0;
+  int64_t launches = 0;
+  double bytes = 0.0;  // algorithmic bytes per launch x launches (if known)
+};
+
+struct PendingTiming {
+  int stat;
+  hipEvent_t start, stop;
+};
+
+// Workspace: one growing device buffer, carved into aligned slices per call.
+struct Workspace {
+  char* base = nullptr;
+  size_t size = 0;
+};
+
+}  // namespace trn
+
+struct trn_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;      // stream every launch goes to
+  hipStream_t own_stream = nullptr;  // the context's own (default priority)
+  trn::Workspace ws;
+  bool timing = false;
+  std::vector<trn::KernelStat> stats;
+  std::vector<trn::PendingTiming> pending;
+  std::vector<hipEvent_t> event_pool;
+  int32_t* pinned_i32 = nullptr;     // small pinned host staging buffers
+  double* pinned_f64 = nullptr;
+};
+
+struct trn_mesh {
+  trn_ctx* ctx = nullptr;
+  int32_t nv = 0, nt = 0;
+  int64_t nnz_v2v = 0;
+  int32_t max_deg_t = 0;     // max incident triangles per vertex
+  int32_t max_deg_v = 0;     // max 1-ring size
+  int32_t max_patch = 0;     // max 2-ring element patch size
+  int32_t patch_cap = 0;     // LDS slots per thread used by the gathers
+  int32_t* tri = nullptr;    // [3nt]
+  uint8_t* tags = nullptr;   // [nv]
+  int32_t* v2t_off = nullptr;
+  int32_t* v2t = nullptr;
+  int32_t* v2v_off = nullptr;
+  int32_t* v2v = nullptr;
+};
+
+namespace trn {
+
+// Launch helpers ---------------------------------------------------------
+int stat_index(trn_ctx* ctx, const char* name);
+void timing_begin(trn_ctx* ctx, int stat, hipEvent_t* start, hipEvent_t* stop);
+void timing_end(trn_ctx* ctx, int stat, hipEvent_t start, hipEvent_t stop,
+                double bytes);
+
+// Launches `kernel` on ctx->stream; when timing is on, brackets it with
+// events and credits `bytes` algorithmic bytes to the named statistic.
+template <typename K, typename... Args>
+inline void launch(trn_ctx* ctx, const char* name, double bytes, K kernel,
+                   dim3 grid, dim3 block, size_t shmem, Args... args) {
+  if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int st = -1;
+  if (ctx->timing) {
+    st = stat_index(ctx, name);
+    timing_begin(ctx, st, &e0, &e1);
+  }
+  hipLaunchKernelGGL(kernel, grid, block, shmem, ctx->stream, args...);
+  TRN_HIP(hipGetLastError());
+  if (ctx->timing) timing_end(ctx, st, e0, e1, bytes);
+}
+
+// Scratch slices from the context workspace.  An entry point sizes all of
+// its scratch up front (Scratch(ctx, total)), which grows the workspace if
+// needed, then carves slices in order.  Slices live until the next entry
+// point on the same context; every entry point is stream-ordered on
+// ctx->stream, so reuse across calls is safe.
+inline size_t scratch_bytes(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+
+class Scratch {
+ public:
+  Scratch(trn_ctx* ctx, size_t total);
+  template <typename T>
+  T* carve(size_t count) {
+    size_t bytes = scratch_bytes(count * sizeof(T));
+    TRN_REQUIRE(off_ + bytes <= ctx_->ws.size, TRN_ERR_STATE, "scratch overflow");
+    T* p = reinterpret_cast<T*>(ctx_->ws.base + off_);
+    off_ += bytes;
+    return p;
+  }
+
+ private:
+  trn_ctx* ctx_;
+  size_t off_ = 0;
+};
+
+void sync(trn_ctx* ctx);
+void memset_async(trn_ctx* ctx, void* p, int v, size_t bytes);
+
+// XCD-aware block index: the hardware hands workgroups to the 8 XCDs round
+// robin (block b -> XCD b % 8).  Remapping gives each XCD one contiguous
+// range of logical blocks, so neighbouring vertex blocks, which share
+// stencil data, run on the same XCD and hit the same L2.
+__device__ __forceinline__ uint32_t xcd_block() {
+  const uint32_t n = gridDim.x, b = blockIdx.x;
+  const uint32_t q = n >> 3, r = n & 7u, x = b & 7u, k = b >> 3;
+  return x * q + (x < r ? x : r) + k;
+}
+
+}  // namespace trn
