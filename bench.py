@@ -110,6 +110,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--color-seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--locality-order", action="store_true",
+                    help="visit colour classes along a Z-curve of the coordinates")
     ap.add_argument("--cpu-sample-n", type=int, default=1024)
     ap.add_argument("--cpu-reps", type=int, default=3)
     args = ap.parse_args()
@@ -127,8 +129,8 @@ def main():
     log(f"building workload {args.workload} ...")
     m = mg.workload(args.workload, seed=args.seed)
     ctx = T.Context(0)
-    dm = T.DeviceMesh(ctx, m.nv, m.tri, m.tags)
     xy0 = torch.from_numpy(m.xy).cuda()
+    dm = T.DeviceMesh(ctx, m.nv, m.tri, m.tags, xy=xy0 if args.locality_order else None)
     f = torch.from_numpy(m.solut).cuda()
     xy = torch.empty_like(xy0)
     tensor = torch.empty((m.nv, 3), dtype=torch.float64, device="cuda")

@@ -77,9 +77,12 @@ int trn_ctx_timing_get(trn_ctx* ctx, int32_t index, const char** name,
  * tri (3nt, 0-based; a negative first index marks a deleted triangle as in
  * mesh.cpp:200) and tags (nv) may be host or device pointers; they are
  * copied into the mesh.  Also measures the largest 2-ring element patch
- * (Mesh::getVicinity(i,2), src/mesh.cpp:390-434) the recovery will gather. */
+ * (Mesh::getVicinity(i,2), src/mesh.cpp:390-434) the recovery will gather.
+ * xy_d (device, optional): coordinates used only to order colour classes
+ * along a Z-curve for memory locality; NULL keeps index order.  Results do
+ * not depend on it. */
 int trn_mesh_create(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
-                    const uint8_t* tags, trn_mesh** out);
+                    const uint8_t* tags, const double* xy_d, trn_mesh** out);
 int trn_mesh_destroy(trn_mesh* mesh);
 /* Sizes: nnz of v2v, max triangles per vertex, max 1-ring, max 2-ring patch. */
 int trn_mesh_info(const trn_mesh* mesh, int64_t* nnz_v2v, int32_t* max_deg_t,
@@ -129,16 +132,20 @@ int trn_normalize_global(trn_ctx* ctx, int32_t nv, double* tensor_d,
  * Partit(mesh, 8), i.e. at most 7).  Returns TRN_ERR_PALETTE if a vertex
  * cannot be coloured within max_colors (<= 64).  Also emits the colour
  * classes (Partit::task.subset / cardin / nb.parts): order_d [nv] lists the
- * coloured vertices grouped by colour, ascending within a colour;
+ * coloured vertices grouped by colour (within a colour in the mesh's
+ * locality order, else ascending);
  * class_off (host, max_colors+2 entries) receives the offsets of colours
  * 1..ncolors in order_d.  Synchronises. */
 int trn_color(trn_ctx* ctx, const trn_mesh* mesh, uint64_t seed,
               int32_t max_colors, int32_t* color_d, int32_t* order_d,
               int32_t* class_off, int32_t* ncolors);
 /* Colour classes of an externally supplied colouring (colour 0 = not
- * coloured), e.g. the reference's own.  class_off as above.  Synchronises. */
-int trn_color_classes(trn_ctx* ctx, int32_t nv, const int32_t* color_d,
-                      int32_t ncolors, int32_t* order_d, int32_t* class_off);
+ * coloured), e.g. the reference's own.  class_off as above.  mesh may be
+ * NULL (classes in index order) or the mesh whose locality order to use.
+ * Synchronises. */
+int trn_color_classes(trn_ctx* ctx, const trn_mesh* mesh, int32_t nv,
+                      const int32_t* color_d, int32_t ncolors, int32_t* order_d,
+                      int32_t* class_off);
 /* Counts edges whose end points share a colour and non-isolated vertices
  * left uncoloured or outside 1..ncolors (0 and 0 = valid).  Synchronises. */
 int trn_color_check(trn_ctx* ctx, const trn_mesh* mesh, const int32_t* color_d,

@@ -130,7 +130,8 @@ def test_colouring_valid_deterministic_seeded(ctx, wl):
         assert T().color_check(ctx, dm, col.color, col.ncolors) == (0, 0)
         assert oracle.color_valid(topo, col.color.cpu().numpy(), col.ncolors)
         assert 1 <= col.ncolors <= 7
-        # classes: colour c lists its vertices in ascending order
+        # classes: colour c lists its vertices in ascending order (no
+        # coordinates were given, so no locality order)
         order = col.order.cpu().numpy()
         cv = col.color.cpu().numpy()
         off = col.class_off
@@ -138,6 +139,21 @@ def test_colouring_valid_deterministic_seeded(ctx, wl):
         for k in range(col.ncolors):
             seg = order[off[k]:off[k + 1]]
             assert (cv[seg] == k + 1).all() and (np.diff(seg) > 0).all()
+
+
+def test_locality_order_is_a_permutation_of_each_class(ctx):
+    m = mg.workload("jitter90:fronts")
+    plain = T().DeviceMesh(ctx, m.nv, m.tri, m.tags)
+    local = T().DeviceMesh(ctx, m.nv, m.tri, m.tags, xy=m.xy)
+    a = T().color(ctx, plain, seed=5)
+    b = T().color(ctx, local, seed=5)
+    assert np.array_equal(a.color.cpu().numpy(), b.color.cpu().numpy())
+    assert np.array_equal(a.class_off, b.class_off)
+    oa, ob = a.order.cpu().numpy(), b.order.cpu().numpy()
+    assert not np.array_equal(oa, ob)
+    for k in range(a.ncolors):
+        s = slice(a.class_off[k], a.class_off[k + 1])
+        assert np.array_equal(np.sort(ob[s]), oa[s])
 
 
 def test_colouring_reports_palette_overflow(ctx):
@@ -168,7 +184,7 @@ def test_invalid_triangles_are_rejected(ctx):
 
 def _smooth_gpu(ctx, dm, xy0, metric, color_np, ncolors, sweeps):
     import torch
-    col = T().color_classes(ctx, dev(color_np, torch.int32), ncolors)
+    col = T().color_classes(ctx, dev(color_np, torch.int32), ncolors, mesh=dm)
     xy = dev(xy0)
     ten = dev(metric)
     q = torch.empty(dm.nt, dtype=torch.float64, device="cuda")
@@ -222,7 +238,8 @@ def test_quality_cache_matches_oracle(ctx):
 def test_smoothing_with_gpu_colouring_matches_oracle(ctx, wl, p, sweeps):
     m = mg.workload(wl)
     st = oracle.metric_stages(m.xy, m.tri, m.solut, 0.8, p)
-    dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags)
+    # with coordinates: classes are visited in Z-curve order (same results)
+    dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags, xy=m.xy)
     col = T().color(ctx, dm, seed=11, max_colors=8)
     cnp = col.color.cpu().numpy()
     xo, to, qo, co, tao = oracle.smooth_run(m.xy, m.tri, m.tags, st["metric"], cnp, col.ncolors,

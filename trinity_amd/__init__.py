@@ -77,7 +77,7 @@ def _load():
         "trn_ctx_timing_count": ([_vp, _P(_i32)], ctypes.c_int),
         "trn_ctx_timing_get": ([_vp, _i32, _P(ctypes.c_char_p), _P(_f64), _P(_i64), _P(_f64)],
                                ctypes.c_int),
-        "trn_mesh_create": ([_vp, _i32, _i32, _vp, _vp, _P(_vp)], ctypes.c_int),
+        "trn_mesh_create": ([_vp, _i32, _i32, _vp, _vp, _vp, _P(_vp)], ctypes.c_int),
         "trn_mesh_destroy": ([_vp], ctypes.c_int),
         "trn_mesh_info": ([_vp, _P(_i64), _P(_i32), _P(_i32), _P(_i32)], ctypes.c_int),
         "trn_mesh_csr": ([_vp, _P(_vp), _P(_vp), _P(_vp), _P(_vp)], ctypes.c_int),
@@ -90,7 +90,7 @@ def _load():
         "trn_normalize_global": ([_vp, _i32, _vp, _f64, _f64, _f64], ctypes.c_int),
         "trn_color": ([_vp, _vp, ctypes.c_uint64, _i32, _vp, _vp, _P(_i32), _P(_i32)],
                       ctypes.c_int),
-        "trn_color_classes": ([_vp, _i32, _vp, _i32, _vp, _P(_i32)], ctypes.c_int),
+        "trn_color_classes": ([_vp, _vp, _i32, _vp, _i32, _vp, _P(_i32)], ctypes.c_int),
         "trn_color_check": ([_vp, _vp, _vp, _i32, _P(_i64), _P(_i64)], ctypes.c_int),
         "trn_quality_cache": ([_vp, _vp, _vp, _vp, _vp], ctypes.c_int),
         "trn_smooth_run": ([_vp, _vp, _vp, _vp, _vp, _vp, _P(_i32), _i32, _i32, _i32, _P(_i32),
@@ -200,7 +200,7 @@ class DeviceMesh:
     """Device topology of a triangle mesh (trn_mesh): v2t/v2v CSR built on
     the GPU with the semantics of Mesh::rebuildTopology."""
 
-    def __init__(self, ctx: Context, nv: int, tri, tags):
+    def __init__(self, ctx: Context, nv: int, tri, tags, xy=None):
         torch = _torch()
         self.ctx = ctx
         self.nv = int(nv)
@@ -209,9 +209,15 @@ class DeviceMesh:
         tags_t = to_device(np.asarray(tags, dtype=np.uint8) if not isinstance(tags, torch.Tensor)
                            else tags, torch.uint8)
         self.nt = tri_t.numel() // 3
+        xy_t = None
+        if xy is not None:
+            xy_t = to_device(xy if isinstance(xy, torch.Tensor) else np.asarray(xy, dtype=np.float64),
+                             torch.float64)
         h = _vp()
         _check(lib().trn_mesh_create(ctx.handle, self.nv, self.nt, _vp(tri_t.data_ptr()),
-                                     _vp(tags_t.data_ptr()), ctypes.byref(h)))
+                                     _vp(tags_t.data_ptr()),
+                                     _vp(xy_t.data_ptr()) if xy_t is not None else None,
+                                     ctypes.byref(h)))
         self.handle = h
         nnz, mdt, mdv, mp = _i64(), _i32(), _i32(), _i32()
         _check(lib().trn_mesh_info(h, ctypes.byref(nnz), ctypes.byref(mdt), ctypes.byref(mdv),
@@ -336,12 +342,12 @@ def color(ctx, mesh, seed=1, max_colors=64) -> Coloring:
     return Coloring(col, order, np.array(off[:nc.value + 1], dtype=np.int32), nc.value)
 
 
-def color_classes(ctx, color_t, ncolors) -> Coloring:
-    torch = _torch()
-    order = torch.empty_like(color_t)
+def color_classes(ctx, color_t, ncolors, mesh=None) -> Coloring:
+    order = _torch().empty_like(color_t)
     off = (ctypes.c_int32 * 80)()
-    _check(lib().trn_color_classes(ctx.handle, color_t.numel(), _vp(color_t.data_ptr()),
-                                   int(ncolors), _vp(order.data_ptr()), off))
+    _check(lib().trn_color_classes(ctx.handle, mesh.handle if mesh is not None else None,
+                                   color_t.numel(), _vp(color_t.data_ptr()), int(ncolors),
+                                   _vp(order.data_ptr()), off))
     return Coloring(color_t, order, np.array(off[:ncolors + 1], dtype=np.int32), int(ncolors))
 
 

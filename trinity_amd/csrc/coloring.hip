@@ -38,9 +38,16 @@ __device__ __forceinline__ uint32_t color_priority(uint64_t seed, uint32_t v) {
   return (uint32_t)(z >> 32);
 }
 
+constexpr int kRoundSlots = 64;   // per-round loser counters, spread over addresses
+constexpr int kRoundsPerSync = 8;  // rounds queued between two host checks
+
+// Rounds are queued in batches; a round whose predecessor left no loser
+// returns at once (prev_total == 0), so over-queued rounds cost a launch.
 __global__ void k_color_tent(int32_t nv, const int32_t* __restrict__ v2v_off,
                              const int32_t* __restrict__ v2v, const int32_t* __restrict__ color,
-                             int32_t* __restrict__ tent, int max_colors, int32_t* fail) {
+                             int32_t* __restrict__ tent, int max_colors, int32_t* fail,
+                             const int32_t* __restrict__ prev_total) {
+  if (prev_total && *prev_total == 0) return;
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nv) return;
   const int32_t b = v2v_off[v], e = v2v_off[v + 1];
@@ -68,7 +75,9 @@ __global__ __launch_bounds__(256) void k_color_resolve(int32_t nv,
                                                        const int32_t* __restrict__ v2v,
                                                        int32_t* __restrict__ color,
                                                        const int32_t* __restrict__ tent,
-                                                       uint64_t seed, int32_t* remaining) {
+                                                       uint64_t seed, int32_t* remaining,
+                                                       const int32_t* __restrict__ prev_total) {
+  if (prev_total && *prev_total == 0) return;
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int lose = 0;
   if (v < nv) {
@@ -88,9 +97,19 @@ __global__ __launch_bounds__(256) void k_color_resolve(int32_t nv,
       if (!lose) color[v] = tv;
     }
   }
-  // one atomic per block (a single hot address serialises in L2)
+  // one atomic per block, spread over kRoundSlots addresses (a single hot
+  // address serialises in L2)
   const int losers = __syncthreads_count(lose);
-  if (threadIdx.x == 0 && losers) atomicAdd(remaining, losers);
+  if (threadIdx.x == 0 && losers) atomicAdd(&remaining[blockIdx.x & (kRoundSlots - 1)], losers);
+}
+
+// total[0] = sum of the round's slots, or 0 when the previous round was
+// already empty (its slots were never written).
+__global__ void k_color_round_total(const int32_t* __restrict__ slots, int32_t* __restrict__ total) {
+  int s = slots[threadIdx.x];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (threadIdx.x == 0) total[0] = s;
 }
 
 __global__ void k_color_check(int32_t nv, const int32_t* __restrict__ v2v_off,
@@ -109,14 +128,16 @@ __global__ void k_color_check(int32_t nv, const int32_t* __restrict__ v2v_off,
 }
 
 // Per-block histogram of colours 1..ncolors, stored colour-major.
+// Position k of the visiting sequence is vertex perm[k] (or k without perm).
 __global__ __launch_bounds__(kClassB) void k_class_count(int32_t nv, const int32_t* __restrict__ color,
+                                                         const int32_t* __restrict__ perm,
                                                          int ncolors, int32_t* __restrict__ cnt) {
   __shared__ int hist[64];
   if (threadIdx.x < 64) hist[threadIdx.x] = 0;
   __syncthreads();
-  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v < nv) {
-    const int c = color[v];
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nv) {
+    const int c = color[perm ? perm[k] : k];
     if (c >= 1 && c <= ncolors) atomicAdd(&hist[c - 1], 1);
   }
   __syncthreads();
@@ -124,6 +145,7 @@ __global__ __launch_bounds__(kClassB) void k_class_count(int32_t nv, const int32
 }
 
 __global__ __launch_bounds__(kClassB) void k_class_scatter(int32_t nv, const int32_t* __restrict__ color,
+                                                           const int32_t* __restrict__ perm,
                                                            int ncolors,
                                                            const int32_t* __restrict__ off,
                                                            int32_t* __restrict__ order) {
@@ -131,9 +153,10 @@ __global__ __launch_bounds__(kClassB) void k_class_scatter(int32_t nv, const int
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int k = threadIdx.x; k < (kClassB / 64) * 64; k += blockDim.x) (&wcnt[0][0])[k] = 0;
   __syncthreads();
-  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t v = (k < nv && perm) ? perm[k] : k;
   int c = 0;
-  if (v < nv) {
+  if (k < nv) {
     c = color[v];
     if (c < 1 || c > ncolors) c = 0;
   }
@@ -165,9 +188,10 @@ __global__ void k_class_offsets(const int32_t* __restrict__ off, int ncolors, in
 
 static unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-// Builds order_d and class_off (host) for colours 1..ncolors.
-static void build_classes(trn_ctx* ctx, int32_t nv, const int32_t* color, int ncolors,
-                          int32_t* order, int32_t* class_off) {
+// Builds order_d and class_off (host) for colours 1..ncolors, visiting the
+// vertices in perm order when given.
+static void build_classes(trn_ctx* ctx, int32_t nv, const int32_t* color, const int32_t* perm,
+                          int ncolors, int32_t* order, int32_t* class_off) {
   const int64_t nb = nblk(nv, kClassB);
   const int64_t n = nb * ncolors;
   Scratch sc(ctx, scratch_bytes(sizeof(int32_t) * (size_t)(n + 1)) * 2 +
@@ -177,10 +201,10 @@ static void build_classes(trn_ctx* ctx, int32_t nv, const int32_t* color, int nc
   int32_t* tmp = sc.carve<int32_t>(exclusive_scan_scratch(n) / 4);
   int32_t* offs = sc.carve<int32_t>(80);
   launch(ctx, "class_count", 4.0 * nv, k_class_count, dim3((unsigned)nb), dim3(kClassB), 0, nv,
-         color, ncolors, cnt);
+         color, perm, ncolors, cnt);
   exclusive_scan_i32(ctx, cnt, off, n, true, tmp);
   launch(ctx, "class_scatter", 8.0 * nv, k_class_scatter, dim3((unsigned)nb), dim3(kClassB), 0,
-         nv, color, ncolors, (const int32_t*)off, order);
+         nv, color, perm, ncolors, (const int32_t*)off, order);
   launch(ctx, "class_offsets", 0.0, k_class_offsets, dim3(1), dim3(128), 0, (const int32_t*)off,
          ncolors, (int)nb, offs);
   copy_async(ctx, ctx->pinned_i32, offs, sizeof(int32_t) * (size_t)(ncolors + 1));
@@ -202,24 +226,37 @@ int trn_color(trn_ctx* ctx, const trn_mesh* mesh, uint64_t seed, int32_t max_col
                 "null argument");
     TRN_REQUIRE(max_colors >= 1 && max_colors <= 64, TRN_ERR_ARG, "max_colors must be in 1..64");
     const int32_t nv = mesh->nv;
-    Scratch sc(ctx, scratch_bytes(sizeof(int32_t) * (size_t)nv) + scratch_bytes(64));
+    const size_t nslots = (size_t)kRoundsPerSync * kRoundSlots;
+    Scratch sc(ctx, scratch_bytes(sizeof(int32_t) * (size_t)nv) + scratch_bytes(64) +
+                        scratch_bytes(sizeof(int32_t) * nslots) +
+                        scratch_bytes(sizeof(int32_t) * kRoundsPerSync));
     int32_t* tent = sc.carve<int32_t>(nv);
     int32_t* flags = sc.carve<int32_t>(16);
+    int32_t* slots = sc.carve<int32_t>(nslots);
+    int32_t* totals = sc.carve<int32_t>(kRoundsPerSync);
     memset_async(ctx, color_d, 0, sizeof(int32_t) * (size_t)nv);
     memset_async(ctx, flags, 0, sizeof(int32_t) * 16);
     int32_t* h = ctx->pinned_i32;
     int rounds = 0;
     for (;;) {
-      memset_async(ctx, flags, 0, sizeof(int32_t) * 2);
-      launch(ctx, "color_tent", 4.0 * nv + 4.0 * (double)mesh->nnz_v2v, k_color_tent,
-             dim3(nblk(nv, 256)), dim3(256), 0, nv, (const int32_t*)mesh->v2v_off,
-             (const int32_t*)mesh->v2v, (const int32_t*)color_d, tent, (int)max_colors, flags);
-      launch(ctx, "color_resolve", 4.0 * nv + 4.0 * (double)mesh->nnz_v2v, k_color_resolve,
-             dim3(nblk(nv, 256)), dim3(256), 0, nv, (const int32_t*)mesh->v2v_off,
-             (const int32_t*)mesh->v2v, color_d, (const int32_t*)tent, seed, flags + 1);
-      copy_async(ctx, h, flags, 2 * sizeof(int32_t));
+      memset_async(ctx, slots, 0, sizeof(int32_t) * nslots);
+      for (int r = 0; r < kRoundsPerSync; ++r) {
+        const int32_t* prev = r ? totals + r - 1 : nullptr;
+        launch(ctx, "color_tent", 4.0 * nv + 4.0 * (double)mesh->nnz_v2v, k_color_tent,
+               dim3(nblk(nv, 256)), dim3(256), 0, nv, (const int32_t*)mesh->v2v_off,
+               (const int32_t*)mesh->v2v, (const int32_t*)color_d, tent, (int)max_colors, flags,
+               prev);
+        launch(ctx, "color_resolve", 4.0 * nv + 4.0 * (double)mesh->nnz_v2v, k_color_resolve,
+               dim3(nblk(nv, 256)), dim3(256), 0, nv, (const int32_t*)mesh->v2v_off,
+               (const int32_t*)mesh->v2v, color_d, (const int32_t*)tent, seed,
+               slots + (size_t)r * kRoundSlots, prev);
+        launch(ctx, "color_round", 0.0, k_color_round_total, dim3(1), dim3(kRoundSlots), 0,
+               (const int32_t*)(slots + (size_t)r * kRoundSlots), totals + r);
+      }
+      copy_async(ctx, h, flags, sizeof(int32_t));
+      copy_async(ctx, h + 1, totals + kRoundsPerSync - 1, sizeof(int32_t));
       sync(ctx);
-      ++rounds;
+      rounds += kRoundsPerSync;
       TRN_REQUIRE(h[0] == 0, TRN_ERR_PALETTE,
                   "a vertex needs more than " + std::to_string(max_colors) + " colours");
       if (h[1] == 0) break;
@@ -229,18 +266,19 @@ int trn_color(trn_ctx* ctx, const trn_mesh* mesh, uint64_t seed, int32_t max_col
     copy_async(ctx, h, flags + 4, sizeof(int32_t));
     sync(ctx);
     const int nc = h[0];
-    build_classes(ctx, nv, color_d, nc, order_d, class_off);
+    build_classes(ctx, nv, color_d, mesh->perm, nc, order_d, class_off);
     *ncolors = nc;
   });
 }
 
-int trn_color_classes(trn_ctx* ctx, int32_t nv, const int32_t* color_d, int32_t ncolors,
-                      int32_t* order_d, int32_t* class_off) {
+int trn_color_classes(trn_ctx* ctx, const trn_mesh* mesh, int32_t nv, const int32_t* color_d,
+                      int32_t ncolors, int32_t* order_d, int32_t* class_off) {
   return guard_impl("trn_color_classes", [&] {
     DeviceScope ds(ctx);
     TRN_REQUIRE(nv >= 0 && color_d && order_d && class_off, TRN_ERR_ARG, "null argument");
     TRN_REQUIRE(ncolors >= 1 && ncolors <= 64, TRN_ERR_ARG, "ncolors must be in 1..64");
-    build_classes(ctx, nv, color_d, ncolors, order_d, class_off);
+    TRN_REQUIRE(!mesh || mesh->nv == nv, TRN_ERR_ARG, "nv differs from the mesh");
+    build_classes(ctx, nv, color_d, mesh ? mesh->perm : nullptr, ncolors, order_d, class_off);
   });
 }
 

@@ -54,7 +54,7 @@ __global__ __launch_bounds__(256) void k_tri_grad(int32_t nt, const int32_t* __r
                                                   const double* __restrict__ f,
                                                   double* __restrict__ area_t,
                                                   double2* __restrict__ ag_t) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= nt) return;
   const int32_t n0 = tri[3 * t], n1 = tri[3 * t + 1], n2 = tri[3 * t + 2];
   if (n0 < 0) return;  // deleted triangles never appear in a patch
@@ -107,7 +107,7 @@ __global__ __launch_bounds__(256) void k_tri_hess(int32_t nt, const int32_t* __r
                                                   const double2* __restrict__ xy,
                                                   const double2* __restrict__ grad,
                                                   double4* __restrict__ ah_t) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= nt) return;
   const int32_t n0 = tri[3 * t], n1 = tri[3 * t + 1], n2 = tri[3 * t + 2];
   if (n0 < 0) return;

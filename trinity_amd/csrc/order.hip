@@ -1,0 +1,115 @@
+// order.hip -- spatial (Morton / Z-curve) order of the mesh vertices.
+//
+// Colour classes are independent sets, so the reference's result does not
+// depend on the order in which a class is visited (src/smoothing.cpp:166-175).
+// Visiting a class along a Z-curve of the vertex coordinates makes the 64
+// vertices of a wave a compact 2-D patch whose 1-ring halo is mostly its own,
+// instead of a row segment whose 3-row band is re-read by the waves of the
+// rows above and below; this is what keeps the sweeps inside L2.
+#include <hipcub/hipcub.hpp>
+
+#include "trn_api.hpp"
+
+namespace trn {
+
+constexpr int kBoxBlocks = 512;
+
+__global__ __launch_bounds__(256) void k_bbox_partial(int32_t nv, const double2* __restrict__ xy,
+                                                      double* __restrict__ part) {
+  __shared__ double s[4][4];
+  double lo_x = INFINITY, lo_y = INFINITY, hi_x = -INFINITY, hi_y = -INFINITY;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 p = xy[i];
+    lo_x = fmin(lo_x, p.x);
+    lo_y = fmin(lo_y, p.y);
+    hi_x = fmax(hi_x, p.x);
+    hi_y = fmax(hi_y, p.y);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lo_x = fmin(lo_x, __shfl_xor(lo_x, o, 64));
+    lo_y = fmin(lo_y, __shfl_xor(lo_y, o, 64));
+    hi_x = fmax(hi_x, __shfl_xor(hi_x, o, 64));
+    hi_y = fmax(hi_y, __shfl_xor(hi_y, o, 64));
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s[w][0] = lo_x;
+    s[w][1] = lo_y;
+    s[w][2] = hi_x;
+    s[w][3] = hi_y;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; ++k) {
+      s[0][0] = fmin(s[0][0], s[k][0]);
+      s[0][1] = fmin(s[0][1], s[k][1]);
+      s[0][2] = fmax(s[0][2], s[k][2]);
+      s[0][3] = fmax(s[0][3], s[k][3]);
+    }
+    for (int c = 0; c < 4; ++c) part[4 * blockIdx.x + c] = s[0][c];
+  }
+}
+
+__device__ __forceinline__ uint32_t spread_bits(uint32_t x) {
+  x &= 0xFFFFu;
+  x = (x | (x << 8)) & 0x00FF00FFu;
+  x = (x | (x << 4)) & 0x0F0F0F0Fu;
+  x = (x | (x << 2)) & 0x33333333u;
+  x = (x | (x << 1)) & 0x55555555u;
+  return x;
+}
+
+__device__ __forceinline__ uint32_t quantize(double v, double lo, double scale) {
+  const double q = (v - lo) * scale;
+  if (!(q >= 0.)) return 0u;  // also NaN
+  return q >= 65535. ? 65535u : (uint32_t)q;
+}
+
+__global__ __launch_bounds__(256) void k_morton(int32_t nv, const double2* __restrict__ xy,
+                                                const double* __restrict__ part, int nparts,
+                                                uint32_t* __restrict__ key,
+                                                int32_t* __restrict__ idx) {
+  __shared__ double box[4];
+  if (threadIdx.x < 4) {
+    double v = part[threadIdx.x];
+    for (int b = 1; b < nparts; ++b)
+      v = threadIdx.x < 2 ? fmin(v, part[4 * b + threadIdx.x]) : fmax(v, part[4 * b + threadIdx.x]);
+    box[threadIdx.x] = v;
+  }
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const double ext = fmax(box[2] - box[0], box[3] - box[1]);
+  const double scale = ext > 0. ? 65536. / ext : 0.;
+  const double2 p = xy[i];
+  key[i] = spread_bits(quantize(p.x, box[0], scale)) |
+           (spread_bits(quantize(p.y, box[1], scale)) << 1);
+  idx[i] = (int32_t)i;
+}
+
+void spatial_order(trn_ctx* ctx, int32_t nv, const double* xy, int32_t* perm) {
+  size_t sort_bytes = 0;
+  TRN_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const uint32_t*)nullptr,
+                                             (uint32_t*)nullptr, (const int32_t*)nullptr,
+                                             (int32_t*)nullptr, nv, 0, 32, ctx->stream));
+  Scratch sc(ctx, scratch_bytes(sizeof(double) * 4 * kBoxBlocks) +
+                      2 * scratch_bytes(sizeof(uint32_t) * (size_t)nv) +
+                      scratch_bytes(sizeof(int32_t) * (size_t)nv) + scratch_bytes(sort_bytes));
+  double* part = sc.carve<double>(4 * kBoxBlocks);
+  uint32_t* key = sc.carve<uint32_t>(nv);
+  uint32_t* key2 = sc.carve<uint32_t>(nv);
+  int32_t* idx = sc.carve<int32_t>(nv);
+  void* tmp = sc.carve<char>(sort_bytes);
+  int blocks = (int)((nv + 255) / 256);
+  if (blocks > kBoxBlocks) blocks = kBoxBlocks;
+  launch(ctx, "order_bbox", 16.0 * nv, k_bbox_partial, dim3(blocks), dim3(256), 0, nv,
+         (const double2*)xy, part);
+  launch(ctx, "order_morton", 24.0 * nv, k_morton, dim3((unsigned)((nv + 255) / 256)), dim3(256),
+         0, nv, (const double2*)xy, (const double*)part, blocks, key, idx);
+  TRN_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, sort_bytes, (const uint32_t*)key, key2,
+                                             (const int32_t*)idx, perm, nv, 0, 32, ctx->stream));
+}
+
+}  // namespace trn

@@ -1,15 +1,30 @@
 // smoothing.hip -- coloured metric-aware Laplacian smoothing on gfx950,
 // replacing Smooth::run after its colouring (src/smoothing.cpp:274-302).
 //
+//   k_tensor_log     per vertex: the matrix logarithm term of
+//                    numeric::interpolateTensor (numeric.cpp:115-154), cached
+//                    as 4 doubles.  The reference recomputes it for every
+//                    tensor of every quality evaluation and interpolation
+//                    (~60 eigen decompositions and logs per move); the
+//                    cached value is the same function of the same tensor,
+//                    so results are unchanged.
 //   k_quality_cache  per triangle: Mesh::computeQuality (Smooth::cacheQuality,
-//                    src/smoothing.cpp:145-155)
+//                    src/smoothing.cpp:145-155) from the cached logs.
 //   k_smooth_phase   one launch per colour class and sweep: every thread
 //                    applies Smooth::moveSmartLaplacian (:40-126) to one
 //                    vertex.  Vertices of one colour share no triangle and
 //                    read only their own 1-ring, so a class is an
-//                    independent set and the launch order inside it does not
-//                    change the result; colours run in ascending order
+//                    independent set and the order inside it does not change
+//                    the result; colours run in ascending order
 //                    (Smooth::movePoints, :166-175), as in the reference.
+//                    The averaged tensor of an incident triangle depends only
+//                    on tensors, so it is formed once per move (lazily, on
+//                    the first try that reaches the triangle) instead of on
+//                    every try; positions change per try and are re-evaluated.
+//                    Triangle corners come from the mesh's star codes (the
+//                    vertex's slot in the triangle and the 1-ring entries of
+//                    the two other corners), so the kernel reads no
+//                    triangle array.
 #include <cfloat>
 
 #include "trn_api.hpp"
@@ -17,32 +32,45 @@
 
 namespace trn {
 
-constexpr int kSmoothB = 64;
 constexpr int kCounterSlots = 64;
+
+__global__ __launch_bounds__(256) void k_tensor_log(int32_t nv, const double* __restrict__ tensor,
+                                                    double4* __restrict__ logm) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  logm[i] = tensor_log4(tensor[3 * i], tensor[3 * i + 1], tensor[3 * i + 2]);
+}
 
 __global__ __launch_bounds__(256) void k_quality_cache(int32_t nt, const int32_t* __restrict__ tri,
                                                        const double2* __restrict__ xy,
-                                                       const double* __restrict__ tensor,
+                                                       const double4* __restrict__ logm,
                                                        double* __restrict__ qualit) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= nt) return;
   const int32_t a = tri[3 * t], b = tri[3 * t + 1], c = tri[3 * t + 2];
   if (a < 0) return;  // Mesh::isActiveElem (src/mesh.cpp:375)
+  double j[3];
+  mean_from_logs(logm[a], logm[b], logm[c], j);
   const double2 pa = xy[a], pb = xy[b], pc = xy[c];
   const double A[2] = {pa.x, pa.y}, B[2] = {pb.x, pb.y}, C[2] = {pc.x, pc.y};
-  qualit[t] = quality(A, B, C, tensor + 3 * (int64_t)a, tensor + 3 * (int64_t)b,
-                      tensor + 3 * (int64_t)c);
+  qualit[t] = quality_from_mean(A, B, C, j);
 }
 
 // Smooth::moveSmartLaplacian (src/smoothing.cpp:40-126) for vertex i.
-// q holds the thread's per-incident-triangle qualities (stride kSmoothB).
+// lds holds, per thread (stride = blockDim.x), the averaged tensor (3
+// doubles) and the quality (1 double) of each incident triangle.  The
+// corners of incident triangle k are taken in the reference's vertex order
+// (the log-Euclidean mean accumulates in that order).
 __device__ int move_vertex(const int i, double2* __restrict__ xy, double* __restrict__ tensor,
-                           double* __restrict__ qualit, const int32_t* __restrict__ tri,
+                           double4* __restrict__ logm, double* __restrict__ qualit,
                            const int32_t* __restrict__ v2t_off, const int32_t* __restrict__ v2t,
-                           const int32_t* __restrict__ v2v_off, const int32_t* __restrict__ v2v,
-                           double* q) {
+                           const int32_t* __restrict__ star, const int32_t* __restrict__ v2v_off,
+                           const int32_t* __restrict__ v2v, double* lds, const int stride) {
   const int32_t t0 = v2t_off[i], deg = v2t_off[i + 1] - t0;
   const int32_t n0 = v2v_off[i], nb = v2v_off[i + 1] - n0;
+  const int32_t* ring = v2v + n0;
+  double* jm = lds;                      // [deg][3]
+  double* qs = lds + 3 * deg * stride;   // [deg]
 
   double q_min = DBL_MAX;  // smoothing.cpp:48-51
   for (int k = 0; k < deg; ++k) q_min = dmin(q_min, qualit[v2t[t0 + k]]);
@@ -53,8 +81,8 @@ __device__ int move_vertex(const int i, double2* __restrict__ xy, double* __rest
                        tensor[3 * (int64_t)i + 2]};
   double opt0 = 0, opt1 = 0;
   double lm[4] = {0., 0., 0., 0.};
-  for (int k = 0; k < nb; ++k) {  // smoothing.cpp:61-79 and numeric.cpp:113-172
-    const int32_t j = v2v[n0 + k];
+  for (int k = 0; k < nb; ++k) {  // smoothing.cpp:61-79, numeric.cpp:113-172
+    const int32_t j = ring[k];
     const double2 pj = xy[j];
     const double Pb[2] = {pj.x, pj.y};
     const double* tb = tensor + 3 * (int64_t)j;
@@ -62,19 +90,19 @@ __device__ int move_vertex(const int i, double2* __restrict__ xy, double* __rest
     const double len = riemann_dist(P, Pb, T, Tb);
     opt0 += P[0] + ((Pb[0] - P[0]) / len);
     opt1 += P[1] + ((Pb[1] - P[1]) / len);
-    double c[4];
-    tensor_log(Tb[0], Tb[1], Tb[2], c);
-    lm[0] += c[0];
-    lm[1] += c[1];
-    lm[2] += c[2];
-    lm[3] += c[3];
+    const double4 L = logm[j];
+    lm[0] += L.x;
+    lm[1] += L.y;
+    lm[2] += L.z;
+    lm[3] += L.w;
   }
   opt0 /= nb;
   opt1 /= nb;
   double TN[3];
   tensor_exp_mean(lm, nb, TN);  // smoothing.cpp:84
+  const double4 LN = tensor_log4(TN[0], TN[1], TN[2]);
 
-  int valid = 0;
+  int valid = 0, nmean = 0;
   double w = 1;
   double px = P[0], py = P[1];
   for (int retry = 0; retry < 5; ++retry) {  // smoothing.cpp:96-114
@@ -82,28 +110,36 @@ __device__ int move_vertex(const int i, double2* __restrict__ xy, double* __rest
     py = (1 - w) * P[1] + w * opt1;
     valid = 1;
     for (int k = 0; valid && k < deg; ++k) {
-      const int32_t t = v2t[t0 + k];
-      const int32_t a = tri[3 * t], b = tri[3 * t + 1], c = tri[3 * t + 2];
-      double pa[2], pb[2], pc[2], ma[3], mb[3], mc[3];
-      const int32_t vv[3] = {a, b, c};
-      double* pp[3] = {pa, pb, pc};
-      double* mm[3] = {ma, mb, mc};
-#pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        if (vv[r] == i) {
-          pp[r][0] = px; pp[r][1] = py;
-          mm[r][0] = TN[0]; mm[r][1] = TN[1]; mm[r][2] = TN[2];
-        } else {
-          const double2 p = xy[vv[r]];
-          pp[r][0] = p.x; pp[r][1] = p.y;
-          const double* m = tensor + 3 * (int64_t)vv[r];
-          mm[r][0] = m[0]; mm[r][1] = m[1]; mm[r][2] = m[2];
-        }
-      }
+      const int32_t code = star[t0 + k];
+      const int r = code & 3;
+      const int32_t ja = ring[(code >> 2) & 255], jb = ring[(code >> 10) & 255];
+      const double2 pa = xy[ja], pb = xy[jb];
+      // corners in triangle order: v[r] = i, v[r+1] = a, v[r+2] = b
+      // (selects, not a runtime-indexed array, to stay in registers)
+      const double pp[3][2] = {
+          {r == 0 ? px : (r == 1 ? pb.x : pa.x), r == 0 ? py : (r == 1 ? pb.y : pa.y)},
+          {r == 0 ? pa.x : (r == 1 ? px : pb.x), r == 0 ? pa.y : (r == 1 ? py : pb.y)},
+          {r == 0 ? pb.x : (r == 1 ? pa.x : px), r == 0 ? pb.y : (r == 1 ? pa.y : py)}};
       valid = 0;
-      if (ccw(pa, pb, pc)) {
-        const double qk = quality(pa, pb, pc, ma, mb, mc);
-        q[k * kSmoothB] = qk;
+      if (ccw(pp[0], pp[1], pp[2])) {
+        double J[3];
+        if (k >= nmean) {  // first try to reach triangle k: average once
+          const double4 La = logm[ja], Lb = logm[jb];
+          const double4 L0 = r == 0 ? LN : (r == 1 ? Lb : La);
+          const double4 L1 = r == 0 ? La : (r == 1 ? LN : Lb);
+          const double4 L2 = r == 0 ? Lb : (r == 1 ? La : LN);
+          mean_from_logs(L0, L1, L2, J);
+          jm[(3 * k) * stride] = J[0];
+          jm[(3 * k + 1) * stride] = J[1];
+          jm[(3 * k + 2) * stride] = J[2];
+          nmean = k + 1;
+        } else {
+          J[0] = jm[(3 * k) * stride];
+          J[1] = jm[(3 * k + 1) * stride];
+          J[2] = jm[(3 * k + 2) * stride];
+        }
+        const double qk = quality_from_mean(pp[0], pp[1], pp[2], J);
+        qs[k * stride] = qk;
         valid = qk > q_min;
       }
     }
@@ -111,31 +147,33 @@ __device__ int move_vertex(const int i, double2* __restrict__ xy, double* __rest
     if (valid) break;
   }
   if (valid) {  // smoothing.cpp:116-123
-    for (int k = 0; k < deg; ++k) qualit[v2t[t0 + k]] = q[k * kSmoothB];
+    for (int k = 0; k < deg; ++k) qualit[v2t[t0 + k]] = qs[k * stride];
     xy[i] = make_double2(px, py);
     tensor[3 * (int64_t)i] = TN[0];
     tensor[3 * (int64_t)i + 1] = TN[1];
     tensor[3 * (int64_t)i + 2] = TN[2];
+    logm[i] = LN;
   }
   return valid;
 }
 
-__global__ __launch_bounds__(kSmoothB) void k_smooth_phase(
+__global__ __launch_bounds__(64) void k_smooth_phase(
     int32_t count, const int32_t* __restrict__ order, double2* __restrict__ xy,
-    double* __restrict__ tensor, double* __restrict__ qualit, const int32_t* __restrict__ tri,
+    double* __restrict__ tensor, double4* __restrict__ logm, double* __restrict__ qualit,
     const uint8_t* __restrict__ tags, const int32_t* __restrict__ v2t_off,
-    const int32_t* __restrict__ v2t, const int32_t* __restrict__ v2v_off,
-    const int32_t* __restrict__ v2v, int32_t* __restrict__ counters) {
-  extern __shared__ double qs[];
-  const int64_t idx = (int64_t)blockIdx.x * kSmoothB + threadIdx.x;
+    const int32_t* __restrict__ v2t, const int32_t* __restrict__ star,
+    const int32_t* __restrict__ v2v_off, const int32_t* __restrict__ v2v,
+    int32_t* __restrict__ counters) {
+  extern __shared__ double lds[];
+  const int64_t idx = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   int commit = 0, task = 0;
   if (idx < count) {
     const int32_t i = order[idx];
     // Smooth::preProcess (:134-138): boundary vertices are never moved
     if (!(tags[i] & 4) && v2t_off[i + 1] > v2t_off[i]) {
       task = 1;
-      commit = move_vertex(i, xy, tensor, qualit, tri, v2t_off, v2t, v2v_off, v2v,
-                           qs + threadIdx.x);
+      commit = move_vertex(i, xy, tensor, logm, qualit, v2t_off, v2t, star, v2v_off, v2v,
+                           lds + threadIdx.x, blockDim.x);
     }
   }
   // counters are spread over kCounterSlots addresses: one hot address
@@ -168,6 +206,27 @@ static unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 // tri24 qualit16 colour-order4 read, xy16 M24 qualit16 written.
 constexpr double kBytesSweepPerV = 196.0;
 
+// Threads per block of the phase kernel and its LDS bytes: 4 doubles per
+// incident triangle per thread, at most 64 KiB per block.
+static void phase_shape(int max_deg, int* block, size_t* shmem) {
+  const int deg = max_deg > 0 ? max_deg : 1;
+  int b = 64;
+  while (b > 16 && (size_t)b * deg * 4 * sizeof(double) > 65536) b >>= 1;
+  *block = b;
+  *shmem = (size_t)b * deg * 4 * sizeof(double);
+}
+
+static void tensor_logs(trn_ctx* ctx, const trn_mesh* m, const double* tensor, double4* logm) {
+  launch(ctx, "tensor_log", 56.0 * m->nv, k_tensor_log, dim3(nblk(m->nv, 256)), dim3(256), 0,
+         m->nv, tensor, logm);
+}
+
+static void quality_cache_logs(trn_ctx* ctx, const trn_mesh* m, const double* xy,
+                               const double4* logm, double* qualit) {
+  launch(ctx, "quality_cache", 0.0, k_quality_cache, dim3(nblk(m->nt, 256)), dim3(256), 0, m->nt,
+         (const int32_t*)m->tri, (const double2*)xy, logm, qualit);
+}
+
 }  // namespace trn
 
 using namespace trn;
@@ -179,8 +238,10 @@ int trn_quality_cache(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d,
   return guard_impl("trn_quality_cache", [&] {
     DeviceScope ds(ctx);
     TRN_REQUIRE(mesh && xy_d && tensor_d && qualit_d, TRN_ERR_ARG, "null argument");
-    launch(ctx, "quality_cache", 0.0, k_quality_cache, dim3(nblk(mesh->nt, 256)), dim3(256), 0,
-           mesh->nt, (const int32_t*)mesh->tri, (const double2*)xy_d, tensor_d, qualit_d);
+    Scratch sc(ctx, scratch_bytes(sizeof(double4) * (size_t)mesh->nv));
+    double4* logm = sc.carve<double4>(mesh->nv);
+    tensor_logs(ctx, mesh, tensor_d, logm);
+    quality_cache_logs(ctx, mesh, xy_d, logm, qualit_d);
   });
 }
 
@@ -194,31 +255,33 @@ int trn_smooth_run(trn_ctx* ctx, const trn_mesh* mesh, double* xy_d, double* ten
                 "null argument");
     TRN_REQUIRE(ncolors >= 0 && ncolors <= 64 && sweeps >= 0, TRN_ERR_ARG,
                 "bad ncolors or sweeps");
+    TRN_REQUIRE(sweeps <= 1024, TRN_ERR_ARG, "at most 1024 sweeps per call");
     for (int c = 0; c < ncolors; ++c)
       TRN_REQUIRE(class_off[c] >= 0 && class_off[c] <= class_off[c + 1] &&
                       class_off[c + 1] <= mesh->nv,
                   TRN_ERR_ARG, "class offsets out of range");
-    TRN_REQUIRE(sweeps <= 1024, TRN_ERR_ARG, "at most 1024 sweeps per call");
     const trn_mesh* m = mesh;
     const size_t ncount = 2 * (size_t)kCounterSlots * (size_t)(sweeps + 1);
     Scratch sc(ctx, scratch_bytes(sizeof(int32_t) * ncount) +
-                        scratch_bytes(sizeof(int32_t) * 2 * (size_t)(sweeps + 1)));
+                        scratch_bytes(sizeof(int32_t) * 2 * (size_t)(sweeps + 1)) +
+                        scratch_bytes(sizeof(double4) * (size_t)m->nv));
     int32_t* counters = sc.carve<int32_t>(ncount);
     int32_t* totals = sc.carve<int32_t>(2 * (size_t)(sweeps + 1));
+    double4* logm = sc.carve<double4>(m->nv);
     memset_async(ctx, counters, 0, sizeof(int32_t) * ncount);
-    if (recompute_qualit)
-      launch(ctx, "quality_cache", 0.0, k_quality_cache, dim3(nblk(m->nt, 256)), dim3(256), 0,
-             m->nt, (const int32_t*)m->tri, (const double2*)xy_d, (const double*)tensor_d,
-             qualit_d);
-    const size_t shmem = sizeof(double) * kSmoothB * (size_t)(m->max_deg_t > 0 ? m->max_deg_t : 1);
+    tensor_logs(ctx, m, tensor_d, logm);
+    if (recompute_qualit) quality_cache_logs(ctx, m, xy_d, logm, qualit_d);
+    int block;
+    size_t shmem;
+    phase_shape(m->max_deg_t, &block, &shmem);
     for (int s = 0; s < sweeps; ++s) {
       for (int c = 0; c < ncolors; ++c) {
         const int32_t count = class_off[c + 1] - class_off[c];
         launch(ctx, "smooth_phase", kBytesSweepPerV * count, k_smooth_phase,
-               dim3(nblk(count, kSmoothB)), dim3(kSmoothB), shmem, count,
-               order_d + class_off[c], (double2*)xy_d, tensor_d, qualit_d,
-               (const int32_t*)m->tri, (const uint8_t*)m->tags, (const int32_t*)m->v2t_off,
-               (const int32_t*)m->v2t, (const int32_t*)m->v2v_off, (const int32_t*)m->v2v,
+               dim3(nblk(count, block)), dim3(block), shmem, count, order_d + class_off[c],
+               (double2*)xy_d, tensor_d, logm, qualit_d, (const uint8_t*)m->tags,
+               (const int32_t*)m->v2t_off, (const int32_t*)m->v2t, (const int32_t*)m->star,
+               (const int32_t*)m->v2v_off, (const int32_t*)m->v2v,
                counters + 2 * kCounterSlots * s);
       }
     }

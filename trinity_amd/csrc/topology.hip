@@ -272,6 +272,37 @@ __global__ __launch_bounds__(64) void k_patch_size(int32_t nv, const int32_t* __
   size[i] = m;
 }
 
+// Star codes (see trn_mesh::star).  Corners are located in the sorted 1-ring
+// by binary search; every corner of an incident triangle is in the 1-ring
+// by construction of v2v.
+__device__ __forceinline__ int ring_index(const int32_t* __restrict__ ring, int n, int32_t x) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (ring[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void k_star(int32_t nv, const int32_t* __restrict__ tri,
+                       const int32_t* __restrict__ v2t_off, const int32_t* __restrict__ v2t,
+                       const int32_t* __restrict__ v2v_off, const int32_t* __restrict__ v2v,
+                       int32_t* __restrict__ star) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const int32_t* ring = v2v + v2v_off[i];
+  const int n = v2v_off[i + 1] - v2v_off[i];
+  for (int32_t k = v2t_off[i]; k < v2t_off[i + 1]; ++k) {
+    const int32_t t = v2t[k];
+    const int32_t c[3] = {tri[3 * t], tri[3 * t + 1], tri[3 * t + 2]};
+    const int r = c[0] == i ? 0 : (c[1] == i ? 1 : 2);
+    const int a = ring_index(ring, n, c[(r + 1) % 3]);
+    const int b = ring_index(ring, n, c[(r + 2) % 3]);
+    star[k] = r | (a << 2) | (b << 10);
+  }
+}
+
 static unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace trn
@@ -281,7 +312,7 @@ using namespace trn;
 extern "C" {
 
 int trn_mesh_create(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
-                    const uint8_t* tags, trn_mesh** out) {
+                    const uint8_t* tags, const double* xy_d, trn_mesh** out) {
   return guard_impl("trn_mesh_create", [&] {
     DeviceScope ds(ctx);
     TRN_REQUIRE(out != nullptr, TRN_ERR_ARG, "null output");
@@ -383,6 +414,21 @@ int trn_mesh_create(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
                   "a 2-ring element patch exceeds " + std::to_string(kMaxPatchCap) + " triangles");
       m->max_patch = h[1];
       m->patch_cap = m->max_patch > 0 ? m->max_patch : 1;
+
+      // 5. star codes of the incident triangles (smoothing reads corners
+      //    through the 1-ring instead of the triangle array)
+      TRN_REQUIRE(m->max_deg_v < 256, TRN_ERR_CAPACITY, "1-ring larger than 255 vertices");
+      TRN_HIP(hipMalloc(&m->star, sizeof(int32_t) * 3 * (size_t)nt));
+      launch(ctx, "topo_star", 0.0, k_star, dim3(blocks_for(nv, 256)), dim3(256), 0, nv,
+             (const int32_t*)m->tri, (const int32_t*)m->v2t_off, (const int32_t*)m->v2t,
+             (const int32_t*)m->v2v_off, (const int32_t*)m->v2v, m->star);
+
+      // 6. locality order of the colour classes
+      if (xy_d) {
+        TRN_HIP(hipMalloc(&m->perm, sizeof(int32_t) * (size_t)nv));
+        spatial_order(ctx, nv, xy_d, m->perm);
+        sync(ctx);
+      }
     } catch (...) {
       cleanup();
       throw;
@@ -402,6 +448,8 @@ int trn_mesh_destroy(trn_mesh* m) {
     if (m->v2t) (void)hipFree(m->v2t);
     if (m->v2v_off) (void)hipFree(m->v2v_off);
     if (m->v2v) (void)hipFree(m->v2v);
+    if (m->star) (void)hipFree(m->star);
+    if (m->perm) (void)hipFree(m->perm);
     delete m;
   });
 }

@@ -33,6 +33,9 @@ void exclusive_scan_i32(trn_ctx* ctx, const int32_t* in, int32_t* out, int64_t n
                         bool with_total, int32_t* tmp);
 size_t exclusive_scan_scratch(int64_t n);
 
+// Z-curve order of the vertices from their coordinates (order.hip).
+void spatial_order(trn_ctx* ctx, int32_t nv, const double* xy, int32_t* perm);
+
 // Reductions used by several passes.
 void reduce_max_i32(trn_ctx* ctx, const int32_t* in, int64_t n, int32_t* out_d);
 

@@ -115,6 +115,21 @@ __device__ __forceinline__ void interpolate3(const double* ta, const double* tb,
   tensor_exp_mean(lm, 3, r);
 }
 
+// x^3 rounded once: x^2 = p + e exactly (FMA error term), p*x = q + f
+// exactly, x^3 = q + f + e*x with e*x below 2^-52 of q, so the final sum
+// rounds correctly except in cases within ~2^-100 of a tie.  The reference's
+// pow(x, 3) is glibc's pow, which is correctly rounded in all but ~1e-3 of
+// cases; the device library's pow(x, 3.) is expanded to x*x*x, which is not
+// in about a quarter of cases.
+__device__ __forceinline__ double cube_rn(const double x) {
+  const double p = x * x;
+  const double e = __builtin_fma(x, x, -p);
+  const double q = p * x;
+  if (!__builtin_isfinite(q)) return q;
+  const double f = __builtin_fma(p, x, -q);
+  return q + (f + e * x);
+}
+
 // Metric-space quality from the averaged tensor j (numeric.cpp:275-305).
 __device__ __forceinline__ double quality_from_mean(const double* pa,
                                                     const double* pb,
@@ -142,8 +157,32 @@ __device__ __forceinline__ double quality_from_mean(const double* pa,
   const double a3 = pc[1] - pa[1];
   const double area = 0.5 * fabs(a0 * a3 - a1 * a2);
   const double mean = dmin(perim / 3, 3 / perim);
-  const double norm = (12 * 1.7320508075688772) * pow(mean * (2. - mean), 3.);
+  const double norm = (12 * 1.7320508075688772) * cube_rn(mean * (2. - mean));
   return area * norm * __builtin_sqrt(det) / (perim * perim);
+}
+
+// Log-Euclidean mean of three tensors from their cached matrix logarithms
+// (numeric.cpp:108-216 with the per-tensor logs precomputed): the log
+// matrix starts at 0 and accumulates the three terms in vertex order.
+__device__ __forceinline__ void mean_from_logs(const double4 la, const double4 lb,
+                                               const double4 lc, double j[3]) {
+  double lm[4] = {0., 0., 0., 0.};
+  lm[0] += la.x; lm[1] += la.y; lm[2] += la.z; lm[3] += la.w;
+  lm[0] += lb.x; lm[1] += lb.y; lm[2] += lb.z; lm[3] += lb.w;
+  lm[0] += lc.x; lm[1] += lc.y; lm[2] += lc.z; lm[3] += lc.w;
+  tensor_exp_mean(lm, 3, j);
+}
+
+__device__ __forceinline__ double4 tensor_log4(const double t0, const double t1,
+                                               const double t2) {
+  double c[4];
+  tensor_log(t0, t1, t2, c);
+  double4 r;
+  r.x = c[0];
+  r.y = c[1];
+  r.z = c[2];
+  r.w = c[3];
+  return r;
 }
 
 // numeric::computeQuality, src/numeric.cpp:251-306.

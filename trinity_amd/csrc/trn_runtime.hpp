@@ -85,6 +85,14 @@ struct trn_mesh {
   int32_t* v2t = nullptr;
   int32_t* v2v_off = nullptr;
   int32_t* v2v = nullptr;
+  // Per incident triangle (parallel to v2t): where the vertex sits in the
+  // triangle and which of its 1-ring entries are the two other corners,
+  // packed r | a << 2 | b << 10 with tri[t][r] = i, tri[t][(r+1)%3] =
+  // v2v[v2v_off[i]+a], tri[t][(r+2)%3] = v2v[v2v_off[i]+b].
+  int32_t* star = nullptr;
+  // Vertices along a Z-curve of their coordinates (NULL: index order);
+  // colour classes list their vertices in this order.
+  int32_t* perm = nullptr;
 };
 
 namespace trn {
