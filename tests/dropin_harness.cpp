@@ -1,0 +1,96 @@
+/*
+ * dropin_harness.cpp -- drives the reference's own Mesh (src/mesh.cpp,
+ * src/io.cpp, ...) with the drop-in Metrics and Smooth of
+ * trinity_amd/dropin/ (GPU), the way src/main.cpp does, and dumps the
+ * results for the parity test tests/test_gpu_dropin.py.
+ *
+ * Built by oracle/Makefile (target `dropin`) into oracle/_ref/ because it
+ * links the reference's sources.  Private members are read after `private`
+ * is mapped to `public` in this translation unit only.
+ *
+ * Usage: dropin_harness MESH.mesh SOLU.bb OUTDIR tf p hmin hmax sweeps
+ */
+#include <stack>
+#include <set>
+#include <map>
+#include <vector>
+#include <string>
+#include <algorithm>
+#include <omp.h>
+#include <thread>
+#include <atomic>
+#include <memory>
+#include <cstring>
+#include <sstream>
+#include <fstream>
+#include <iostream>
+#include <iomanip>
+#include <cmath>
+#include <random>
+#include <limits>
+#include <chrono>
+#include <cstdio>
+#include <cstdint>
+#include <cstdlib>
+#include <csignal>
+#include <cassert>
+#include <functional>
+#include <unordered_map>
+#include <numeric>
+#include <utility>
+#include <mutex>
+#include <unistd.h>
+
+#define private public
+#include "trinity.h"
+#undef private
+
+using namespace trinity;
+
+static void dump(const std::string& path, const void* p, size_t bytes) {
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f || (bytes && std::fwrite(p, 1, bytes, f) != bytes)) {
+    std::fprintf(stderr, "dropin_harness: cannot write %s\n", path.c_str());
+    std::exit(2);
+  }
+  std::fclose(f);
+}
+
+int main(int argc, char** argv) {
+  if (argc < 9) {
+    std::fprintf(stderr, "usage: dropin_harness MESH SOLU OUTDIR tf p hmin hmax sweeps\n");
+    return 2;
+  }
+  const std::string out = argv[3];
+  const double tf = std::atof(argv[4]);
+  const int p = std::atoi(argv[5]);
+  const double hmin = std::atof(argv[6]), hmax = std::atof(argv[7]);
+  const int sweeps = std::atoi(argv[8]);
+
+  std::ifstream file(argv[1], std::ios::in);
+  int size[] = {io::find("Vertices", file), io::find("Triangles", file)};
+  file.close();
+
+  Mesh mesh(size, 64, 2, 0, 1);
+  mesh.load(argv[1], argv[2]);
+  Stats stat[2];
+  Metrics metric(&mesh, tf, p, hmin, hmax);
+  metric.run(stat);
+  const int nv = mesh.nb.nodes, nt = mesh.nb.elems;
+  dump(out + "/metric.bin", mesh.geom.tensor.data(), sizeof(double) * 3 * nv);
+  dump(out + "/complexity.bin", &metric.field.complexity, sizeof(double));
+
+  Partit partit(&mesh, 8);
+  Smooth smooth(&mesh, &partit, sweeps);
+  smooth.run(stat + 1);
+  std::vector<int32_t> color(nv, 0);
+  for (int c = 0; c < partit.nb.parts; ++c)
+    for (int k = 0; k < partit.task.cardin[c]; ++k) color[partit.task.subset[c][k]] = c + 1;
+  dump(out + "/color.bin", color.data(), sizeof(int32_t) * nv);
+  dump(out + "/xy.bin", mesh.geom.points.data(), sizeof(double) * 2 * nv);
+  dump(out + "/tensor.bin", mesh.geom.tensor.data(), sizeof(double) * 3 * nv);
+  dump(out + "/qualit.bin", mesh.geom.qualit.data(), sizeof(double) * nt);
+  const int32_t counts[4] = {stat[0].eval, stat[0].task, stat[1].eval, stat[1].task};
+  dump(out + "/stats.bin", counts, sizeof(counts));
+  return 0;
+}

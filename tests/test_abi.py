@@ -58,3 +58,24 @@ def test_header_compiles_as_c():
                          os.path.join(ROOT, "include", "trinity_amd.h")], capture_output=True,
                         text=True)
     assert cc.returncode == 0, cc.stderr
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference/include"),
+                    reason="reference headers absent")
+def test_dropin_sources_compile_against_reference_headers():
+    dropin = os.path.join(ROOT, "trinity_amd", "dropin")
+    for src in ("metric.cpp", "smoothing.cpp", "trn_dropin.cpp"):
+        r = subprocess.run(["g++", "-std=gnu++14", "-fopenmp", "-fsyntax-only", "-w",
+                            "-I/root/reference/include", "-I" + os.path.join(ROOT, "include"),
+                            "-I" + dropin, "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__",
+                            os.path.join(dropin, src)], capture_output=True, text=True)
+        assert r.returncode == 0, (src, r.stderr[-2000:])
+
+
+def test_product_package_does_not_import_the_oracle():
+    pkg = os.path.join(ROOT, "trinity_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".hpp", ".cpp")):
+                text = open(os.path.join(dirpath, f)).read()
+                assert "import oracle" not in text and "liboracle" not in text, f

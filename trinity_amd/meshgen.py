@@ -212,22 +212,24 @@ def read_trnm(path: str) -> Mesh:
     return Mesh(xy=xy, tri=tri, tags=tags, solut=solut)
 
 
+def boundary_edges(tri: np.ndarray) -> list[tuple[int, int]]:
+    """Edges that belong to exactly one triangle, in triangle orientation."""
+    t = np.asarray(tri, dtype=np.int64)
+    e = np.concatenate([t[:, [0, 1]], t[:, [1, 2]], t[:, [2, 0]]])
+    key = np.minimum(e[:, 0], e[:, 1]) * (int(t.max()) + 1) + np.maximum(e[:, 0], e[:, 1])
+    uniq, inv, cnt = np.unique(key, return_inverse=True, return_counts=True)
+    once = cnt[inv] == 1
+    return [(int(a), int(b)) for a, b in e[once]]
+
+
 def write_medit(mesh_path: str, bb_path: str, m: Mesh) -> None:
     """Write the .mesh/.bb pair parsed by Mesh::load (src/io.cpp:53-161)."""
-    n = m.meta.get("n")
     with open(mesh_path, "w") as fh:
         fh.write("MeshVersionFormatted 1\nDimension\n2\nVertices\n%d\n" % m.nv)
         for (x, y) in m.xy:
             fh.write("%.17g %.17g 1\n" % (x, y))
-        # boundary edges of the unit-square grid
-        edges = []
-        if n is not None:
-            s = n + 1
-            for i in range(n):
-                edges.append((i, i + 1))
-                edges.append((n * s + i, n * s + i + 1))
-                edges.append((i * s, (i + 1) * s))
-                edges.append((i * s + n, (i + 1) * s + n))
+        # boundary edges: those of a single triangle
+        edges = boundary_edges(m.tri)
         fh.write("Edges\n%d\n" % len(edges))
         for a, b in edges:
             fh.write("%d %d 1\n" % (a + 1, b + 1))
