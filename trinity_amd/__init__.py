@@ -37,7 +37,10 @@ EXPORTS = [
     "trn_mesh_create", "trn_mesh_destroy", "trn_mesh_info", "trn_mesh_csr",
     "trn_metric_run", "trn_gradient", "trn_hessian", "trn_normalize_local",
     "trn_complexity", "trn_normalize_global", "trn_color", "trn_color_classes",
-    "trn_color_check", "trn_quality_cache", "trn_smooth_run", "trn_debug_math",
+    "trn_color_check", "trn_quality_cache", "trn_smooth_run",
+    "trn_batch_create", "trn_batch_destroy", "trn_batch_info", "trn_batch_color",
+    "trn_batch_set_config", "trn_batch_get_config", "trn_batch_pass", "trn_batch_pass_graph",
+    "trn_batch_pass_pipelined", "trn_batch_check", "trn_batch_autotune", "trn_debug_math",
 ]
 
 
@@ -95,6 +98,21 @@ def _load():
         "trn_quality_cache": ([_vp, _vp, _vp, _vp, _vp], ctypes.c_int),
         "trn_smooth_run": ([_vp, _vp, _vp, _vp, _vp, _vp, _P(_i32), _i32, _i32, _i32, _P(_i32),
                             _P(_i32)], ctypes.c_int),
+        "trn_batch_create": ([_vp, _i32, _P(_i32), _P(_i32), _vp, _vp, _P(_vp)], ctypes.c_int),
+        "trn_batch_destroy": ([_vp], ctypes.c_int),
+        "trn_batch_info": ([_vp, _P(_i32), _P(_i32), _P(_vp)], ctypes.c_int),
+        "trn_batch_color": ([_vp, _vp, ctypes.c_uint64, _i32, _P(_i32)], ctypes.c_int),
+        "trn_batch_set_config": ([_vp, _i32, _i32, _i32, _i32], ctypes.c_int),
+        "trn_batch_get_config": ([_vp, _P(_i32)], ctypes.c_int),
+        "trn_batch_pass": ([_vp, _vp, _vp, _vp, _vp, _f64, _i32, _f64, _f64, _vp, _vp, _i32, _vp,
+                            _P(_i32), _P(_i32)], ctypes.c_int),
+        "trn_batch_pass_graph": ([_vp, _vp, _vp, _vp, _vp, _f64, _i32, _f64, _f64, _vp, _vp,
+                                  _i32], ctypes.c_int),
+        "trn_batch_pass_pipelined": ([_vp, _vp, _vp, _vp, _vp, _f64, _i32, _f64, _f64, _vp, _vp,
+                                      _i32], ctypes.c_int),
+        "trn_batch_check": ([_vp, _vp], ctypes.c_int),
+        "trn_batch_autotune": ([_vp, _vp, _vp, _vp, _vp, _f64, _i32, _f64, _f64, _vp, _vp, _i32,
+                                _P(_i32), _P(_f64)], ctypes.c_int),
         "trn_debug_math": ([_vp, _i32, _i64, _vp, _vp, _vp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
@@ -381,6 +399,110 @@ def smooth_run(ctx, mesh, xy, tensor, qualit, coloring: Coloring, sweeps: int,
     if counters:
         return list(commits[:sweeps]), list(tasks[:sweeps])
     return None
+
+
+class Batch:
+    """Many meshes of uneven sizes processed in one persistent launch per
+    pass (trn_batch).  No counterpart in the reference, which runs
+    Metrics::run and Smooth::run on one mesh at a time.
+
+    meshes: list of (nv, tri[nt,3] 0-based per mesh, tags[nv]).  Field
+    arrays passed to `run` are the per-mesh arrays concatenated in list
+    order; `voff` / `toff` give each mesh's first vertex / triangle."""
+
+    def __init__(self, ctx: Context, meshes):
+        self.ctx = ctx
+        self.nmesh = len(meshes)
+        nv = np.array([int(m[0]) for m in meshes], dtype=np.int32)
+        tris = [np.asarray(m[1], dtype=np.int32).reshape(-1, 3) for m in meshes]
+        nt = np.array([t.shape[0] for t in tris], dtype=np.int32)
+        tri = np.ascontiguousarray(np.concatenate(tris).reshape(-1))
+        tags = np.ascontiguousarray(np.concatenate([np.asarray(m[2], dtype=np.uint8)
+                                                    for m in meshes]))
+        self.voff = np.concatenate([[0], np.cumsum(nv, dtype=np.int64)])
+        self.toff = np.concatenate([[0], np.cumsum(nt, dtype=np.int64)])
+        self.nv, self.nt = int(self.voff[-1]), int(self.toff[-1])
+        h = _vp()
+        _check(lib().trn_batch_create(ctx.handle, self.nmesh,
+                                      nv.ctypes.data_as(_P(_i32)), nt.ctypes.data_as(_P(_i32)),
+                                      tri.ctypes.data_as(_vp), tags.ctypes.data_as(_vp),
+                                      ctypes.byref(h)))
+        self.handle = h
+        self.ncolors = 0
+
+    def color(self, seed=1, max_colors=64) -> int:
+        nc = _i32()
+        _check(lib().trn_batch_color(self.ctx.handle, self.handle, int(seed) & ((1 << 64) - 1),
+                                     int(max_colors), ctypes.byref(nc)))
+        self.ncolors = nc.value
+        return nc.value
+
+    def set_config(self, block=64, tile_v=256, tile_t=1024, blocks_per_cu=4) -> None:
+        _check(lib().trn_batch_set_config(self.handle, int(block), int(tile_v), int(tile_t),
+                                          int(blocks_per_cu)))
+
+    def config(self) -> tuple:
+        c = (_i32 * 4)()
+        _check(lib().trn_batch_get_config(self.handle, c))
+        return tuple(c)
+
+    def run(self, xy_in, xy, solut, tensor, qualit, target_factor=0.8, lp_norm=2, h_min=1e-9,
+            h_max=0.4, sweeps=3, complexity=None, counters=False):
+        """One pass (metric + quality cache + sweeps) over every mesh.
+        Returns (commits, tasks) per sweep when counters=True."""
+        commits = (_i32 * max(sweeps, 1))()
+        tasks = (_i32 * max(sweeps, 1))()
+        _check(lib().trn_batch_pass(
+            self.ctx.handle, self.handle, _vp(_ptr(xy_in)) if xy_in is not None else None,
+            _vp(xy.data_ptr()), _vp(solut.data_ptr()), float(target_factor), int(lp_norm),
+            float(h_min), float(h_max), _vp(tensor.data_ptr()), _vp(qualit.data_ptr()),
+            int(sweeps), _vp(complexity.data_ptr()) if complexity is not None else None,
+            commits if counters else None, tasks if counters else None))
+        if counters:
+            return list(commits[:sweeps]), list(tasks[:sweeps])
+        return None
+
+    def run_graph(self, xy_in, xy, solut, tensor, qualit, target_factor=0.8, lp_norm=2,
+                  h_min=1e-9, h_max=0.4, sweeps=3):
+        """The same pass replayed from a captured hipGraph (needs the
+        context's own stream: Context.use_own_stream)."""
+        _check(lib().trn_batch_pass_graph(
+            self.ctx.handle, self.handle, _vp(xy_in.data_ptr()), _vp(xy.data_ptr()),
+            _vp(solut.data_ptr()), float(target_factor), int(lp_norm), float(h_min),
+            float(h_max), _vp(tensor.data_ptr()), _vp(qualit.data_ptr()), int(sweeps)))
+
+    def run_pipelined(self, xy_in, xy, solut, tensor, qualit, target_factor=0.8, lp_norm=2,
+                      h_min=1e-9, h_max=0.4, sweeps=3):
+        """The single-launch pipelined pass (switched off: raises)."""
+        _check(lib().trn_batch_pass_pipelined(
+            self.ctx.handle, self.handle, _vp(xy_in.data_ptr()), _vp(xy.data_ptr()),
+            _vp(solut.data_ptr()), float(target_factor), int(lp_norm), float(h_min),
+            float(h_max), _vp(tensor.data_ptr()), _vp(qualit.data_ptr()), int(sweeps)))
+
+    def check(self) -> None:
+        _check(lib().trn_batch_check(self.ctx.handle, self.handle))
+
+    def autotune(self, xy_in, xy, solut, tensor, qualit, target_factor=0.8, lp_norm=2,
+                 h_min=1e-9, h_max=0.4, sweeps=3):
+        best = (_i32 * 4)()
+        ms = _f64()
+        _check(lib().trn_batch_autotune(
+            self.ctx.handle, self.handle, _vp(xy_in.data_ptr()), _vp(xy.data_ptr()),
+            _vp(solut.data_ptr()), float(target_factor), int(lp_norm), float(h_min),
+            float(h_max), _vp(tensor.data_ptr()), _vp(qualit.data_ptr()), int(sweeps), best,
+            ctypes.byref(ms)))
+        return tuple(best), ms.value
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().trn_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def debug_math(ctx, op: int, x, y=None):

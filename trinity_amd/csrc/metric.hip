@@ -18,36 +18,16 @@
 // The per-triangle terms do not depend on the vertex whose patch contains
 // the triangle, so computing them once per triangle and summing the stored
 // products in patch order reproduces the reference's roundings exactly.
+// The element bodies live in trn_recovery.hpp (shared with batch.hip).
 #include <cmath>
 
-#include "trn_api.hpp"
-#include "trn_numeric.hpp"
+#include "trn_recovery.hpp"
 
 namespace trn {
 
 constexpr int kGatherB = 64;  // one wave per block for the LDS-set gathers
 constexpr int kCplxBlocks = 1024;
 constexpr int kCplxB = 256;
-
-// Builds the ascending 2-ring element patch of vertex i in the thread's LDS
-// set (element k at s[k*kGatherB]).  Returns the size; ok=false on overflow.
-__device__ __forceinline__ int build_patch(const int i, const int32_t* __restrict__ v2t_off,
-                                           const int32_t* __restrict__ v2t,
-                                           const int32_t* __restrict__ v2v_off,
-                                           const int32_t* __restrict__ v2v, int* s,
-                                           const int cap, bool& ok) {
-  int m = 0;
-  ok = true;
-  const int32_t b0 = v2t_off[i], e0 = v2t_off[i + 1];
-  for (int32_t k = b0; k < e0; ++k) ok = ok && lds_sorted_insert(s, kGatherB, m, cap, v2t[k]);
-  const int32_t nb0 = v2v_off[i], ne0 = v2v_off[i + 1];
-  for (int32_t a = nb0; a < ne0; ++a) {
-    const int32_t j = v2v[a];
-    const int32_t bj = v2t_off[j], ej = v2t_off[j + 1];
-    for (int32_t k = bj; k < ej; ++k) ok = ok && lds_sorted_insert(s, kGatherB, m, cap, v2t[k]);
-  }
-  return m;
-}
 
 __global__ __launch_bounds__(256) void k_tri_grad(int32_t nt, const int32_t* __restrict__ tri,
                                                   const double2* __restrict__ xy,
@@ -56,21 +36,7 @@ __global__ __launch_bounds__(256) void k_tri_grad(int32_t nt, const int32_t* __r
                                                   double2* __restrict__ ag_t) {
   const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= nt) return;
-  const int32_t n0 = tri[3 * t], n1 = tri[3 * t + 1], n2 = tri[3 * t + 2];
-  if (n0 < 0) return;  // deleted triangles never appear in a patch
-  double a, d[6];
-  tri_geometry(xy[n0], xy[n1], xy[n2], a, d);
-  area_t[t] = a;
-  if (f == nullptr) return;  // area only
-  const double f0 = f[n0], f1 = f[n1], f2 = f[n2];
-  double g0 = 0., g1 = 0.;  // hessian.cpp:36, accumulated j = 0,1,2 (:65-67)
-  g0 += f0 * d[0];
-  g1 += f0 * d[1];
-  g0 += f1 * d[2];
-  g1 += f1 * d[3];
-  g0 += f2 * d[4];
-  g1 += f2 * d[5];
-  ag_t[t] = make_double2(a * g0, a * g1);  // the products of hessian.cpp:78
+  tri_grad_one(t, tri, xy, f, area_t, ag_t);
 }
 
 template <bool WITH_GRAD>
@@ -84,23 +50,12 @@ __global__ __launch_bounds__(kGatherB) void k_vtx_grad(
   if (i >= nv) return;
   int* s = lds + threadIdx.x;
   bool ok;
-  const int m = build_patch(i, v2t_off, v2t, v2v_off, v2v, s, cap, ok);
+  const int m = build_patch(i, v2t_off, v2t, v2v_off, v2v, s, kGatherB, cap, ok);
   if (!ok) {
     atomicOr(err, 1);
     return;
   }
-  double ta = 0., gx = 0., gy = 0.;
-  for (int e = 0; e < m; ++e) {
-    const int t = s[e * kGatherB];
-    ta += area_t[t];       // hessian.cpp:53
-    if (WITH_GRAD) {
-      const double2 r = ag_t[t];
-      gx += r.x;           // hessian.cpp:76-78
-      gy += r.y;
-    }
-  }
-  if (WITH_GRAD) grad[i] = make_double2(gx / ta, gy / ta);  // hessian.cpp:81-82
-  tarea[i] = ta;
+  vtx_grad_one<WITH_GRAD>(i, s, kGatherB, m, area_t, ag_t, grad, tarea);
 }
 
 __global__ __launch_bounds__(256) void k_tri_hess(int32_t nt, const int32_t* __restrict__ tri,
@@ -109,38 +64,7 @@ __global__ __launch_bounds__(256) void k_tri_hess(int32_t nt, const int32_t* __r
                                                   double4* __restrict__ ah_t) {
   const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= nt) return;
-  const int32_t n0 = tri[3 * t], n1 = tri[3 * t + 1], n2 = tri[3 * t + 2];
-  if (n0 < 0) return;
-  double a, d[6];
-  tri_geometry(xy[n0], xy[n1], xy[n2], a, d);
-  const double2 g0 = grad[n0], g1 = grad[n1], g2 = grad[n2];
-  double l0 = 0., l1 = 0., l2 = 0., l3 = 0.;  // hessian.cpp:99, :131-136
-  l0 += g0.x * d[0]; l1 += g0.x * d[1]; l2 += g0.y * d[0]; l3 += g0.y * d[1];
-  l0 += g1.x * d[2]; l1 += g1.x * d[3]; l2 += g1.y * d[2]; l3 += g1.y * d[3];
-  l0 += g2.x * d[4]; l1 += g2.x * d[5]; l2 += g2.y * d[4]; l3 += g2.y * d[5];
-  double4 r;
-  r.x = a * l0;  // the products of hessian.cpp:143
-  r.y = a * l1;
-  r.z = a * l2;
-  r.w = a * l3;
-  ah_t[t] = r;
-}
-
-// Metrics::normalizeLocally for one tensor (src/metric.cpp:113-148).
-__device__ __forceinline__ void normalize_local(double& t0, double& t1, double& t2,
-                                                const double ex) {
-  double val[2], vec[4], h[4];
-  eigen(t0, t1, t1, t2, val, vec);
-  val[0] = dmax(fabs(val[0]), kEps);
-  val[1] = dmax(fabs(val[1]), kEps);
-  const double det = val[0] * val[1];
-  const double scale_loc = pow(det, ex);
-  val[0] *= scale_loc;
-  val[1] *= scale_loc;
-  rebuild(val, vec, h);
-  t0 = h[0];
-  t1 = h[1];
-  t2 = h[3];
+  tri_hess_one(t, tri, xy, grad, ah_t);
 }
 
 template <bool NORMALIZE>
@@ -154,34 +78,12 @@ __global__ __launch_bounds__(kGatherB) void k_vtx_hess(
   if (i >= nv) return;
   int* s = lds + threadIdx.x;
   bool ok;
-  const int m = build_patch(i, v2t_off, v2t, v2v_off, v2v, s, cap, ok);
+  const int m = build_patch(i, v2t_off, v2t, v2v_off, v2v, s, kGatherB, cap, ok);
   if (!ok) {
     atomicOr(err, 1);
     return;
   }
-  double h0 = 0., h1 = 0., h2 = 0., h3 = 0.;  // hessian.cpp:93
-  for (int e = 0; e < m; ++e) {
-    const double4 r = ah_t[s[e * kGatherB]];
-    h0 += r.x;
-    h1 += r.y;
-    h2 += r.z;
-    h3 += r.w;
-  }
-  const double ta = tarea[i];  // same sum as the gradient's (hessian.cpp:120)
-  h0 /= ta;
-  h1 /= ta;
-  h2 /= ta;
-  h3 /= ta;
-  double t0 = h0, t1 = 0.5 * (h1 + h2), t2 = h3;  // hessian.cpp:150-152
-  if (hess) {
-    hess[3 * (int64_t)i] = t0;
-    hess[3 * (int64_t)i + 1] = t1;
-    hess[3 * (int64_t)i + 2] = t2;
-  }
-  if (NORMALIZE) normalize_local(t0, t1, t2, ex);
-  out[3 * (int64_t)i] = t0;
-  out[3 * (int64_t)i + 1] = t1;
-  out[3 * (int64_t)i + 2] = t2;
+  vtx_hess_one<NORMALIZE>(i, s, kGatherB, m, ah_t, tarea, hess, out, ex);
 }
 
 __global__ __launch_bounds__(256) void k_normalize_local(int32_t nv, double* __restrict__ tensor,
@@ -198,25 +100,6 @@ __global__ __launch_bounds__(256) void k_normalize_local(int32_t nv, double* __r
 // Fixed-shape deterministic sum: thread g of the fixed grid adds triangles
 // g, g+G, ... in order; waves reduce with a fixed butterfly; blocks write
 // partials that one block adds in a fixed tree.
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-__device__ __forceinline__ double block_sum(double v, double* lds) {
-  v = wave_sum(v);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) lds[wid] = v;
-  __syncthreads();
-  double r = 0.;
-  if (threadIdx.x == 0) {
-    r = lds[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r += lds[w];
-  }
-  return r;
-}
-
 __global__ __launch_bounds__(kCplxB) void k_tri_cplx(int32_t nt, const int32_t* __restrict__ tri,
                                                      const double2* __restrict__ xy,
                                                      const double* __restrict__ tensor,
@@ -224,26 +107,8 @@ __global__ __launch_bounds__(kCplxB) void k_tri_cplx(int32_t nt, const int32_t* 
   __shared__ double lds[kCplxB / 64];
   double phi = 0.;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nt;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int32_t v0 = tri[3 * t], v1 = tri[3 * t + 1], v2 = tri[3 * t + 2];
-    if (v0 < 0) continue;
-    const double2 p0 = xy[v0], p1 = xy[v1], p2 = xy[v2];
-    const double s0 = p1.x - p0.x;  // metric.cpp:163-167
-    const double s1 = p2.x - p0.x;
-    const double s2 = p1.y - p0.y;
-    const double s3 = p2.y - p0.y;
-    const double area = 0.5 * (s0 * s3 - s1 * s2);
-    double rho = 0.;  // metric.cpp:171-179
-    const int32_t vs[3] = {v0, v1, v2};
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const double* m = tensor + 3 * (int64_t)vs[j];
-      const double det = m[0] * m[2] - m[1] * m[1];
-      rho += __builtin_sqrt(det);
-    }
-    rho /= 3;
-    phi += rho * area;
-  }
+       t += (int64_t)gridDim.x * blockDim.x)
+    phi += cplx_one(t, tri, xy, tensor);
   const double b = block_sum(phi, lds);
   if (threadIdx.x == 0) partial[blockIdx.x] = b;
 }
@@ -266,20 +131,7 @@ __global__ __launch_bounds__(256) void k_vtx_global(int32_t nv, double* __restri
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nv) return;
   const double fact = cplx ? target / cplx[0] : fact_given;
-  double val[2], vec[4], h[4];
-  const double t0 = tensor[3 * i], t1 = tensor[3 * i + 1], t2 = tensor[3 * i + 2];
-  eigen(t0, t1, t1, t2, val, vec);
-  val[0] *= fact;
-  val[1] *= fact;
-#pragma unroll
-  for (int j = 0; j < 2; ++j) {  // metric.cpp:213-216
-    val[j] = dmin(val[j], emin);
-    val[j] = dmax(val[j], emax);
-  }
-  rebuild(val, vec, h);
-  tensor[3 * i] = h[0];
-  tensor[3 * i + 1] = h[1];
-  tensor[3 * i + 2] = h[3];
+  global_one(tensor + 3 * i, fact, emin, emax);
 }
 
 static unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -370,8 +222,6 @@ static void check_err(trn_ctx* ctx, GradBuffers& b) {
   sync(ctx);
   TRN_REQUIRE(ctx->pinned_i32[0] == 0, TRN_ERR_STATE, "2-ring patch larger than measured");
 }
-
-static double metric_exponent(int32_t p) { return p ? -1. / (2 * p + 2) : 0.5; }  // metric.cpp:44
 
 }  // namespace trn
 
