@@ -239,6 +239,44 @@ int trn_batch_autotune(trn_ctx* ctx, trn_batch* batch, const double* xy_in_d,
                        double h_max, double* tensor_d, double* qualit_d,
                        int32_t sweeps, int32_t* best, double* best_ms);
 
+/* ---- adjoint: gradient of a metric-space quality energy -------------------
+ * No counterpart in the reference.  The energy is
+ *     E(xy, f) = sum over active triangles t of (1 - q_M(t))^2
+ * with q_M numeric::computeQuality (src/numeric.cpp:251-306) and M the metric
+ * Metrics::run (src/metric.cpp:59-70) builds from the coordinates xy and the
+ * nodal values f, i.e. exactly what trn_metric_run returns.  The backward
+ * gives dE/dxy and dE/df through the global and local normalisation, the
+ * complexity and the Hessian and gradient recoveries (src/hessian.cpp),
+ * differentiating the reference's branches (eigen shortcuts, clamps) as
+ * they are taken; triangle geometry is recomputed, not stored. */
+typedef struct trn_adjoint trn_adjoint;
+/* Tape and scratch for one mesh (about 200 B per vertex and 270 B per
+ * triangle of device memory). */
+int trn_adjoint_create(trn_ctx* ctx, const trn_mesh* mesh, trn_adjoint** out);
+int trn_adjoint_destroy(trn_adjoint* adjoint);
+/* Forward: builds the metric, the element qualities and E (host), keeping
+ * the tape and the xy_d / f_d pointers, which must stay unchanged until the
+ * backward.  tensor_d (3nv) and qualit_d (nt) are optional device copies of
+ * the metric and the qualities.  Synchronises. */
+int trn_adjoint_forward(trn_ctx* ctx, trn_adjoint* adjoint, const double* xy_d,
+                        const double* f_d, double target_factor, int32_t lp_norm,
+                        double h_min, double h_max, double* energy,
+                        double* tensor_d, double* qualit_d);
+/* Backward of the last forward: dE/dxy (device, 2nv) and dE/df (device, nv);
+ * either may be NULL.  Synchronises. */
+int trn_adjoint_backward(trn_ctx* ctx, trn_adjoint* adjoint, double* dxy_d,
+                         double* df_d);
+/* Gradient-based smoothing: iters steps of steepest descent on E over the
+ * vertices Smooth::run may move (src/smoothing.cpp:134-138: not boundary),
+ * each with a largest displacement of `step` halved until E decreases and no
+ * triangle inverts.  Moves xy_d in place; energies (host, iters + 1, may be
+ * NULL) receives E before and after each step; accepted (host) the number
+ * of steps taken.  Stops early when no step decreases E. */
+int trn_adjoint_smooth(trn_ctx* ctx, trn_adjoint* adjoint, double* xy_d,
+                       const double* f_d, double target_factor, int32_t lp_norm,
+                       double h_min, double h_max, int32_t iters, double step,
+                       double* energies, int32_t* accepted);
+
 /* ---- device math probe ----------------------------------------------------
  * Evaluates the device transcendental used by the path on n inputs
  * (op: 0 log, 1 exp, 2 pow(x, y), 3 pow(x, 3), 4 sqrt) -- lets tests compare

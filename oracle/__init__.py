@@ -66,6 +66,14 @@ def lib():
                                      _i, _i, _i, _i, _i, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, _i, _i]
         L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_energy.argtypes = [ctypes.c_int, ctypes.c_int, _d, _i, _d, _i, _i, _i, _i,
+                                 ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                 _d, _d]
+        L.orc_energy.restype = ctypes.c_double
+        L.orc_energy_grad.argtypes = [ctypes.c_int, ctypes.c_int, _d, _i, _d, _i, _i, _i, _i,
+                                      ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                      ctypes.c_double, _d, _d]
+        L.orc_energy_grad.restype = ctypes.c_double
         L.orc_libm.argtypes = [ctypes.c_int, ctypes.c_int64, _d, _d, _d]
         _lib = L
     return _lib
@@ -138,6 +146,35 @@ def metric_stages(xy, tri, solut, tf=0.8, p=2, hmin=1e-9, hmax=0.4, topo=None) -
     L.orc_normalize_global(nv, _p(metric, _d), target / cplx, float(hmin), float(hmax))
     return {"gradient": grad, "hessian": hess, "metric_local": mloc,
             "complexity": cplx, "metric": metric, "target": target, "topo": topo}
+
+
+def energy(xy, tri, f, tf=0.8, p=2, hmin=1e-9, hmax=0.4, topo=None, want_fields=False):
+    """E = sum_t (1 - q_M(t))^2 with M built from (xy, f) as Metrics::run
+    does (trinity_adjoint.c).  With want_fields, also (metric, qualities)."""
+    xy = f64(xy).reshape(-1, 2)
+    nv = xy.shape[0]
+    tri = np.ascontiguousarray(tri, dtype=np.int32).reshape(-1, 3)
+    topo = topo or Topology(nv, tri)
+    ten = np.zeros((nv, 3)) if want_fields else None
+    q = np.zeros(tri.shape[0]) if want_fields else None
+    E = lib().orc_energy(nv, tri.shape[0], _p(xy, _d), _p(tri, _i), _p(f64(f), _d), *topo.args(),
+                         float(tf), int(p), float(hmin), float(hmax),
+                         _p(ten, _d) if want_fields else None, _p(q, _d) if want_fields else None)
+    return (E, ten, q) if want_fields else E
+
+
+def energy_grad(xy, tri, f, tf=0.8, p=2, hmin=1e-9, hmax=0.4, topo=None):
+    """(E, dE/dxy [nv,2], dE/df [nv]) -- the FP64 reverse-mode gradient."""
+    xy = f64(xy).reshape(-1, 2)
+    nv = xy.shape[0]
+    tri = np.ascontiguousarray(tri, dtype=np.int32).reshape(-1, 3)
+    topo = topo or Topology(nv, tri)
+    dxy = np.zeros((nv, 2))
+    df = np.zeros(nv)
+    E = lib().orc_energy_grad(nv, tri.shape[0], _p(xy, _d), _p(tri, _i), _p(f64(f), _d),
+                              *topo.args(), float(tf), int(p), float(hmin), float(hmax),
+                              _p(dxy, _d), _p(df, _d))
+    return E, dxy, df
 
 
 def eigen(m) -> tuple[np.ndarray, np.ndarray]:

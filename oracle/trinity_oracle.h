@@ -129,4 +129,19 @@ void   orc_libm(int op, int64_t n, const double* x, const double* y,
 #ifdef __cplusplus
 }
 #endif
+
+/* Metric-space quality energy E = sum_t (1 - q_M(t))^2 of the metric that
+ * orc_metric_run builds from (xy, f), and its reverse-mode gradient
+ * (trinity_adjoint.c).  tensor (3nv) and qualit (nt) are optional outputs. */
+double orc_energy(int nv, int nt, const double* xy, const int32_t* tri,
+                  const double* f, const int32_t* v2t_off, const int32_t* v2t,
+                  const int32_t* v2v_off, const int32_t* v2v,
+                  double target_factor, int lp_norm, double h_min,
+                  double h_max, double* tensor, double* qualit);
+double orc_energy_grad(int nv, int nt, const double* xy, const int32_t* tri,
+                       const double* f, const int32_t* v2t_off,
+                       const int32_t* v2t, const int32_t* v2v_off,
+                       const int32_t* v2v, double target_factor, int lp_norm,
+                       double h_min, double h_max, double* dxy, double* df);
+
 #endif

@@ -40,7 +40,9 @@ EXPORTS = [
     "trn_color_check", "trn_quality_cache", "trn_smooth_run",
     "trn_batch_create", "trn_batch_destroy", "trn_batch_info", "trn_batch_color",
     "trn_batch_set_config", "trn_batch_get_config", "trn_batch_pass", "trn_batch_pass_graph",
-    "trn_batch_pass_pipelined", "trn_batch_check", "trn_batch_autotune", "trn_debug_math",
+    "trn_batch_pass_pipelined", "trn_batch_check", "trn_batch_autotune",
+    "trn_adjoint_create", "trn_adjoint_destroy", "trn_adjoint_forward", "trn_adjoint_backward",
+    "trn_adjoint_smooth", "trn_debug_math",
 ]
 
 
@@ -113,6 +115,13 @@ def _load():
         "trn_batch_check": ([_vp, _vp], ctypes.c_int),
         "trn_batch_autotune": ([_vp, _vp, _vp, _vp, _vp, _f64, _i32, _f64, _f64, _vp, _vp, _i32,
                                 _P(_i32), _P(_f64)], ctypes.c_int),
+        "trn_adjoint_create": ([_vp, _vp, _P(_vp)], ctypes.c_int),
+        "trn_adjoint_destroy": ([_vp], ctypes.c_int),
+        "trn_adjoint_forward": ([_vp, _vp, _vp, _vp, _f64, _i32, _f64, _f64, _P(_f64), _vp, _vp],
+                                ctypes.c_int),
+        "trn_adjoint_backward": ([_vp, _vp, _vp, _vp], ctypes.c_int),
+        "trn_adjoint_smooth": ([_vp, _vp, _vp, _vp, _f64, _i32, _f64, _f64, _i32, _f64, _P(_f64),
+                               _P(_i32)], ctypes.c_int),
         "trn_debug_math": ([_vp, _i32, _i64, _vp, _vp, _vp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
@@ -496,6 +505,62 @@ class Batch:
     def close(self) -> None:
         if getattr(self, "handle", None):
             lib().trn_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Adjoint:
+    """Energy E = sum_t (1 - q_M(t))^2 of the metric built from (xy, f) and
+    its gradient (trn_adjoint).  No counterpart in the reference."""
+
+    def __init__(self, ctx: Context, mesh: DeviceMesh):
+        self.ctx, self.mesh = ctx, mesh
+        h = _vp()
+        _check(lib().trn_adjoint_create(ctx.handle, mesh.handle, ctypes.byref(h)))
+        self.handle = h
+        self._keep = None
+
+    def forward(self, xy, f, target_factor=0.8, lp_norm=2, h_min=1e-9, h_max=0.4,
+                tensor=None, qualit=None) -> float:
+        e = _f64()
+        self._keep = (xy, f)  # the backward reads them
+        _check(lib().trn_adjoint_forward(
+            self.ctx.handle, self.handle, _vp(xy.data_ptr()), _vp(f.data_ptr()),
+            float(target_factor), int(lp_norm), float(h_min), float(h_max), ctypes.byref(e),
+            _vp(tensor.data_ptr()) if tensor is not None else None,
+            _vp(qualit.data_ptr()) if qualit is not None else None))
+        return e.value
+
+    def backward(self, dxy=None, df=None):
+        torch = _torch()
+        dev = f"cuda:{self.ctx.device}"
+        dxy = dxy if dxy is not None else torch.empty((self.mesh.nv, 2), dtype=torch.float64,
+                                                      device=dev)
+        df = df if df is not None else torch.empty(self.mesh.nv, dtype=torch.float64, device=dev)
+        _check(lib().trn_adjoint_backward(self.ctx.handle, self.handle, _vp(dxy.data_ptr()),
+                                          _vp(df.data_ptr())))
+        return dxy, df
+
+    def smooth(self, xy, f, target_factor=0.8, lp_norm=2, h_min=1e-9, h_max=0.4, iters=5,
+               step=1e-3):
+        """Gradient-based smoothing in place on xy; returns (energies, accepted)."""
+        en = (_f64 * (iters + 1))()
+        acc = _i32()
+        self._keep = (xy, f)
+        _check(lib().trn_adjoint_smooth(
+            self.ctx.handle, self.handle, _vp(xy.data_ptr()), _vp(f.data_ptr()),
+            float(target_factor), int(lp_norm), float(h_min), float(h_max), int(iters),
+            float(step), en, ctypes.byref(acc)))
+        return list(en), acc.value
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().trn_adjoint_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -112,6 +112,17 @@ def field_boundary_layer(xy: np.ndarray) -> np.ndarray:
     return np.exp(-(y - 0.3) / 1e-3) + 0.1 * np.sin(8.0 * math.pi * x)
 
 
+def field_fronts_quad(xy: np.ndarray, seed: int = 7) -> tuple[np.ndarray, dict]:
+    """Fronts plus the anisotropic quadratic 0.3x^2 + 0.2xy + 0.5y^2, whose
+    Hessian [[0.6, 0.2], [0.2, 1.0]] keeps the recovered Hessians away from
+    the eigenvalue clamp and from the diagonal shortcut of the eigen solver
+    (off-diagonal products below 1e-9), where the metric is not smooth: the
+    energy of this field can be checked by central differences."""
+    f, info = field_fronts(xy, seed=seed)
+    x, y = xy[:, 0], xy[:, 1]
+    return f + 0.3 * x * x + 0.2 * x * y + 0.5 * y * y, info
+
+
 def field_linear(xy: np.ndarray, a: float = 2.0, b: float = -5.0) -> np.ndarray:
     return a * xy[:, 0] + b * xy[:, 1]
 
@@ -133,7 +144,7 @@ def workload(name: str, seed: int = 1234) -> Mesh:
     jitter4096   : 4096^2 jittered grid, boundary layer     (16,785,409 V)
     Other sizes: 'jitterN' / 'uniformN' with the field picked as above
     ('fronts' for jittered, 'tanh' for uniform) or suffixes ':tanh',
-    ':fronts', ':layer', ':linear', ':quadratic'.
+    ':fronts', ':fronts+quad', ':layer', ':linear', ':quadratic'.
     """
     base, _, fld = name.partition(":")
     if base.startswith("uniform"):
@@ -151,6 +162,9 @@ def workload(name: str, seed: int = 1234) -> Mesh:
         m.solut = field_tanh_front(m.xy)
     elif fld == "fronts":
         m.solut, info = field_fronts(m.xy, seed=seed + 1)
+        m.meta["fronts"] = info
+    elif fld == "fronts+quad":
+        m.solut, info = field_fronts_quad(m.xy, seed=seed + 1)
         m.meta["fronts"] = info
     elif fld == "layer":
         m.solut = field_boundary_layer(m.xy)
