@@ -16,7 +16,11 @@ them.
 
 Default workload: 2048^2 jittered grid (4,198,401 vertices, 8,388,608
 triangles), sum of 3 seeded tanh fronts, L^2 metric with target factor 0.8,
-h_min=1e-9, h_max=0.4, GPU colouring, 3 sweeps.
+h_min=1e-9, h_max=0.4, GPU colouring, 3 sweeps.  Unless --no-extras, the
+JSON line also carries (under "extras") the same pass with L^1 and L^4, the
+16.8M-vertex boundary-layer pass (4096^2, L^4, 5 sweeps), the adjoint
+forward/backward on the 1024^2 grid, and gradient-based against Laplacian
+smoothing scored by the same energy.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -99,6 +103,186 @@ def cpu_baseline(args) -> dict | None:
             "kind": "port", "sample": sample}
 
 
+def free_gpu():
+    import gc
+    import torch
+    gc.collect()
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+
+
+def run_plain(T, mg, ctx, workload, p, sweeps, steps, warmup, seed, color_seed,
+              locality=False):
+    """Times `steps` passes (coordinates restored, metric, colouring, sweeps)
+    and returns the wall time per pass and the per-kernel statistics."""
+    import numpy as np
+    import torch
+    m = mg.workload(workload, seed=seed)
+    xy0 = torch.from_numpy(m.xy).cuda()
+    dm = T.DeviceMesh(ctx, m.nv, m.tri, m.tags, xy=xy0 if locality else None)
+    f = torch.from_numpy(m.solut).cuda()
+    xy = torch.empty_like(xy0)
+    tensor = torch.empty((m.nv, 3), dtype=torch.float64, device="cuda")
+    qualit = torch.empty(m.nt, dtype=torch.float64, device="cuda")
+    log(f"{workload}: V={m.nv:,} T={m.nt:,} max 2-ring patch {dm.max_patch}, p={p}, "
+        f"{sweeps} sweeps")
+    ncolors = []
+
+    def step():
+        xy.copy_(xy0)
+        T.metric_run(ctx, dm, xy, f, 0.8, p, 1e-9, 0.4, tensor=tensor, want_complexity=False)
+        col = T.color(ctx, dm, seed=color_seed, max_colors=64)
+        ncolors.append(col.ncolors)
+        T.smooth_run(ctx, dm, xy, tensor, qualit, col, sweeps, recompute_qualit=True,
+                     counters=False)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.set_timing(True)
+    ctx.reset_timing()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    stats = ctx.timings()
+    ctx.set_timing(False)
+    ms = (t1 - t0) * 1e3 / steps
+    nv, nt = m.nv, m.nt
+    dm.close()
+    del m, xy0, dm, f, xy, tensor, qualit
+    free_gpu()
+    return {"nv": nv, "nt": nt, "ms": ms, "stats": stats, "colours": int(np.median(ncolors)),
+            "steps": steps}
+
+
+def summarize(r, sweeps, label=None):
+    """Per-kernel averages and per-stage rates from the HIP-event timings of
+    run_plain (the same timed region)."""
+    steps, stats, nv = r["steps"], r["stats"], r["nv"]
+    kern = {}
+    for name, s in stats.items():
+        kern[name] = {"ms_per_step": s["ms"] / steps, "launches": s["launches"],
+                      "avg_us": 1e3 * s["ms"] / max(s["launches"], 1),
+                      "alg_GBps": (s["bytes"] / (s["ms"] * 1e-3) / 1e9) if s["ms"] > 0 else 0.0}
+    if label:
+        log(f"-- {label}: {r['ms']:.3f} ms/pass, {nv / (r['ms'] * 1e-3) / 1e6:.1f} Mvertices/s")
+    for name, k in sorted(kern.items(), key=lambda kv: -kv[1]["ms_per_step"])[:12]:
+        log(f"  {name:18s} {k['ms_per_step']:8.3f} ms/step  x{k['launches']:5d}  "
+            f"avg {k['avg_us']:9.1f} us  alg {k['alg_GBps']:8.1f} GB/s")
+    metric_names = ["tri_grad", "vtx_grad", "tri_hess", "vtx_hess_norm", "tri_cplx",
+                    "cplx_final", "vtx_global"]
+    metric_ms = sum(kern[n]["ms_per_step"] for n in metric_names if n in kern)
+    color_ms = sum(v["ms_per_step"] for n, v in kern.items()
+                   if n.startswith("color") or n.startswith("class") or n.startswith("scan")
+                   or n == "reduce_max")
+    smooth_ms = sum(kern[n]["ms_per_step"] for n in ("tensor_log", "quality_cache",
+                                                     "smooth_phase", "counter_reduce")
+                    if n in kern)
+    stages = {
+        "recovery_metric": {"ms": round(metric_ms, 4),
+                            "alg_GBps": round(BYTES_METRIC_PER_V * nv / (metric_ms * 1e-3) / 1e9, 1),
+                            "hbm_frac": round(BYTES_METRIC_PER_V * nv / (metric_ms * 1e-3) / 1e9
+                                              / HBM_PEAK_GBS, 4),
+                            "bytes_per_vertex": BYTES_METRIC_PER_V},
+        "colouring": {"ms": round(color_ms, 4), "colours": r["colours"]},
+        "smoothing": {"ms": round(smooth_ms, 4), "sweeps": sweeps,
+                      "Mvertex_moves_per_s": round(nv * sweeps / (smooth_ms * 1e-3) / 1e6, 1),
+                      "alg_GBps": round(BYTES_SWEEP_PER_V * nv * sweeps / (smooth_ms * 1e-3) / 1e9, 1)},
+    }
+    return kern, stages
+
+
+def read_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 PMC summary (profiles/traffic.txt: lines 'kernel workload
+    fetch_bytes write_bytes'), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 4 and not line.startswith("#") and parts[0] == kernel \
+                and parts[1] == workload:
+            return float(parts[2]) + float(parts[3])
+    return None
+
+
+def run_adjoint(T, mg, ctx, workload, steps):
+    """Forward (metric + qualities + energy) and backward (dE/dxy, dE/df)
+    of the energy sum_t (1 - q_M(t))^2."""
+    import torch
+    m = mg.workload(workload)
+    dm = T.DeviceMesh(ctx, m.nv, m.tri, m.tags)
+    adj = T.Adjoint(ctx, dm)
+    xy = torch.from_numpy(m.xy).cuda()
+    f = torch.from_numpy(m.solut).cuda()
+    gx = torch.empty_like(xy)
+    gf = torch.empty_like(f)
+    adj.forward(xy, f, 0.8, 2)
+    adj.backward(gx, gf)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        E = adj.forward(xy, f, 0.8, 2)
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        adj.backward(gx, gf)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    fwd, bwd = (t1 - t0) * 1e3 / steps, (t2 - t1) * 1e3 / steps
+    log(f"adjoint {workload}: forward {fwd:.3f} ms, backward {bwd:.3f} ms, E={E:.6g}")
+    out = {"workload": workload, "vertices": m.nv, "triangles": m.nt, "lp_norm": 2,
+           "forward_ms": round(fwd, 4), "backward_ms": round(bwd, 4),
+           "Mvertices_per_s_forward_backward": round(m.nv / ((fwd + bwd) * 1e-3) / 1e6, 1),
+           "energy": E}
+    adj.close()
+    dm.close()
+    del adj, dm, xy, f, gx, gf
+    free_gpu()
+    return out
+
+
+def compare_smoothing(T, mg, ctx, workload, iters):
+    """Gradient-based smoothing (descent on the energy) against the
+    Laplacian sweeps, from the same mesh, scored by the same energy."""
+    import numpy as np
+    import torch
+    m = mg.workload(workload)
+    dm = T.DeviceMesh(ctx, m.nv, m.tri, m.tags)
+    adj = T.Adjoint(ctx, dm)
+    f = torch.from_numpy(m.solut).cuda()
+    xy = torch.from_numpy(m.xy).cuda()
+    E0 = adj.forward(xy, f, 0.8, 2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    en, acc = adj.smooth(xy, f, 0.8, 2, iters=iters, step=0.25 / math.sqrt(m.nv))
+    torch.cuda.synchronize()
+    tg = time.perf_counter() - t0
+    xl = torch.from_numpy(m.xy).cuda()
+    ten = torch.empty((m.nv, 3), dtype=torch.float64, device="cuda")
+    q = torch.empty(m.nt, dtype=torch.float64, device="cuda")
+    t0 = time.perf_counter()
+    T.metric_run(ctx, dm, xl, f, 0.8, 2, 1e-9, 0.4, tensor=ten, want_complexity=False)
+    col = T.color(ctx, dm, seed=1)
+    T.smooth_run(ctx, dm, xl, ten, q, col, iters, counters=False)
+    torch.cuda.synchronize()
+    tl = time.perf_counter() - t0
+    El = adj.forward(xl, f, 0.8, 2)
+    log(f"smoothing modes {workload}: E0={E0:.6g}  gradient({acc} steps)={en[-1]:.6g} "
+        f"in {tg * 1e3:.1f} ms  laplacian({iters} sweeps)={El:.6g} in {tl * 1e3:.1f} ms")
+    out = {"workload": workload, "iterations": iters, "energy_initial": E0,
+           "gradient": {"energy": en[-1], "accepted_steps": acc, "ms": round(tg * 1e3, 2)},
+           "laplacian": {"energy": El, "sweeps": iters, "ms": round(tl * 1e3, 2)}}
+    adj.close()
+    dm.close()
+    del adj, dm, xy, xl, f, ten, q
+    free_gpu()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,92 +294,72 @@ def main():
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--color-seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the headline pass (no p sweep, adjoint, 16.8M stress run)")
     ap.add_argument("--locality-order", action="store_true",
                     help="visit colour classes along a Z-curve of the coordinates")
     ap.add_argument("--cpu-sample-n", type=int, default=1024)
     ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 512 MiB device-to-device 3 times (scales the rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE counters against known traffic)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 1 << 26
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "copies": 3}))
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; the multi-GPU sharding is a design "
               "(see the sharding section of the design notes) and is not measured", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
-    import torch
     import trinity_amd as T
     from trinity_amd import meshgen as mg
 
-    log(f"building workload {args.workload} ...")
-    m = mg.workload(args.workload, seed=args.seed)
     ctx = T.Context(0)
-    xy0 = torch.from_numpy(m.xy).cuda()
-    dm = T.DeviceMesh(ctx, m.nv, m.tri, m.tags, xy=xy0 if args.locality_order else None)
-    f = torch.from_numpy(m.solut).cuda()
-    xy = torch.empty_like(xy0)
-    tensor = torch.empty((m.nv, 3), dtype=torch.float64, device="cuda")
-    qualit = torch.empty(m.nt, dtype=torch.float64, device="cuda")
-    log(f"V={m.nv:,} T={m.nt:,} max 2-ring patch {dm.max_patch}")
-
-    ncolors = []
-
-    def step():
-        xy.copy_(xy0)
-        T.metric_run(ctx, dm, xy, f, 0.8, args.p, 1e-9, 0.4, tensor=tensor, want_complexity=False)
-        col = T.color(ctx, dm, seed=args.color_seed, max_colors=64)
-        ncolors.append(col.ncolors)
-        T.smooth_run(ctx, dm, xy, tensor, qualit, col, args.sweeps, recompute_qualit=True,
-                     counters=False)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    ctx.set_timing(True)
-    ctx.reset_timing()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    stats = ctx.timings()
-    ctx.set_timing(False)
-    ms = (t1 - t0) * 1e3 / args.steps
-    value = m.nv / (ms * 1e-3) / 1e6
-
-    # per-kernel averages (HIP events on the launch stream, same timed region)
-    kern = {}
-    for name, s in stats.items():
-        kern[name] = {"ms_per_step": s["ms"] / args.steps, "launches": s["launches"],
-                      "avg_us": 1e3 * s["ms"] / max(s["launches"], 1),
-                      "alg_GBps": (s["bytes"] / (s["ms"] * 1e-3) / 1e9) if s["ms"] > 0 else 0.0}
-    for name, k in sorted(kern.items(), key=lambda kv: -kv[1]["ms_per_step"]):
-        log(f"  {name:18s} {k['ms_per_step']:8.3f} ms/step  x{k['launches']:5d}  "
-            f"avg {k['avg_us']:9.1f} us  alg {k['alg_GBps']:8.1f} GB/s")
-    dom = max(kern.items(), key=lambda kv: kv[1]["ms_per_step"])
-    dname, d = dom
+    r = run_plain(T, mg, ctx, args.workload, args.p, args.sweeps, args.steps, args.warmup,
+                  args.seed, args.color_seed, args.locality_order)
+    ms = r["ms"]
+    value = r["nv"] / (ms * 1e-3) / 1e6
+    kern, stages = summarize(r, args.sweeps, f"{args.workload} p={args.p}")
+    stats = r["stats"]
+    dname = max(kern.items(), key=lambda kv: kv[1]["ms_per_step"])[0]
     dstat = stats[dname]
     bytes_per_launch = dstat["bytes"] / max(dstat["launches"], 1)
     avg_s = dstat["ms"] * 1e-3 / max(dstat["launches"], 1)
     achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    traffic = read_traffic(dname, args.workload)
 
-    metric_names = ["tri_grad", "vtx_grad", "tri_hess", "vtx_hess_norm", "tri_cplx",
-                    "cplx_final", "vtx_global"]
-    metric_ms = sum(kern[n]["ms_per_step"] for n in metric_names if n in kern)
-    color_ms = sum(v["ms_per_step"] for n, v in kern.items()
-                   if n.startswith("color") or n.startswith("class") or n.startswith("scan")
-                   or n == "reduce_max")
-    smooth_ms = sum(kern[n]["ms_per_step"] for n in ("quality_cache", "smooth_phase") if n in kern)
-    stages = {
-        "recovery_metric": {"ms": round(metric_ms, 4),
-                            "alg_GBps": round(BYTES_METRIC_PER_V * m.nv / (metric_ms * 1e-3) / 1e9, 1),
-                            "hbm_frac": round(BYTES_METRIC_PER_V * m.nv / (metric_ms * 1e-3) / 1e9
-                                              / HBM_PEAK_GBS, 4),
-                            "bytes_per_vertex": BYTES_METRIC_PER_V},
-        "colouring": {"ms": round(color_ms, 4), "colours": int(np.median(ncolors))},
-        "smoothing": {"ms": round(smooth_ms, 4), "sweeps": args.sweeps,
-                      "Mvertex_moves_per_s": round(m.nv * args.sweeps / (smooth_ms * 1e-3) / 1e6, 1)},
-    }
+    extras = {}
+    if not args.no_extras:
+        for p in (1, 4):
+            if p == args.p:
+                continue
+            rp = run_plain(T, mg, ctx, args.workload, p, args.sweeps, 3, 1, args.seed,
+                           args.color_seed)
+            _, st = summarize(rp, args.sweeps, f"{args.workload} p={p}")
+            extras[f"p{p}"] = {"ms_per_pass": round(rp["ms"], 4),
+                               "Mvertices_per_s": round(rp["nv"] / (rp["ms"] * 1e-3) / 1e6, 2),
+                               "stages": st}
+        rs = run_plain(T, mg, ctx, "jitter4096:layer", 4, 5, 2, 1, args.seed, args.color_seed)
+        _, st = summarize(rs, 5, "jitter4096:layer p=4")
+        extras["stress_16M"] = {"workload": "jitter4096:layer", "vertices": rs["nv"],
+                                "triangles": rs["nt"], "lp_norm": 4, "sweeps": 5,
+                                "ms_per_pass": round(rs["ms"], 3),
+                                "Mvertices_per_s": round(rs["nv"] / (rs["ms"] * 1e-3) / 1e6, 2),
+                                "stages": st}
+        extras["adjoint"] = run_adjoint(T, mg, ctx, "jitter1024:fronts+quad", 5)
+        extras["smoothing_modes"] = compare_smoothing(T, mg, ctx, "jitter256:fronts+quad", 5)
+    extras["batched"] = {"status": "switched off",
+                         "reason": "the batched pass stalled on MI355X and its cause has not "
+                                   "been shown; its entry points refuse without launching"}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -217,16 +381,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": args.workload, "vertices": m.nv, "triangles": m.nt,
+        "config": {"workload": args.workload, "vertices": r["nv"], "triangles": r["nt"],
                    "lp_norm": args.p, "target_factor": 0.8, "h_min": 1e-9, "h_max": 0.4,
                    "sweeps": args.sweeps, "colouring": "gpu seeded speculative greedy",
                    "parallelism": "single GPU"},
         "roofline": {"kernel": dname, "bound": "hbm", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": None,
-                     "note": ("algorithmic bytes/launch over HIP-event launch time; the smoothing "
-                              "kernel is FP64-transcendental bound, see stages")},
+                     "traffic": traffic,
+                     "note": ("achieved = algorithmic bytes per launch / HIP-event launch time; "
+                              "traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE and "
+                              "WRITE_SIZE (profiles/traffic.txt)")},
         "stages": stages,
+        "extras": extras,
         "cpu_baseline": cpu,
     }
     if cpu:
