@@ -164,7 +164,9 @@ int trn_quality_cache(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d,
  * recompute_qualit != 0 the quality cache is rebuilt first (the reference
  * rebuilds it once per Smooth::run, :290).  commits/tasks (host, [sweeps],
  * may be NULL) receive the per-sweep accepted-move and task counts; when
- * either is given the call synchronises. */
+ * either is given the call synchronises.  The sweeps work on a per-mesh
+ * copy of the vertex data: smoothing calls on one mesh must not overlap
+ * (issue them on one stream, or synchronise between them). */
 int trn_smooth_run(trn_ctx* ctx, const trn_mesh* mesh, double* xy_d,
                    double* tensor_d, double* qualit_d, const int32_t* order_d,
                    const int32_t* class_off, int32_t ncolors, int32_t sweeps,

@@ -32,6 +32,21 @@ def test_triangles_are_ccw_and_jitter_bounded():
     assert ((m.tags & mg.TAG_CORNER) != 0).sum() == 4
 
 
+def test_flipped_diagonals_keep_a_valid_triangulation():
+    n = 48
+    m = mg.grid_mesh(n, 0.25, seed=5, flip=0.5)
+    a, b, c = m.xy[m.tri[:, 0]], m.xy[m.tri[:, 1]], m.xy[m.tri[:, 2]]
+    cross = (b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (c[:, 0] - a[:, 0]) * (b[:, 1] - a[:, 1])
+    assert (cross > 0).all()
+    # total area of [0,1]^2, every interior edge shared by exactly two triangles
+    assert abs(0.5 * cross.sum() - 1.0) < 1e-12
+    assert len(mg.boundary_edges(m.tri)) == 4 * n
+    deg = np.bincount(m.tri.ravel(), minlength=m.nv)
+    interior = (m.tags & mg.TAG_BOUND) == 0
+    assert deg[interior].min() == 4 and deg[interior].max() == 8
+    assert np.array_equal(m.xy, mg.grid_mesh(n, 0.25, seed=5).xy)
+
+
 def test_jitter_is_seeded():
     a = mg.grid_mesh(16, 0.25, seed=1)
     b = mg.grid_mesh(16, 0.25, seed=1)

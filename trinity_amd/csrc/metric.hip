@@ -4,9 +4,11 @@
 // Layout and kernels (V vertices, T triangles, all FP64):
 //   k_tri_grad    per triangle: area A_t, A_t*g_t (g_t = element gradient)
 //                 (src/hessian.cpp:42-67)
-//   k_vtx_grad    per vertex: ascending 2-ring patch built in a per-thread
-//                 LDS set, sum A_t and A_t*g_t in patch order, divide
-//                 (src/hessian.cpp:75-82; patch = Mesh::getVicinity(i,2))
+//   k_vtx_grad    per vertex: sum A_t and A_t*g_t over the ascending 2-ring
+//                 patch, divide (src/hessian.cpp:75-82; patch =
+//                 Mesh::getVicinity(i,2)); the patches are stored with the
+//                 mesh in sliced ELL form, one wave per slab, so the patch
+//                 ids are read with coalesced loads
 //   k_tri_hess    per triangle: A_t*delta_t (src/hessian.cpp:108-143)
 //   k_vtx_hess    per vertex: patch-ordered sums / total area, symmetrise
 //                 (src/hessian.cpp:141-152), optionally fused with the local
@@ -39,23 +41,23 @@ __global__ __launch_bounds__(256) void k_tri_grad(int32_t nt, const int32_t* __r
   tri_grad_one(t, tri, xy, f, area_t, ag_t);
 }
 
+static_assert(kGatherB == kPatchSlab, "a gather block reads one slab of stored patches");
+
+// Column of vertex i's stored patch in its slab (block = slab).
+__device__ __forceinline__ const int32_t* patch_col(const int32_t* __restrict__ row,
+                                                    const int32_t* __restrict__ ell) {
+  return ell + (int64_t)row[xcd_block()] * kPatchSlab + threadIdx.x;
+}
+
 template <bool WITH_GRAD>
 __global__ __launch_bounds__(kGatherB) void k_vtx_grad(
-    int32_t nv, const int32_t* __restrict__ v2t_off, const int32_t* __restrict__ v2t,
-    const int32_t* __restrict__ v2v_off, const int32_t* __restrict__ v2v,
-    const double* __restrict__ area_t, const double2* __restrict__ ag_t,
-    double2* __restrict__ grad, double* __restrict__ tarea, int cap, int32_t* err) {
-  extern __shared__ int lds[];
+    int32_t nv, const int32_t* __restrict__ patch_n, const int32_t* __restrict__ patch_row,
+    const int32_t* __restrict__ patch_ell, const double* __restrict__ area_t,
+    const double2* __restrict__ ag_t, double2* __restrict__ grad, double* __restrict__ tarea) {
   const int i = (int)(xcd_block() * kGatherB + threadIdx.x);
   if (i >= nv) return;
-  int* s = lds + threadIdx.x;
-  bool ok;
-  const int m = build_patch(i, v2t_off, v2t, v2v_off, v2v, s, kGatherB, cap, ok);
-  if (!ok) {
-    atomicOr(err, 1);
-    return;
-  }
-  vtx_grad_one<WITH_GRAD>(i, s, kGatherB, m, area_t, ag_t, grad, tarea);
+  vtx_grad_one<WITH_GRAD>(i, patch_col(patch_row, patch_ell), kPatchSlab, patch_n[i], area_t,
+                          ag_t, grad, tarea);
 }
 
 __global__ __launch_bounds__(256) void k_tri_hess(int32_t nt, const int32_t* __restrict__ tri,
@@ -69,21 +71,14 @@ __global__ __launch_bounds__(256) void k_tri_hess(int32_t nt, const int32_t* __r
 
 template <bool NORMALIZE>
 __global__ __launch_bounds__(kGatherB) void k_vtx_hess(
-    int32_t nv, const int32_t* __restrict__ v2t_off, const int32_t* __restrict__ v2t,
-    const int32_t* __restrict__ v2v_off, const int32_t* __restrict__ v2v,
-    const double4* __restrict__ ah_t, const double* __restrict__ tarea,
-    double* __restrict__ hess, double* __restrict__ out, double ex, int cap, int32_t* err) {
-  extern __shared__ int lds[];
+    int32_t nv, const int32_t* __restrict__ patch_n, const int32_t* __restrict__ patch_row,
+    const int32_t* __restrict__ patch_ell, const double4* __restrict__ ah_t,
+    const double* __restrict__ tarea, double* __restrict__ hess, double* __restrict__ out,
+    double ex) {
   const int i = (int)(xcd_block() * kGatherB + threadIdx.x);
   if (i >= nv) return;
-  int* s = lds + threadIdx.x;
-  bool ok;
-  const int m = build_patch(i, v2t_off, v2t, v2v_off, v2v, s, kGatherB, cap, ok);
-  if (!ok) {
-    atomicOr(err, 1);
-    return;
-  }
-  vtx_hess_one<NORMALIZE>(i, s, kGatherB, m, ah_t, tarea, hess, out, ex);
+  vtx_hess_one<NORMALIZE>(i, patch_col(patch_row, patch_ell), kPatchSlab, patch_n[i], ah_t,
+                          tarea, hess, out, ex);
 }
 
 __global__ __launch_bounds__(256) void k_normalize_local(int32_t nv, double* __restrict__ tensor,
@@ -176,17 +171,15 @@ static GradBuffers carve_grad(Scratch& sc, const trn_mesh* m) {
   return b;
 }
 
-static size_t gather_shmem(const trn_mesh* m) { return sizeof(int) * kGatherB * (size_t)m->patch_cap; }
 
 static void run_gradient(trn_ctx* ctx, const trn_mesh* m, const double* xy, const double* f,
                          double* grad, GradBuffers& b) {
   launch(ctx, "tri_grad", 0.0, k_tri_grad, dim3(nblk(m->nt, 256)), dim3(256), 0, m->nt,
          (const int32_t*)m->tri, (const double2*)xy, f, b.area_t, b.ag_t);
   launch(ctx, "vtx_grad", kBytesGradPerV * m->nv, k_vtx_grad<true>, dim3(nblk(m->nv, kGatherB)),
-         dim3(kGatherB), gather_shmem(m), m->nv, (const int32_t*)m->v2t_off,
-         (const int32_t*)m->v2t, (const int32_t*)m->v2v_off, (const int32_t*)m->v2v,
-         (const double*)b.area_t, (const double2*)b.ag_t, (double2*)grad, b.tarea, m->patch_cap,
-         b.err);
+         dim3(kGatherB), 0, m->nv, (const int32_t*)m->patch_n, (const int32_t*)m->patch_row,
+         (const int32_t*)m->patch_ell, (const double*)b.area_t, (const double2*)b.ag_t,
+         (double2*)grad, b.tarea);
 }
 
 static void run_hessian(trn_ctx* ctx, const trn_mesh* m, const double* xy, const double* grad,
@@ -195,16 +188,14 @@ static void run_hessian(trn_ctx* ctx, const trn_mesh* m, const double* xy, const
          (const int32_t*)m->tri, (const double2*)xy, (const double2*)grad, b.ah_t);
   if (normalize)
     launch(ctx, "vtx_hess_norm", kBytesHessPerV * m->nv, k_vtx_hess<true>,
-           dim3(nblk(m->nv, kGatherB)), dim3(kGatherB), gather_shmem(m), m->nv,
-           (const int32_t*)m->v2t_off, (const int32_t*)m->v2t, (const int32_t*)m->v2v_off,
-           (const int32_t*)m->v2v, (const double4*)b.ah_t, (const double*)b.tarea, hess, out, ex,
-           m->patch_cap, b.err);
+           dim3(nblk(m->nv, kGatherB)), dim3(kGatherB), 0, m->nv, (const int32_t*)m->patch_n,
+           (const int32_t*)m->patch_row, (const int32_t*)m->patch_ell, (const double4*)b.ah_t,
+           (const double*)b.tarea, hess, out, ex);
   else
     launch(ctx, "vtx_hess", kBytesHessPerV * m->nv, k_vtx_hess<false>,
-           dim3(nblk(m->nv, kGatherB)), dim3(kGatherB), gather_shmem(m), m->nv,
-           (const int32_t*)m->v2t_off, (const int32_t*)m->v2t, (const int32_t*)m->v2v_off,
-           (const int32_t*)m->v2v, (const double4*)b.ah_t, (const double*)b.tarea, hess, out, ex,
-           m->patch_cap, b.err);
+           dim3(nblk(m->nv, kGatherB)), dim3(kGatherB), 0, m->nv, (const int32_t*)m->patch_n,
+           (const int32_t*)m->patch_row, (const int32_t*)m->patch_ell, (const double4*)b.ah_t,
+           (const double*)b.tarea, hess, out, ex);
 }
 
 static void run_complexity(trn_ctx* ctx, const trn_mesh* m, const double* xy,
@@ -286,10 +277,9 @@ int trn_hessian(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d, const do
            (const int32_t*)mesh->tri, (const double2*)xy_d, (const double*)nullptr, b.area_t,
            b.ag_t);
     launch(ctx, "vtx_area", 0.0, k_vtx_grad<false>, dim3(nblk(mesh->nv, kGatherB)),
-           dim3(kGatherB), gather_shmem(mesh), mesh->nv, (const int32_t*)mesh->v2t_off,
-           (const int32_t*)mesh->v2t, (const int32_t*)mesh->v2v_off, (const int32_t*)mesh->v2v,
-           (const double*)b.area_t, (const double2*)b.ag_t, (double2*)nullptr, b.tarea,
-           mesh->patch_cap, b.err);
+           dim3(kGatherB), 0, mesh->nv, (const int32_t*)mesh->patch_n,
+           (const int32_t*)mesh->patch_row, (const int32_t*)mesh->patch_ell,
+           (const double*)b.area_t, (const double2*)b.ag_t, (double2*)nullptr, b.tarea);
     run_hessian(ctx, mesh, xy_d, gradient_d, nullptr, tensor_d, false, 0.0, b);
     check_err(ctx, b);
   });

@@ -48,25 +48,52 @@ __global__ __launch_bounds__(256) void k_quality_cache(int32_t nt, const int32_t
   qualit[t] = quality_logs(t, tri, xy, logm);
 }
 
+// Data part of the vertex records (metric log, coordinates, metric) from
+// the caller's arrays, and the logs as a dense array for the quality
+// cache; replaces k_tensor_log for the sweeps.  Whole 128-byte records are
+// stored (the topology part read back), so no line is written in part.
+__global__ __launch_bounds__(256) void k_pack_records(int32_t nv, const double2* __restrict__ xy,
+                                                      const double* __restrict__ tensor,
+                                                      VRecord* __restrict__ rec,
+                                                      double4* __restrict__ logm) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  VRecord r;
+  r.topo = rec[i].topo;
+  r.t[0] = tensor[3 * i];
+  r.t[1] = tensor[3 * i + 1];
+  r.t[2] = tensor[3 * i + 2];
+  r.logm = tensor_log4(r.t[0], r.t[1], r.t[2]);
+  r.xy = xy[i];
+  rec[i] = r;
+  logm[i] = r.logm;
+}
+
+// Coordinates and metric back from the records into the caller's arrays.
+__global__ __launch_bounds__(256) void k_unpack_records(int32_t nv,
+                                                        const VRecord* __restrict__ rec,
+                                                        double2* __restrict__ xy,
+                                                        double* __restrict__ tensor) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  xy[i] = rec[i].xy;
+  tensor[3 * i] = rec[i].t[0];
+  tensor[3 * i + 1] = rec[i].t[1];
+  tensor[3 * i + 2] = rec[i].t[2];
+}
+
 __global__ __launch_bounds__(64) void k_smooth_phase(
-    int32_t count, const int32_t* __restrict__ order, double2* __restrict__ xy,
-    double* __restrict__ tensor, double4* __restrict__ logm, double* __restrict__ qualit,
-    const uint8_t* __restrict__ tags, const int32_t* __restrict__ v2t_off,
+    int32_t count, const int32_t* __restrict__ order, VRecord* __restrict__ rec,
+    double* __restrict__ qualit, const int32_t* __restrict__ v2t_off,
     const int32_t* __restrict__ v2t, const int32_t* __restrict__ star,
     const int32_t* __restrict__ v2v_off, const int32_t* __restrict__ v2v,
     int32_t* __restrict__ counters) {
   extern __shared__ double lds[];
   const int64_t idx = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   int commit = 0, task = 0;
-  if (idx < count) {
-    const int32_t i = order[idx];
-    // Smooth::preProcess (:134-138): boundary vertices are never moved
-    if (!(tags[i] & 4) && v2t_off[i + 1] > v2t_off[i]) {
-      task = 1;
-      commit = move_vertex(i, xy, tensor, logm, qualit, v2t_off, v2t, star, v2v_off, v2v,
-                           lds + threadIdx.x, blockDim.x);
-    }
-  }
+  if (idx < count)
+    commit = move_vertex_rec(order[idx], rec, qualit, v2t_off, v2t, star, v2v_off, v2v,
+                             lds + threadIdx.x, blockDim.x, &task);
   // counters are spread over kCounterSlots addresses: one hot address
   // serialises the atomics of every wave in L2
   const uint64_t mc = __ballot(commit), mt = __ballot(task);
@@ -97,14 +124,18 @@ static unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 // tri24 qualit16 colour-order4 read, xy16 M24 qualit16 written.
 constexpr double kBytesSweepPerV = 196.0;
 
-// Threads per block of the phase kernel and its LDS bytes: 4 doubles per
-// incident triangle per thread, at most 64 KiB per block.
-static void phase_shape(int max_deg, int* block, size_t* shmem) {
-  const int deg = max_deg > 0 ? max_deg : 1;
+// Threads per block of the phase kernel and its LDS bytes per block.  Per
+// thread: 3 doubles per incident triangle on the fast path (at most
+// kVtopoDeg triangles), 4 per incident triangle on the generic path, which
+// only a mesh with a vertex of more than kVtopoDeg incident triangles or
+// neighbours takes.  At most 64 KiB per block.
+static void phase_shape(const trn_mesh* m, int* block, size_t* shmem) {
+  const bool generic = m->max_deg_t > kVtopoDeg || m->max_deg_v > kVtopoDeg;
+  const int per_thread = generic ? 4 * m->max_deg_t : 3 * kVtopoDeg;
   int b = 64;
-  while (b > 16 && (size_t)b * deg * 4 * sizeof(double) > 65536) b >>= 1;
+  while (b > 16 && (size_t)b * per_thread * sizeof(double) > 65536) b >>= 1;
   *block = b;
-  *shmem = (size_t)b * deg * 4 * sizeof(double);
+  *shmem = (size_t)b * per_thread * sizeof(double);
 }
 
 static void tensor_logs(trn_ctx* ctx, const trn_mesh* m, const double* tensor, double4* logm) {
@@ -159,23 +190,26 @@ int trn_smooth_run(trn_ctx* ctx, const trn_mesh* mesh, double* xy_d, double* ten
     int32_t* counters = sc.carve<int32_t>(ncount);
     int32_t* totals = sc.carve<int32_t>(2 * (size_t)(sweeps + 1));
     double4* logm = sc.carve<double4>(m->nv);
+    VRecord* rec = m->vrec;
     memset_async(ctx, counters, 0, sizeof(int32_t) * ncount);
-    tensor_logs(ctx, m, tensor_d, logm);
+    launch(ctx, "pack_records", 256.0 * m->nv, k_pack_records, dim3(nblk(m->nv, 256)),
+           dim3(256), 0, m->nv, (const double2*)xy_d, (const double*)tensor_d, rec, logm);
     if (recompute_qualit) quality_cache_logs(ctx, m, xy_d, logm, qualit_d);
     int block;
     size_t shmem;
-    phase_shape(m->max_deg_t, &block, &shmem);
+    phase_shape(m, &block, &shmem);
     for (int s = 0; s < sweeps; ++s) {
       for (int c = 0; c < ncolors; ++c) {
         const int32_t count = class_off[c + 1] - class_off[c];
         launch(ctx, "smooth_phase", kBytesSweepPerV * count, k_smooth_phase,
-               dim3(nblk(count, block)), dim3(block), shmem, count, order_d + class_off[c],
-               (double2*)xy_d, tensor_d, logm, qualit_d, (const uint8_t*)m->tags,
-               (const int32_t*)m->v2t_off, (const int32_t*)m->v2t, (const int32_t*)m->star,
-               (const int32_t*)m->v2v_off, (const int32_t*)m->v2v,
+               dim3(nblk(count, block)), dim3(block), shmem, count, order_d + class_off[c], rec,
+               qualit_d, (const int32_t*)m->v2t_off, (const int32_t*)m->v2t,
+               (const int32_t*)m->star, (const int32_t*)m->v2v_off, (const int32_t*)m->v2v,
                counters + 2 * kCounterSlots * s);
       }
     }
+    launch(ctx, "unpack_records", 80.0 * m->nv, k_unpack_records, dim3(nblk(m->nv, 256)),
+           dim3(256), 0, m->nv, (const VRecord*)rec, (double2*)xy_d, tensor_d);
     if ((commits || tasks) && sweeps > 0) {
       launch(ctx, "counter_reduce", 0.0, k_counter_reduce, dim3(1), dim3(1024), 0, sweeps,
              (const int32_t*)counters, totals);

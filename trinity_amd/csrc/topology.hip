@@ -272,6 +272,45 @@ __global__ __launch_bounds__(64) void k_patch_size(int32_t nv, const int32_t* __
   size[i] = m;
 }
 
+// Rows of each patch slab: the largest patch among its kPatchSlab vertices
+// (one wave per slab).
+__global__ __launch_bounds__(kPatchSlab) void k_patch_slab_rows(int32_t nv,
+                                                               const int32_t* __restrict__ size,
+                                                               int32_t* __restrict__ rows) {
+  const int64_t i = (int64_t)blockIdx.x * kPatchSlab + threadIdx.x;
+  int m = i < nv ? size[i] : 0;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    const int o = __shfl_xor(m, d, 64);
+    m = o > m ? o : m;
+  }
+  if (threadIdx.x == 0) rows[blockIdx.x] = m;
+}
+
+// Stored patches: the ascending 2-ring element set of every vertex, as
+// k_patch_size builds it, written column-wise into its slab, padded with -1.
+__global__ __launch_bounds__(kPatchSlab) void k_patch_fill(
+    int32_t nv, const int32_t* __restrict__ v2t_off, const int32_t* __restrict__ v2t,
+    const int32_t* __restrict__ v2v_off, const int32_t* __restrict__ v2v, int cap,
+    const int32_t* __restrict__ row, int32_t* __restrict__ ell) {
+  extern __shared__ int lds[];
+  const int64_t i = (int64_t)blockIdx.x * kPatchSlab + threadIdx.x;
+  if (i >= nv) return;
+  int* s = lds + threadIdx.x;
+  int m = 0;
+  bool ok = true;
+  for (int32_t k = v2t_off[i]; k < v2t_off[i + 1]; ++k)
+    ok = ok && lds_sorted_insert(s, kPatchSlab, m, cap, v2t[k]);
+  for (int32_t a = v2v_off[i]; a < v2v_off[i + 1] && ok; ++a) {
+    const int32_t j = v2v[a];
+    for (int32_t k = v2t_off[j]; k < v2t_off[j + 1]; ++k)
+      ok = ok && lds_sorted_insert(s, kPatchSlab, m, cap, v2t[k]);
+  }
+  const int32_t r0 = row[blockIdx.x], w = row[blockIdx.x + 1] - r0;
+  int32_t* col = ell + (int64_t)r0 * kPatchSlab + threadIdx.x;
+  for (int e = 0; e < w; ++e) col[(int64_t)e * kPatchSlab] = e < m ? s[e * kPatchSlab] : -1;
+}
+
 // Star codes (see trn_mesh::star).  Corners are located in the sorted 1-ring
 // by binary search; every corner of an incident triangle is in the 1-ring
 // by construction of v2v.
@@ -301,6 +340,33 @@ __global__ void k_star(int32_t nv, const int32_t* __restrict__ tri,
     const int b = ring_index(ring, n, c[(r + 2) % 3]);
     star[k] = r | (a << 2) | (b << 10);
   }
+}
+
+// Compact topology (VTopo) of the vertex records from the CSR arrays and
+// star codes.
+__global__ __launch_bounds__(256) void k_vtopo(int32_t nv, const uint8_t* __restrict__ tags,
+                                               const int32_t* __restrict__ v2t_off,
+                                               const int32_t* __restrict__ v2t,
+                                               const int32_t* __restrict__ star,
+                                               const int32_t* __restrict__ v2v_off,
+                                               const int32_t* __restrict__ v2v,
+                                               VRecord* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const int32_t t0 = v2t_off[i], deg = v2t_off[i + 1] - t0;
+  const int32_t n0 = v2v_off[i], nb = v2v_off[i + 1] - n0;
+  VTopo r;
+  r.tag = tags[i];
+  r.meta = (deg <= kVtopoDeg && nb <= kVtopoDeg) ? (uint8_t)(deg | nb << 4) : kVtopoGeneric;
+  const bool fast = r.meta != kVtopoGeneric;
+#pragma unroll
+  for (int k = 0; k < kVtopoDeg; ++k) {
+    r.ring[k] = (fast && k < nb) ? v2v[n0 + k] : (int32_t)i;
+    r.tris[k] = (fast && k < deg) ? v2t[t0 + k] : 0;
+    const int32_t c = (fast && k < deg) ? star[t0 + k] : 0;
+    r.star[k] = (uint8_t)((c & 3) | (((c >> 2) & 7) << 2) | (((c >> 10) & 7) << 5));
+  }
+  out[i].topo = r;
 }
 
 static unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -415,6 +481,23 @@ int trn_mesh_create(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
       m->max_patch = h[1];
       m->patch_cap = m->max_patch > 0 ? m->max_patch : 1;
 
+      // 4b. stored patches (sliced ELL) for the recovery gathers
+      const int64_t nslab = (nv + kPatchSlab - 1) / kPatchSlab;
+      TRN_HIP(hipMalloc(&m->patch_n, sizeof(int32_t) * (size_t)nv));
+      TRN_HIP(hipMalloc(&m->patch_row, sizeof(int32_t) * (size_t)(nslab + 1)));
+      copy_async(ctx, m->patch_n, bound, sizeof(int32_t) * (size_t)nv);
+      launch(ctx, "topo_patch_rows", 0.0, k_patch_slab_rows, dim3((unsigned)nslab),
+             dim3(kPatchSlab), 0, nv, (const int32_t*)bound, m->patch_row);
+      exclusive_scan_i32(ctx, m->patch_row, m->patch_row, nslab, true, scan_tmp);
+      copy_async(ctx, h, m->patch_row + nslab, sizeof(int32_t));
+      sync(ctx);
+      const int64_t ell_n = (int64_t)h[0] * kPatchSlab;
+      TRN_HIP(hipMalloc(&m->patch_ell, sizeof(int32_t) * (size_t)(ell_n > 0 ? ell_n : 1)));
+      launch(ctx, "topo_patch_fill", 0.0, k_patch_fill, dim3((unsigned)nslab), dim3(kPatchSlab),
+             sizeof(int) * kPatchSlab * (size_t)m->patch_cap, nv, (const int32_t*)m->v2t_off,
+             (const int32_t*)m->v2t, (const int32_t*)m->v2v_off, (const int32_t*)m->v2v,
+             m->patch_cap, (const int32_t*)m->patch_row, m->patch_ell);
+
       // 5. star codes of the incident triangles (smoothing reads corners
       //    through the 1-ring instead of the triangle array)
       TRN_REQUIRE(m->max_deg_v < 256, TRN_ERR_CAPACITY, "1-ring larger than 255 vertices");
@@ -422,6 +505,11 @@ int trn_mesh_create(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
       launch(ctx, "topo_star", 0.0, k_star, dim3(blocks_for(nv, 256)), dim3(256), 0, nv,
              (const int32_t*)m->tri, (const int32_t*)m->v2t_off, (const int32_t*)m->v2t,
              (const int32_t*)m->v2v_off, (const int32_t*)m->v2v, m->star);
+      TRN_HIP(hipMalloc(&m->vrec, sizeof(VRecord) * (size_t)nv));
+      launch(ctx, "topo_vtopo", 0.0, k_vtopo, dim3(blocks_for(nv, 256)), dim3(256), 0, nv,
+             (const uint8_t*)m->tags, (const int32_t*)m->v2t_off, (const int32_t*)m->v2t,
+             (const int32_t*)m->star, (const int32_t*)m->v2v_off, (const int32_t*)m->v2v,
+             m->vrec);
 
       // 6. locality order of the colour classes
       if (xy_d) {
@@ -450,6 +538,10 @@ int trn_mesh_destroy(trn_mesh* m) {
     if (m->v2v) (void)hipFree(m->v2v);
     if (m->star) (void)hipFree(m->star);
     if (m->perm) (void)hipFree(m->perm);
+    if (m->vrec) (void)hipFree(m->vrec);
+    if (m->patch_n) (void)hipFree(m->patch_n);
+    if (m->patch_row) (void)hipFree(m->patch_row);
+    if (m->patch_ell) (void)hipFree(m->patch_ell);
     delete m;
   });
 }

@@ -71,6 +71,40 @@ struct trn_ctx {
   double* pinned_f64 = nullptr;
 };
 
+namespace trn {
+// Compact topology of one vertex for the smoothing sweeps (56 bytes): the
+// 1-ring and incident triangles of a vertex with at most kVtopoDeg of each,
+// in CSR order, and per incident triangle a star byte r | a << 2 | b << 5
+// (tri[t][r] = i, tri[t][(r+1)%3] = ring[a], tri[t][(r+2)%3] = ring[b]).
+// meta = deg | nb << 4, or kVtopoGeneric when the vertex has more than
+// kVtopoDeg incident triangles or neighbours (then the CSR arrays are used).
+constexpr int kVtopoDeg = 6;
+constexpr uint8_t kVtopoGeneric = 0xFF;
+struct VTopo {
+  int32_t ring[kVtopoDeg];
+  int32_t tris[kVtopoDeg];
+  uint8_t star[kVtopoDeg];
+  uint8_t meta;
+  uint8_t tag;
+};
+static_assert(sizeof(VTopo) == 56, "VTopo is 56 bytes");
+
+// Vertex record of the smoothing sweeps: one 128-byte line per vertex with
+// what a move reads of a vertex (metric log, coordinates, metric) and its
+// compact topology.  The topology is written when the mesh is created; each
+// smoothing call fills the first 72 bytes from the caller's arrays and
+// copies coordinates and metric back at its end.
+struct alignas(128) VRecord {
+  double4 logm;  // matrix log of the metric (numeric.cpp:115-154)
+  double2 xy;
+  double t[3];   // metric
+  VTopo topo;
+};
+static_assert(sizeof(VRecord) == 128, "VRecord is one 128-byte line");
+
+constexpr int kPatchSlab = 64;  // vertices per slab of the stored patches
+}  // namespace trn
+
 struct trn_mesh {
   trn_ctx* ctx = nullptr;
   int32_t nv = 0, nt = 0;
@@ -93,6 +127,17 @@ struct trn_mesh {
   // Vertices along a Z-curve of their coordinates (NULL: index order);
   // colour classes list their vertices in this order.
   int32_t* perm = nullptr;
+  // [nv] working copy of the smoothing sweeps (topology filled at
+  // creation); smoothing calls on one mesh must not overlap.
+  trn::VRecord* vrec = nullptr;
+  // 2-ring element patches (Mesh::getVicinity(i, 2), ascending) in sliced
+  // ELL form: vertices in slabs of kPatchSlab; entry e of the patch of
+  // vertex i = b * kPatchSlab + l is patch_ell[(patch_row[b] + e) *
+  // kPatchSlab + l], e < patch_n[i]; slab b has patch_row[b+1] -
+  // patch_row[b] rows (its largest patch), shorter patches padded with -1.
+  int32_t* patch_n = nullptr;    // [nv]
+  int32_t* patch_row = nullptr;  // [nslab + 1]
+  int32_t* patch_ell = nullptr;  // [patch_row[nslab] * kPatchSlab]
 };
 
 namespace trn {

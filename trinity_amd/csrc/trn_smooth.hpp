@@ -1,9 +1,18 @@
 // trn_smooth.hpp -- per-vertex body of the coloured smoothing
 // (Smooth::moveSmartLaplacian, src/smoothing.cpp:40-126), shared by the
 // per-colour kernels of smoothing.hip and the persistent batch kernel.
+//
+// The move reads, for the vertex and its 1-ring, coordinates, metric and
+// the cached matrix log of the metric.  Two layouts of that vertex data
+// share one move body through an accessor:
+//   SoaVerts   separate xy / metric / log arrays (the batch kernel);
+//   RecVerts   one 128-byte VRecord per vertex holding all three and the
+//              vertex's compact topology, so a neighbour costs one cache
+//              line instead of three or four (the per-colour sweeps).
 #pragma once
 
 #include <cfloat>
+#include <cstddef>
 
 #include "trn_api.hpp"
 #include "trn_numeric.hpp"
@@ -23,56 +32,159 @@ __device__ __forceinline__ double quality_logs(const int64_t t, const int32_t* _
   return quality_from_mean(A, B, C, j);
 }
 
-// Fast path of move_vertex for a vertex with at most kFastDeg incident
+static_assert(offsetof(VRecord, xy) == 32 && offsetof(VRecord, t) == 48 &&
+                  offsetof(VRecord, topo) == 72,
+              "VRecord layout");
+
+struct SoaVerts {
+  double2* __restrict__ xy;
+  double* __restrict__ tensor;
+  double4* __restrict__ logm;
+  __device__ __forceinline__ double2 pos(const int32_t j) const { return xy[j]; }
+  __device__ __forceinline__ void metric(const int32_t j, double T[3]) const {
+    const double* tj = tensor + 3 * (int64_t)j;
+    T[0] = tj[0];
+    T[1] = tj[1];
+    T[2] = tj[2];
+  }
+  __device__ __forceinline__ double4 log(const int32_t j) const { return logm[j]; }
+  __device__ __forceinline__ void commit(const int32_t i, const double px, const double py,
+                                         const double TN[3], const double4 LN) const {
+    xy[i] = make_double2(px, py);
+    tensor[3 * (int64_t)i] = TN[0];
+    tensor[3 * (int64_t)i + 1] = TN[1];
+    tensor[3 * (int64_t)i + 2] = TN[2];
+    logm[i] = LN;
+  }
+};
+
+struct RecVerts {
+  VRecord* __restrict__ rec;
+  __device__ __forceinline__ double2 pos(const int32_t j) const { return rec[j].xy; }
+  __device__ __forceinline__ void metric(const int32_t j, double T[3]) const {
+    T[0] = rec[j].t[0];
+    T[1] = rec[j].t[1];
+    T[2] = rec[j].t[2];
+  }
+  __device__ __forceinline__ double4 log(const int32_t j) const { return rec[j].logm; }
+  __device__ __forceinline__ void commit(const int32_t i, const double px, const double py,
+                                         const double TN[3], const double4 LN) const {
+    rec[i].xy = make_double2(px, py);
+    rec[i].t[0] = TN[0];
+    rec[i].t[1] = TN[1];
+    rec[i].t[2] = TN[2];
+    rec[i].logm = LN;
+  }
+};
+
+// Topology of a vertex with at most kVtopoDeg incident triangles and 1-ring
+// neighbours: ring entries, incident triangles, and per triangle the slot
+// of the vertex (rot) and the two other corners (ja, jb).  Every array is
+// indexed by unrolled constants only, so the struct lives in registers.
+struct FastTopo {
+  int deg, nb;
+  int32_t ring[kVtopoDeg], tris[kVtopoDeg], ja[kVtopoDeg], jb[kVtopoDeg];
+  int rot[kVtopoDeg];
+};
+
+// v[k] for a runtime k in [0, 6) as a select chain over named values (a
+// runtime-indexed array would be placed in scratch memory).
+__device__ __forceinline__ int32_t sel6(const int k, const int32_t v0, const int32_t v1,
+                                        const int32_t v2, const int32_t v3, const int32_t v4,
+                                        const int32_t v5) {
+  int32_t r = v0;
+  r = k == 1 ? v1 : r;
+  r = k == 2 ? v2 : r;
+  r = k == 3 ? v3 : r;
+  r = k == 4 ? v4 : r;
+  r = k == 5 ? v5 : r;
+  return r;
+}
+
+// FastTopo from the CSR arrays and the mesh's star codes
+// (r | a << 2 | b << 10, a and b 1-ring slots).
+__device__ __forceinline__ FastTopo fast_topo_csr(const int32_t i, const int32_t t0,
+                                                  const int deg, const int32_t n0, const int nb,
+                                                  const int32_t* __restrict__ v2t,
+                                                  const int32_t* __restrict__ star,
+                                                  const int32_t* __restrict__ v2v) {
+  FastTopo f;
+  f.deg = deg;
+  f.nb = nb;
+#pragma unroll
+  for (int k = 0; k < kVtopoDeg; ++k) {
+    f.ring[k] = k < nb ? v2v[n0 + k] : i;
+    if (k < deg) {
+      const int32_t ck = star[t0 + k];
+      f.tris[k] = v2t[t0 + k];
+      f.rot[k] = ck & 3;
+      f.ja[k] = v2v[n0 + ((ck >> 2) & 255)];
+      f.jb[k] = v2v[n0 + ((ck >> 10) & 255)];
+    } else {
+      f.tris[k] = 0;
+      f.rot[k] = 0;
+      f.ja[k] = f.jb[k] = i;
+    }
+  }
+  return f;
+}
+
+// FastTopo from a VTopo record (star bytes r | a << 2 | b << 5).
+__device__ __forceinline__ FastTopo fast_topo_rec(const int32_t i, const VTopo& v) {
+  FastTopo f;
+  f.deg = v.meta & 15;
+  f.nb = v.meta >> 4;
+#pragma unroll
+  for (int k = 0; k < kVtopoDeg; ++k) {
+    f.ring[k] = v.ring[k];
+    f.tris[k] = v.tris[k];
+  }
+#pragma unroll
+  for (int k = 0; k < kVtopoDeg; ++k) {
+    const int s = v.star[k];
+    f.rot[k] = s & 3;
+    f.ja[k] = sel6((s >> 2) & 7, f.ring[0], f.ring[1], f.ring[2], f.ring[3], f.ring[4],
+                   f.ring[5]);
+    f.jb[k] = sel6((s >> 5) & 7, f.ring[0], f.ring[1], f.ring[2], f.ring[3], f.ring[4],
+                   f.ring[5]);
+    if (k >= f.deg) f.ja[k] = f.jb[k] = i;
+  }
+  return f;
+}
+
+// Fast path of the move for a vertex with at most kVtopoDeg incident
 // triangles and 1-ring neighbours (every interior vertex of a triangulated
 // grid).  The generic loop issues its loads one dependent step at a time
-// (CSR entry, then the neighbour, then its data); here the CSR lines, the
-// corner ids of every incident triangle and then all neighbour data are
-// each loaded in one batch, the averaged tensor of every incident triangle
+// (CSR entry, then the neighbour, then its data); here all neighbour data
+// is loaded in one batch, the averaged tensor of every incident triangle
 // is formed once before the first try (it depends on tensors only, so the
 // value is the one the lazy form computes, and only triangles a try reaches
 // are used), and the corner coordinates stay in registers across the
-// tries.  Arithmetic and its order are those of move_vertex; every array is
-// indexed by unrolled constants so it lives in registers.
-constexpr int kFastDeg = 6;
-
-__device__ __forceinline__ int move_vertex_fast(
-    const int i, const int32_t t0, const int deg, const int32_t n0, const int nb,
-    double2* __restrict__ xy, double* __restrict__ tensor, double4* __restrict__ logm,
-    double* __restrict__ qualit, const int32_t* __restrict__ v2t,
-    const int32_t* __restrict__ star, const int32_t* __restrict__ v2v, double* lds,
-    const int stride) {
-  int32_t rot[kFastDeg], ja[kFastDeg], jb[kFastDeg];
+// tries.  Arithmetic and its order are those of move_vertex_generic.
+// lds holds the averaged tensors, per thread with stride `stride`.
+template <class V>
+__device__ __forceinline__ int move_vertex_fast(const int32_t i, const FastTopo& ft,
+                                                const double2 pini, const double T[3],
+                                                const V& v, double* __restrict__ qualit,
+                                                double* lds, const int stride) {
+  const int deg = ft.deg, nb = ft.nb;
   double q_min = DBL_MAX;  // smoothing.cpp:48-51
 #pragma unroll
-  for (int k = 0; k < kFastDeg; ++k) {
-    if (k < deg) {
-      const int32_t ck = star[t0 + k];
-      rot[k] = ck & 3;
-      ja[k] = v2v[n0 + ((ck >> 2) & 255)];
-      jb[k] = v2v[n0 + ((ck >> 10) & 255)];
-      q_min = dmin(q_min, qualit[v2t[t0 + k]]);
-    } else {
-      rot[k] = 0;
-      ja[k] = jb[k] = i;
-    }
-  }
+  for (int k = 0; k < kVtopoDeg; ++k)
+    if (k < deg) q_min = dmin(q_min, qualit[ft.tris[k]]);
 
-  const double2 pini = xy[i];
   const double P[2] = {pini.x, pini.y};
-  const double T[3] = {tensor[3 * (int64_t)i], tensor[3 * (int64_t)i + 1],
-                       tensor[3 * (int64_t)i + 2]};
   double opt0 = 0, opt1 = 0;
   double lm[4] = {0., 0., 0., 0.};
 #pragma unroll
-  for (int k = 0; k < kFastDeg; ++k) {  // smoothing.cpp:61-79, numeric.cpp:113-172
+  for (int k = 0; k < kVtopoDeg; ++k) {  // smoothing.cpp:61-79, numeric.cpp:113-172
     if (k < nb) {
-      const int32_t j = v2v[n0 + k];
-      const double2 pb = xy[j];
+      const int32_t j = ft.ring[k];
+      const double2 pb = v.pos(j);
       const double Pb[2] = {pb.x, pb.y};
-      const double* tb = tensor + 3 * (int64_t)j;
-      const double Tb[3] = {tb[0], tb[1], tb[2]};
-      const double4 L = logm[j];
+      double Tb[3];
+      v.metric(j, Tb);
+      const double4 L = v.log(j);
       const double len = riemann_dist(P, Pb, T, Tb);
       opt0 += P[0] + ((Pb[0] - P[0]) / len);
       opt1 += P[1] + ((Pb[1] - P[1]) / len);
@@ -88,15 +200,15 @@ __device__ __forceinline__ int move_vertex_fast(
   tensor_exp_mean(lm, nb, TN);  // smoothing.cpp:84
   const double4 LN = tensor_log4(TN[0], TN[1], TN[2]);
 
-  double2 pa[kFastDeg], pb[kFastDeg];
-  double* jm = lds;  // [kFastDeg][3]
+  double2 pa[kVtopoDeg], pb[kVtopoDeg];
+  double* jm = lds;  // [kVtopoDeg][3]
 #pragma unroll
-  for (int k = 0; k < kFastDeg; ++k) {
+  for (int k = 0; k < kVtopoDeg; ++k) {
     if (k < deg) {
-      pa[k] = xy[ja[k]];
-      pb[k] = xy[jb[k]];
-      const double4 La = logm[ja[k]], Lb = logm[jb[k]];
-      const int r = rot[k];
+      pa[k] = v.pos(ft.ja[k]);
+      pb[k] = v.pos(ft.jb[k]);
+      const double4 La = v.log(ft.ja[k]), Lb = v.log(ft.jb[k]);
+      const int r = ft.rot[k];
       const double4 L0 = r == 0 ? LN : (r == 1 ? Lb : La);
       const double4 L1 = r == 0 ? La : (r == 1 ? LN : Lb);
       const double4 L2 = r == 0 ? Lb : (r == 1 ? La : LN);
@@ -113,16 +225,17 @@ __device__ __forceinline__ int move_vertex_fast(
   int valid = 0;
   double w = 1;
   double px = P[0], py = P[1];
-  double qs[kFastDeg];
+  double qs[kVtopoDeg];
   for (int retry = 0; retry < 5; ++retry) {  // smoothing.cpp:96-114
     px = (1 - w) * P[0] + w * opt0;
     py = (1 - w) * P[1] + w * opt1;
     valid = 1;
 #pragma unroll
-    for (int k = 0; k < kFastDeg; ++k) {
+    for (int k = 0; k < kVtopoDeg; ++k) {
       if (valid && k < deg) {
-        const int r = rot[k];
+        const int r = ft.rot[k];
         const double2 a = pa[k], b = pb[k];
+        // corners in triangle order: v[r] = i, v[r+1] = a, v[r+2] = b
         const double pp[3][2] = {
             {r == 0 ? px : (r == 1 ? b.x : a.x), r == 0 ? py : (r == 1 ? b.y : a.y)},
             {r == 0 ? a.x : (r == 1 ? px : b.x), r == 0 ? a.y : (r == 1 ? py : b.y)},
@@ -141,32 +254,25 @@ __device__ __forceinline__ int move_vertex_fast(
   }
   if (valid) {  // smoothing.cpp:116-123
 #pragma unroll
-    for (int k = 0; k < kFastDeg; ++k)
-      if (k < deg) qualit[v2t[t0 + k]] = qs[k];
-    xy[i] = make_double2(px, py);
-    tensor[3 * (int64_t)i] = TN[0];
-    tensor[3 * (int64_t)i + 1] = TN[1];
-    tensor[3 * (int64_t)i + 2] = TN[2];
-    logm[i] = LN;
+    for (int k = 0; k < kVtopoDeg; ++k)
+      if (k < deg) qualit[ft.tris[k]] = qs[k];
+    v.commit(i, px, py, TN, LN);
   }
   return valid;
 }
 
-// Smooth::moveSmartLaplacian (src/smoothing.cpp:40-126) for vertex i.
-// lds holds, per thread (stride = blockDim.x), the averaged tensor (3
-// doubles) and the quality (1 double) of each incident triangle.  The
-// corners of incident triangle k are taken in the reference's vertex order
-// (the log-Euclidean mean accumulates in that order).
-__device__ __forceinline__ int move_vertex(const int i, double2* __restrict__ xy, double* __restrict__ tensor,
-                           double4* __restrict__ logm, double* __restrict__ qualit,
-                           const int32_t* __restrict__ v2t_off, const int32_t* __restrict__ v2t,
-                           const int32_t* __restrict__ star, const int32_t* __restrict__ v2v_off,
-                           const int32_t* __restrict__ v2v, double* lds, const int stride) {
-  const int32_t t0 = v2t_off[i], deg = v2t_off[i + 1] - t0;
-  const int32_t n0 = v2v_off[i], nb = v2v_off[i + 1] - n0;
-  if (deg <= kFastDeg && nb <= kFastDeg)
-    return move_vertex_fast(i, t0, deg, n0, nb, xy, tensor, logm, qualit, v2t, star, v2v, lds,
-                            stride);
+// Smooth::moveSmartLaplacian (src/smoothing.cpp:40-126) for vertex i of any
+// degree, topology from the CSR arrays.  lds holds, per thread (stride =
+// blockDim.x), the averaged tensor (3 doubles) and the quality (1 double) of
+// each incident triangle.  The corners of incident triangle k are taken in
+// the reference's vertex order (the log-Euclidean mean accumulates in that
+// order).
+template <class V>
+__device__ __forceinline__ int move_vertex_generic(
+    const int32_t i, const int32_t t0, const int deg, const int32_t n0, const int nb,
+    const double2 pini, const double T[3], const V& v, double* __restrict__ qualit,
+    const int32_t* __restrict__ v2t, const int32_t* __restrict__ star,
+    const int32_t* __restrict__ v2v, double* lds, const int stride) {
   const int32_t* ring = v2v + n0;
   double* jm = lds;                      // [deg][3]
   double* qs = lds + 3 * deg * stride;   // [deg]
@@ -174,22 +280,19 @@ __device__ __forceinline__ int move_vertex(const int i, double2* __restrict__ xy
   double q_min = DBL_MAX;  // smoothing.cpp:48-51
   for (int k = 0; k < deg; ++k) q_min = dmin(q_min, qualit[v2t[t0 + k]]);
 
-  const double2 pini = xy[i];
   const double P[2] = {pini.x, pini.y};
-  const double T[3] = {tensor[3 * (int64_t)i], tensor[3 * (int64_t)i + 1],
-                       tensor[3 * (int64_t)i + 2]};
   double opt0 = 0, opt1 = 0;
   double lm[4] = {0., 0., 0., 0.};
   for (int k = 0; k < nb; ++k) {  // smoothing.cpp:61-79, numeric.cpp:113-172
     const int32_t j = ring[k];
-    const double2 pj = xy[j];
+    const double2 pj = v.pos(j);
     const double Pb[2] = {pj.x, pj.y};
-    const double* tb = tensor + 3 * (int64_t)j;
-    const double Tb[3] = {tb[0], tb[1], tb[2]};
+    double Tb[3];
+    v.metric(j, Tb);
     const double len = riemann_dist(P, Pb, T, Tb);
     opt0 += P[0] + ((Pb[0] - P[0]) / len);
     opt1 += P[1] + ((Pb[1] - P[1]) / len);
-    const double4 L = logm[j];
+    const double4 L = v.log(j);
     lm[0] += L.x;
     lm[1] += L.y;
     lm[2] += L.z;
@@ -212,7 +315,7 @@ __device__ __forceinline__ int move_vertex(const int i, double2* __restrict__ xy
       const int32_t code = star[t0 + k];
       const int r = code & 3;
       const int32_t ja = ring[(code >> 2) & 255], jb = ring[(code >> 10) & 255];
-      const double2 pa = xy[ja], pb = xy[jb];
+      const double2 pa = v.pos(ja), pb = v.pos(jb);
       // corners in triangle order: v[r] = i, v[r+1] = a, v[r+2] = b
       // (selects, not a runtime-indexed array, to stay in registers)
       const double pp[3][2] = {
@@ -223,7 +326,7 @@ __device__ __forceinline__ int move_vertex(const int i, double2* __restrict__ xy
       if (ccw(pp[0], pp[1], pp[2])) {
         double J[3];
         if (k >= nmean) {  // first try to reach triangle k: average once
-          const double4 La = logm[ja], Lb = logm[jb];
+          const double4 La = v.log(ja), Lb = v.log(jb);
           const double4 L0 = r == 0 ? LN : (r == 1 ? Lb : La);
           const double4 L1 = r == 0 ? La : (r == 1 ? LN : Lb);
           const double4 L2 = r == 0 ? Lb : (r == 1 ? La : LN);
@@ -247,13 +350,64 @@ __device__ __forceinline__ int move_vertex(const int i, double2* __restrict__ xy
   }
   if (valid) {  // smoothing.cpp:116-123
     for (int k = 0; k < deg; ++k) qualit[v2t[t0 + k]] = qs[k * stride];
-    xy[i] = make_double2(px, py);
-    tensor[3 * (int64_t)i] = TN[0];
-    tensor[3 * (int64_t)i + 1] = TN[1];
-    tensor[3 * (int64_t)i + 2] = TN[2];
-    logm[i] = LN;
+    v.commit(i, px, py, TN, LN);
   }
   return valid;
+}
+
+// The move on separate xy / metric / log arrays (batch kernel).
+__device__ __forceinline__ int move_vertex(const int i, double2* __restrict__ xy,
+                                           double* __restrict__ tensor,
+                                           double4* __restrict__ logm,
+                                           double* __restrict__ qualit,
+                                           const int32_t* __restrict__ v2t_off,
+                                           const int32_t* __restrict__ v2t,
+                                           const int32_t* __restrict__ star,
+                                           const int32_t* __restrict__ v2v_off,
+                                           const int32_t* __restrict__ v2v, double* lds,
+                                           const int stride) {
+  const SoaVerts v{xy, tensor, logm};
+  const int32_t t0 = v2t_off[i], deg = v2t_off[i + 1] - t0;
+  const int32_t n0 = v2v_off[i], nb = v2v_off[i + 1] - n0;
+  const double2 pini = xy[i];
+  double T[3];
+  v.metric(i, T);
+  if (deg <= kVtopoDeg && nb <= kVtopoDeg)
+    return move_vertex_fast(i, fast_topo_csr(i, t0, deg, n0, nb, v2t, star, v2v), pini, T, v,
+                            qualit, lds, stride);
+  return move_vertex_generic(i, t0, deg, n0, nb, pini, T, v, qualit, v2t, star, v2v, lds,
+                             stride);
+}
+
+// The move on vertex records (per-colour sweeps).  Boundary vertices (tag
+// bit 2) and vertices without incident triangles are not moved
+// (Smooth::preProcess, src/smoothing.cpp:134-138).  *task = 1 if the
+// vertex is a smoothing task; returns 1 if the move was committed.
+__device__ __forceinline__ int move_vertex_rec(const int32_t i, VRecord* __restrict__ rec,
+                                               double* __restrict__ qualit,
+                                               const int32_t* __restrict__ v2t_off,
+                                               const int32_t* __restrict__ v2t,
+                                               const int32_t* __restrict__ star,
+                                               const int32_t* __restrict__ v2v_off,
+                                               const int32_t* __restrict__ v2v, double* lds,
+                                               const int stride, int* task) {
+  const RecVerts v{rec};
+  const double2 pini = rec[i].xy;
+  const double T[3] = {rec[i].t[0], rec[i].t[1], rec[i].t[2]};
+  const VTopo topo = rec[i].topo;
+  *task = 0;
+  if (topo.tag & 4) return 0;
+  if (topo.meta != kVtopoGeneric) {
+    if ((topo.meta & 15) == 0) return 0;
+    *task = 1;
+    return move_vertex_fast(i, fast_topo_rec(i, topo), pini, T, v, qualit, lds, stride);
+  }
+  const int32_t t0 = v2t_off[i], deg = v2t_off[i + 1] - t0;
+  const int32_t n0 = v2v_off[i], nb = v2v_off[i + 1] - n0;
+  if (deg == 0) return 0;
+  *task = 1;
+  return move_vertex_generic(i, t0, deg, n0, nb, pini, T, v, qualit, v2t, star, v2v, lds,
+                             stride);
 }
 
 }  // namespace trn

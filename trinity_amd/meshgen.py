@@ -44,8 +44,10 @@ class Mesh:
         return int(self.tri.shape[0])
 
 
-def grid_mesh(n: int, jitter: float = 0.0, seed: int = 1234) -> Mesh:
-    """n x n cell grid on [0,1]^2 (V=(n+1)^2, T=2n^2), optional jitter."""
+def grid_mesh(n: int, jitter: float = 0.0, seed: int = 1234, flip: float = 0.0) -> Mesh:
+    """n x n cell grid on [0,1]^2 (V=(n+1)^2, T=2n^2), optional jitter.
+    flip: fraction of cells (random by seed) split along the other diagonal,
+    which gives interior vertex degrees 4..8 instead of 6 everywhere."""
     if n < 1:
         raise ValueError("n must be >= 1")
     h = 1.0 / n
@@ -78,9 +80,18 @@ def grid_mesh(n: int, jitter: float = 0.0, seed: int = 1234) -> Mesh:
     tri[1::2, 0] = v00
     tri[1::2, 1] = v11
     tri[1::2, 2] = v01
+    if flip > 0.0:
+        rng = np.random.default_rng(seed + 7)
+        sel = np.flatnonzero(rng.random(n * n) < flip)
+        tri[2 * sel, 0] = v00[sel]
+        tri[2 * sel, 1] = v10[sel]
+        tri[2 * sel, 2] = v01[sel]
+        tri[2 * sel + 1, 0] = v10[sel]
+        tri[2 * sel + 1, 1] = v11[sel]
+        tri[2 * sel + 1, 2] = v01[sel]
     xy = np.stack([x, y], axis=1)
     return Mesh(xy=np.ascontiguousarray(xy), tri=tri, tags=tags,
-                meta={"n": n, "jitter": jitter, "seed": seed})
+                meta={"n": n, "jitter": jitter, "seed": seed, "flip": flip})
 
 
 # --------------------------------------------------------------------------
@@ -142,6 +153,8 @@ def workload(name: str, seed: int = 1234) -> Mesh:
     jitter2048   : 2048^2 jittered grid, 3 random fronts    (4,198,401 V)
     jitter1024   : 1024^2 jittered grid, tanh front         (1,050,625 V)
     jitter4096   : 4096^2 jittered grid, boundary layer     (16,785,409 V)
+    flipN        : N^2 jittered grid, half the cells split along the other
+                   diagonal (vertex degrees 4..8), 3 random fronts
     Other sizes: 'jitterN' / 'uniformN' with the field picked as above
     ('fronts' for jittered, 'tanh' for uniform) or suffixes ':tanh',
     ':fronts', ':fronts+quad', ':layer', ':linear', ':quadratic'.
@@ -155,6 +168,10 @@ def workload(name: str, seed: int = 1234) -> Mesh:
         n = int(base[len("jitter"):])
         m = grid_mesh(n, 0.25, seed)
         default = {1024: "tanh", 4096: "layer"}.get(n, "fronts")
+    elif base.startswith("flip"):
+        n = int(base[len("flip"):])
+        m = grid_mesh(n, 0.25, seed, flip=0.5)
+        default = "fronts"
     else:
         raise ValueError(f"unknown workload {name!r}")
     fld = fld or default
