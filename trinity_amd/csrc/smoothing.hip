@@ -125,13 +125,13 @@ static unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 constexpr double kBytesSweepPerV = 196.0;
 
 // Threads per block of the phase kernel and its LDS bytes per block.  Per
-// thread: 3 doubles per incident triangle on the fast path (at most
-// kVtopoDeg triangles), 4 per incident triangle on the generic path, which
-// only a mesh with a vertex of more than kVtopoDeg incident triangles or
-// neighbours takes.  At most 64 KiB per block.
+// thread: 4 doubles per incident triangle, for at most kVtopoDeg triangles
+// on the fast path or for the largest degree when the mesh has a vertex of
+// more than kVtopoDeg incident triangles or neighbours (generic path).  At
+// most 64 KiB per block.
 static void phase_shape(const trn_mesh* m, int* block, size_t* shmem) {
   const bool generic = m->max_deg_t > kVtopoDeg || m->max_deg_v > kVtopoDeg;
-  const int per_thread = generic ? 4 * m->max_deg_t : 3 * kVtopoDeg;
+  const int per_thread = 4 * (generic ? m->max_deg_t : kVtopoDeg);
   int b = 64;
   while (b > 16 && (size_t)b * per_thread * sizeof(double) > 65536) b >>= 1;
   *block = b;

@@ -130,12 +130,27 @@ __device__ __forceinline__ double cube_rn(const double x) {
   return q + (f + e * x);
 }
 
-// Metric-space quality from the averaged tensor j (numeric.cpp:275-305).
+// min(perim / 3, 3 / perim) as numeric.cpp:302 takes it, with one division:
+// rounding is monotonic, so for perim < 3 the rounded perim / 3 is at most
+// 1 and the rounded 3 / perim at least 1 (the reverse for perim > 3), and
+// equal roundings are the same value.  0, inf and NaN give the same result
+// as the two-division form.
+__device__ __forceinline__ double perim_mean(const double perim) {
+  return perim < 3 ? perim / 3 : 3 / perim;
+}
+
+// sqrt(det j) of an averaged tensor, the factor numeric.cpp:305 applies.
+__device__ __forceinline__ double mean_sqrt_det(const double j[3]) {
+  return __builtin_sqrt(j[0] * j[2] - j[1] * j[1]);
+}
+
+// Metric-space quality from the averaged tensor j and its sqrt(det j)
+// (numeric.cpp:275-305).
 __device__ __forceinline__ double quality_from_mean(const double* pa,
                                                     const double* pb,
                                                     const double* pc,
-                                                    const double j[3]) {
-  const double det = j[0] * j[2] - j[1] * j[1];
+                                                    const double j[3],
+                                                    const double sdet) {
   double coef[6];
   coef[0] = pb[0] - pa[0];
   coef[1] = pb[1] - pa[1];
@@ -156,9 +171,16 @@ __device__ __forceinline__ double quality_from_mean(const double* pa,
   const double a2 = pb[1] - pa[1];
   const double a3 = pc[1] - pa[1];
   const double area = 0.5 * fabs(a0 * a3 - a1 * a2);
-  const double mean = dmin(perim / 3, 3 / perim);
+  const double mean = perim_mean(perim);
   const double norm = (12 * 1.7320508075688772) * cube_rn(mean * (2. - mean));
-  return area * norm * __builtin_sqrt(det) / (perim * perim);
+  return area * norm * sdet / (perim * perim);
+}
+
+__device__ __forceinline__ double quality_from_mean(const double* pa,
+                                                    const double* pb,
+                                                    const double* pc,
+                                                    const double j[3]) {
+  return quality_from_mean(pa, pb, pc, j, mean_sqrt_det(j));
 }
 
 // Log-Euclidean mean of three tensors from their cached matrix logarithms

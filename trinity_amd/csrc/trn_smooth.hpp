@@ -161,7 +161,9 @@ __device__ __forceinline__ FastTopo fast_topo_rec(const int32_t i, const VTopo& 
 // value is the one the lazy form computes, and only triangles a try reaches
 // are used), and the corner coordinates stay in registers across the
 // tries.  Arithmetic and its order are those of move_vertex_generic.
-// lds holds the averaged tensors, per thread with stride `stride`.
+// lds holds, per thread with stride `stride`, the averaged tensor of each
+// incident triangle (3 doubles) and its sqrt(det) (1 double): 4 *
+// kVtopoDeg doubles.
 template <class V>
 __device__ __forceinline__ int move_vertex_fast(const int32_t i, const FastTopo& ft,
                                                 const double2 pini, const double T[3],
@@ -201,7 +203,7 @@ __device__ __forceinline__ int move_vertex_fast(const int32_t i, const FastTopo&
   const double4 LN = tensor_log4(TN[0], TN[1], TN[2]);
 
   double2 pa[kVtopoDeg], pb[kVtopoDeg];
-  double* jm = lds;  // [kVtopoDeg][3]
+  double* jm = lds;  // [kVtopoDeg][3], then [kVtopoDeg] sqrt(det)
 #pragma unroll
   for (int k = 0; k < kVtopoDeg; ++k) {
     if (k < deg) {
@@ -217,6 +219,7 @@ __device__ __forceinline__ int move_vertex_fast(const int32_t i, const FastTopo&
       jm[(3 * k) * stride] = J[0];
       jm[(3 * k + 1) * stride] = J[1];
       jm[(3 * k + 2) * stride] = J[2];
+      jm[(3 * kVtopoDeg + k) * stride] = mean_sqrt_det(J);
     } else {
       pa[k] = pb[k] = pini;
     }
@@ -244,7 +247,7 @@ __device__ __forceinline__ int move_vertex_fast(const int32_t i, const FastTopo&
         if (ccw(pp[0], pp[1], pp[2])) {
           const double J[3] = {jm[(3 * k) * stride], jm[(3 * k + 1) * stride],
                                jm[(3 * k + 2) * stride]};
-          qs[k] = quality_from_mean(pp[0], pp[1], pp[2], J);
+          qs[k] = quality_from_mean(pp[0], pp[1], pp[2], J, jm[(3 * kVtopoDeg + k) * stride]);
           valid = qs[k] > q_min;
         }
       }
