@@ -234,7 +234,8 @@ def test_quality_cache_matches_oracle(ctx):
 
 
 @pytest.mark.parametrize("wl,p,sweeps", [("jitter200:fronts", 2, 3), ("uniform256:tanh", 2, 1),
-                                         ("jitter160:layer", 4, 2), ("flip160:fronts", 2, 3)])
+                                         ("jitter160:layer", 4, 2), ("flip160:fronts", 2, 3),
+                                         ("jitter128:fronts+quad", 2, 8)])
 def test_smoothing_with_gpu_colouring_matches_oracle(ctx, wl, p, sweeps):
     m = mg.workload(wl)
     st = oracle.metric_stages(m.xy, m.tri, m.solut, 0.8, p)

@@ -77,6 +77,18 @@ void timing_end(trn_ctx* ctx, int stat, hipEvent_t start, hipEvent_t stop,
   if (ctx->pending.size() > 4096) flush_timing(ctx);
 }
 
+unsigned long long* timing_units(trn_ctx* ctx, const char* name, double bytes_per_unit) {
+  if (!ctx->timing) return nullptr;
+  const int st = stat_index(ctx, name);
+  if (st >= kTimingUnitSlots) return nullptr;
+  if (!ctx->unit_dev) {
+    TRN_HIP(hipMalloc(&ctx->unit_dev, sizeof(unsigned long long) * kTimingUnitSlots));
+    TRN_HIP(hipMemset(ctx->unit_dev, 0, sizeof(unsigned long long) * kTimingUnitSlots));
+  }
+  ctx->stats[st].bytes_per_unit = bytes_per_unit;
+  return ctx->unit_dev + st;
+}
+
 void flush_timing(trn_ctx* ctx) {
   if (ctx->pending.empty()) return;
   TRN_HIP(hipStreamSynchronize(ctx->stream));
@@ -88,6 +100,13 @@ void flush_timing(trn_ctx* ctx) {
     ctx->event_pool.push_back(p.stop);
   }
   ctx->pending.clear();
+  if (ctx->unit_dev) {
+    unsigned long long units[kTimingUnitSlots];
+    TRN_HIP(hipMemcpy(units, ctx->unit_dev, sizeof(units), hipMemcpyDeviceToHost));
+    TRN_HIP(hipMemset(ctx->unit_dev, 0, sizeof(units)));
+    for (size_t st = 0; st < ctx->stats.size() && st < (size_t)kTimingUnitSlots; ++st)
+      ctx->stats[st].bytes += (double)units[st] * ctx->stats[st].bytes_per_unit;
+  }
 }
 
 Scratch::Scratch(trn_ctx* ctx, size_t total) : ctx_(ctx) {
@@ -184,6 +203,7 @@ int trn_ctx_destroy(trn_ctx* ctx) {
     if (ctx->ws.base) (void)hipFree(ctx->ws.base);
     if (ctx->pinned_i32) (void)hipHostFree(ctx->pinned_i32);
     if (ctx->pinned_f64) (void)hipHostFree(ctx->pinned_f64);
+    if (ctx->unit_dev) (void)hipFree(ctx->unit_dev);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
   });

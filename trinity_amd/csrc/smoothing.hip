@@ -25,6 +25,8 @@
 //                    vertex's slot in the triangle and the 1-ring entries of
 //                    the two other corners), so the kernel reads no
 //                    triangle array.
+#include <algorithm>
+
 #include "trn_smooth.hpp"
 
 namespace trn {
@@ -82,17 +84,34 @@ __global__ __launch_bounds__(256) void k_unpack_records(int32_t nv,
   tensor[3 * i + 2] = rec[i].t[2];
 }
 
+// One colour phase.  count_d == nullptr: `order` lists `count` vertices
+// (the whole class); otherwise it is a live list of *count_d <= count
+// vertices written by k_class_live, the grid is sized for `count`, and the
+// blocks past the live ones exit at once while the others take XCD-chunked
+// ranges of the live blocks.
 __global__ __launch_bounds__(64) void k_smooth_phase(
-    int32_t count, const int32_t* __restrict__ order, VRecord* __restrict__ rec,
-    double* __restrict__ qualit, const int32_t* __restrict__ v2t_off,
-    const int32_t* __restrict__ v2t, const int32_t* __restrict__ star,
-    const int32_t* __restrict__ v2v_off, const int32_t* __restrict__ v2v,
-    int32_t* __restrict__ counters) {
+    int32_t count, const int32_t* __restrict__ count_d, const int32_t* __restrict__ order,
+    VRecord* __restrict__ rec, double* __restrict__ qualit, uint8_t* __restrict__ stale,
+    const int32_t* __restrict__ v2t_off, const int32_t* __restrict__ v2t,
+    const int32_t* __restrict__ star, const int32_t* __restrict__ v2v_off,
+    const int32_t* __restrict__ v2v, int32_t* __restrict__ counters,
+    unsigned long long* __restrict__ units) {
   extern __shared__ double lds[];
-  const int64_t idx = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  int32_t n = count;
+  uint32_t lb;
+  if (count_d) {
+    n = *count_d;
+    if (units && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(units, (unsigned long long)n);
+    const uint32_t live_blocks = (uint32_t)((n + (int32_t)blockDim.x - 1) / (int32_t)blockDim.x);
+    if (blockIdx.x >= live_blocks) return;
+    lb = xcd_block_of(blockIdx.x, live_blocks);
+  } else {
+    lb = xcd_block();
+  }
+  const int64_t idx = (int64_t)lb * blockDim.x + threadIdx.x;
   int commit = 0, task = 0;
-  if (idx < count)
-    commit = move_vertex_rec(order[idx], rec, qualit, v2t_off, v2t, star, v2v_off, v2v,
+  if (idx < n)
+    commit = move_vertex_rec(order[idx], rec, qualit, stale, v2t_off, v2t, star, v2v_off, v2v,
                              lds + threadIdx.x, blockDim.x, &task);
   // counters are spread over kCounterSlots addresses: one hot address
   // serialises the atomics of every wave in L2
@@ -102,6 +121,50 @@ __global__ __launch_bounds__(64) void k_smooth_phase(
     if (mc) atomicAdd(&slot[0], (int32_t)__popcll(mc));
     if (mt) atomicAdd(&slot[1], (int32_t)__popcll(mt));
   }
+}
+
+// Live list of a colour class: the vertices not marked stale, appended one
+// block of kLiveB class entries at a time with one atomic per block (a
+// block's vertices keep their class order, so each phase block still covers
+// a compact region; the order of the blocks does not change the result, a
+// class being an independent set).  Stale vertices are counted as tasks.
+constexpr int kLiveB = 1024;
+
+__global__ __launch_bounds__(kLiveB) void k_class_live(int32_t count,
+                                                       const int32_t* __restrict__ order,
+                                                       const uint8_t* __restrict__ stale,
+                                                       int32_t* __restrict__ live,
+                                                       int32_t* __restrict__ live_count,
+                                                       int32_t* __restrict__ counters) {
+  __shared__ int32_t wave_live[kLiveB / 64], wave_skip[kLiveB / 64], base;
+  const int64_t idx = (int64_t)xcd_block() * kLiveB + threadIdx.x;
+  int32_t i = 0;
+  bool is_live = false, skipped = false;
+  if (idx < count) {
+    i = order[idx];
+    skipped = stale[i] != 0;
+    is_live = !skipped;
+  }
+  const uint64_t ml = __ballot(is_live), ms = __ballot(skipped);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    wave_live[wave] = (int32_t)__popcll(ml);
+    wave_skip[wave] = (int32_t)__popcll(ms);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t nl = 0, nsk = 0;
+    for (int w = 0; w < kLiveB / 64; ++w) {
+      const int32_t c = wave_live[w];
+      wave_live[w] = nl;  // exclusive prefix
+      nl += c;
+      nsk += wave_skip[w];
+    }
+    base = nl ? atomicAdd(live_count, nl) : 0;
+    if (nsk) atomicAdd(&counters[2 * (blockIdx.x & (kCounterSlots - 1)) + 1], nsk);
+  }
+  __syncthreads();
+  if (is_live) live[base + wave_live[wave] + (int32_t)__popcll(ml & ((1ull << lane) - 1))] = i;
 }
 
 // counters[s][slot][2] -> out[s][2]
@@ -184,14 +247,24 @@ int trn_smooth_run(trn_ctx* ctx, const trn_mesh* mesh, double* xy_d, double* ten
                   TRN_ERR_ARG, "class offsets out of range");
     const trn_mesh* m = mesh;
     const size_t ncount = 2 * (size_t)kCounterSlots * (size_t)(sweeps + 1);
-    Scratch sc(ctx, scratch_bytes(sizeof(int32_t) * ncount) +
+    const size_t nlive = (size_t)(sweeps + 1) * (size_t)(ncolors + 1);
+    int32_t max_class = 1;
+    for (int c = 0; c < ncolors; ++c)
+      max_class = std::max(max_class, class_off[c + 1] - class_off[c]);
+    Scratch sc(ctx, scratch_bytes(sizeof(int32_t) * (ncount + nlive)) +
                         scratch_bytes(sizeof(int32_t) * 2 * (size_t)(sweeps + 1)) +
-                        scratch_bytes(sizeof(double4) * (size_t)m->nv));
-    int32_t* counters = sc.carve<int32_t>(ncount);
+                        scratch_bytes(sizeof(double4) * (size_t)m->nv) +
+                        scratch_bytes((size_t)m->nv) +
+                        scratch_bytes(sizeof(int32_t) * (size_t)max_class));
+    int32_t* counters = sc.carve<int32_t>(ncount + nlive);
+    int32_t* live_count = counters + ncount;
     int32_t* totals = sc.carve<int32_t>(2 * (size_t)(sweeps + 1));
     double4* logm = sc.carve<double4>(m->nv);
+    uint8_t* stale = sc.carve<uint8_t>(m->nv);
+    int32_t* live = sc.carve<int32_t>((size_t)max_class);
     VRecord* rec = m->vrec;
-    memset_async(ctx, counters, 0, sizeof(int32_t) * ncount);
+    memset_async(ctx, counters, 0, sizeof(int32_t) * (ncount + nlive));
+    memset_async(ctx, stale, 0, (size_t)m->nv);
     launch(ctx, "pack_records", 256.0 * m->nv, k_pack_records, dim3(nblk(m->nv, 256)),
            dim3(256), 0, m->nv, (const double2*)xy_d, (const double*)tensor_d, rec, logm);
     if (recompute_qualit) quality_cache_logs(ctx, m, xy_d, logm, qualit_d);
@@ -199,13 +272,28 @@ int trn_smooth_run(trn_ctx* ctx, const trn_mesh* mesh, double* xy_d, double* ten
     size_t shmem;
     phase_shape(m, &block, &shmem);
     for (int s = 0; s < sweeps; ++s) {
+      int32_t* cs = counters + 2 * kCounterSlots * s;
       for (int c = 0; c < ncolors; ++c) {
         const int32_t count = class_off[c + 1] - class_off[c];
-        launch(ctx, "smooth_phase", kBytesSweepPerV * count, k_smooth_phase,
-               dim3(nblk(count, block)), dim3(block), shmem, count, order_d + class_off[c], rec,
-               qualit_d, (const int32_t*)m->v2t_off, (const int32_t*)m->v2t,
-               (const int32_t*)m->star, (const int32_t*)m->v2v_off, (const int32_t*)m->v2v,
-               counters + 2 * kCounterSlots * s);
+        if (count == 0) continue;
+        const int32_t* list = order_d + class_off[c];
+        const int32_t* count_d = nullptr;
+        if (s > 0) {  // nothing is stale before the first sweep
+          int32_t* lc = live_count + (size_t)s * (ncolors + 1) + c;
+          launch(ctx, "class_live", 5.0 * count, k_class_live, dim3(nblk(count, kLiveB)),
+                 dim3(kLiveB), 0, count, list, (const uint8_t*)stale, live, lc, cs);
+          list = live;
+          count_d = lc;
+        }
+        // algorithmic bytes: per moved vertex; a live list's length is
+        // known on the device only, so its launch reports it as work units
+        unsigned long long* units =
+            count_d ? timing_units(ctx, "smooth_phase", kBytesSweepPerV) : nullptr;
+        launch(ctx, "smooth_phase", count_d ? 0.0 : kBytesSweepPerV * count, k_smooth_phase,
+               dim3(nblk(count, block)), dim3(block), shmem, count, count_d, list, rec, qualit_d,
+               stale, (const int32_t*)m->v2t_off, (const int32_t*)m->v2t,
+               (const int32_t*)m->star, (const int32_t*)m->v2v_off, (const int32_t*)m->v2v, cs,
+               units);
       }
     }
     launch(ctx, "unpack_records", 80.0 * m->nv, k_unpack_records, dim3(nblk(m->nv, 256)),

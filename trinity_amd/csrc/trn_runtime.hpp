@@ -43,6 +43,7 @@ struct KernelStat {
   double total_ms = 0.0;
   int64_t launches = 0;
   double bytes = 0.0;  // algorithmic bytes per launch x launches (if known)
+  double bytes_per_unit = 0.0;  // for work counted on the device (timing_units)
 };
 
 struct PendingTiming {
@@ -69,6 +70,7 @@ struct trn_ctx {
   std::vector<hipEvent_t> event_pool;
   int32_t* pinned_i32 = nullptr;     // small pinned host staging buffers
   double* pinned_f64 = nullptr;
+  unsigned long long* unit_dev = nullptr;  // [kTimingUnitSlots] device work counters
 };
 
 namespace trn {
@@ -144,6 +146,12 @@ namespace trn {
 
 // Launch helpers ---------------------------------------------------------
 int stat_index(trn_ctx* ctx, const char* name);
+// Work of a launch that only the device knows (e.g. the length of a list it
+// reads): with timing on, returns a device counter the kernel adds its work
+// units to; they are turned into bytes (units x bytes_per_unit) when the
+// timings are collected.  nullptr with timing off.
+constexpr int kTimingUnitSlots = 64;
+unsigned long long* timing_units(trn_ctx* ctx, const char* name, double bytes_per_unit);
 void timing_begin(trn_ctx* ctx, int stat, hipEvent_t* start, hipEvent_t* stop);
 void timing_end(trn_ctx* ctx, int stat, hipEvent_t start, hipEvent_t stop,
                 double bytes);
@@ -196,10 +204,11 @@ void memset_async(trn_ctx* ctx, void* p, int v, size_t bytes);
 // robin (block b -> XCD b % 8).  Remapping gives each XCD one contiguous
 // range of logical blocks, so neighbouring vertex blocks, which share
 // stencil data, run on the same XCD and hit the same L2.
-__device__ __forceinline__ uint32_t xcd_block() {
-  const uint32_t n = gridDim.x, b = blockIdx.x;
+__device__ __forceinline__ uint32_t xcd_block_of(const uint32_t b, const uint32_t n) {
   const uint32_t q = n >> 3, r = n & 7u, x = b & 7u, k = b >> 3;
   return x * q + (x < r ? x : r) + k;
 }
+
+__device__ __forceinline__ uint32_t xcd_block() { return xcd_block_of(blockIdx.x, gridDim.x); }
 
 }  // namespace trn

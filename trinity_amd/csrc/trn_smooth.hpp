@@ -386,8 +386,19 @@ __device__ __forceinline__ int move_vertex(const int i, double2* __restrict__ xy
 // bit 2) and vertices without incident triangles are not moved
 // (Smooth::preProcess, src/smoothing.cpp:134-138).  *task = 1 if the
 // vertex is a smoothing task; returns 1 if the move was committed.
+//
+// stale[i] records that the last move of i failed with nothing in its
+// closed 1-ring committed since.  A move reads only the closed 1-ring
+// (coordinates, metric and logs of i and its neighbours, and the cached
+// qualities of its incident triangles, which only a commit at one of their
+// corners rewrites), so such a vertex would fail again with the same
+// result; the sweeps leave it out of the phase (it still counts as a task).
+// A failed move sets stale[i]; a commit clears the flags of the 1-ring.
+// Within one phase only the vertex itself and vertices of other colours
+// touch stale[i], apart from concurrent commits that all store 0.
 __device__ __forceinline__ int move_vertex_rec(const int32_t i, VRecord* __restrict__ rec,
                                                double* __restrict__ qualit,
+                                               uint8_t* __restrict__ stale,
                                                const int32_t* __restrict__ v2t_off,
                                                const int32_t* __restrict__ v2t,
                                                const int32_t* __restrict__ star,
@@ -403,14 +414,29 @@ __device__ __forceinline__ int move_vertex_rec(const int32_t i, VRecord* __restr
   if (topo.meta != kVtopoGeneric) {
     if ((topo.meta & 15) == 0) return 0;
     *task = 1;
-    return move_vertex_fast(i, fast_topo_rec(i, topo), pini, T, v, qualit, lds, stride);
+    const FastTopo ft = fast_topo_rec(i, topo);
+    const int c = move_vertex_fast(i, ft, pini, T, v, qualit, lds, stride);
+    if (c) {
+#pragma unroll
+      for (int k = 0; k < kVtopoDeg; ++k)
+        if (k < ft.nb) stale[ft.ring[k]] = 0;
+    } else {
+      stale[i] = 1;
+    }
+    return c;
   }
   const int32_t t0 = v2t_off[i], deg = v2t_off[i + 1] - t0;
   const int32_t n0 = v2v_off[i], nb = v2v_off[i + 1] - n0;
   if (deg == 0) return 0;
   *task = 1;
-  return move_vertex_generic(i, t0, deg, n0, nb, pini, T, v, qualit, v2t, star, v2v, lds,
-                             stride);
+  const int c = move_vertex_generic(i, t0, deg, n0, nb, pini, T, v, qualit, v2t, star, v2v,
+                                    lds, stride);
+  if (c) {
+    for (int k = 0; k < nb; ++k) stale[v2v[n0 + k]] = 0;
+  } else {
+    stale[i] = 1;
+  }
+  return c;
 }
 
 }  // namespace trn
