@@ -89,7 +89,7 @@ __global__ __launch_bounds__(256) void k_unpack_records(int32_t nv,
 // vertices written by k_class_live, the grid is sized for `count`, and the
 // blocks past the live ones exit at once while the others take XCD-chunked
 // ranges of the live blocks.
-__global__ __launch_bounds__(64) void k_smooth_phase(
+__global__ __launch_bounds__(64, 3) void k_smooth_phase(
     int32_t count, const int32_t* __restrict__ count_d, const int32_t* __restrict__ order,
     VRecord* __restrict__ rec, double* __restrict__ qualit, uint8_t* __restrict__ stale,
     const int32_t* __restrict__ v2t_off, const int32_t* __restrict__ v2t,
@@ -188,13 +188,13 @@ static unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 constexpr double kBytesSweepPerV = 196.0;
 
 // Threads per block of the phase kernel and its LDS bytes per block.  Per
-// thread: 4 doubles per incident triangle, for at most kVtopoDeg triangles
-// on the fast path or for the largest degree when the mesh has a vertex of
-// more than kVtopoDeg incident triangles or neighbours (generic path).  At
-// most 64 KiB per block.
+// thread: 5 doubles for each of at most kVtopoDeg incident triangles on the
+// fast path, and 4 per incident triangle of the largest degree when the
+// mesh has a vertex of more than kVtopoDeg incident triangles or neighbours
+// (generic path).  At most 64 KiB per block.
 static void phase_shape(const trn_mesh* m, int* block, size_t* shmem) {
   const bool generic = m->max_deg_t > kVtopoDeg || m->max_deg_v > kVtopoDeg;
-  const int per_thread = 4 * (generic ? m->max_deg_t : kVtopoDeg);
+  const int per_thread = std::max(5 * kVtopoDeg, generic ? 4 * m->max_deg_t : 0);
   int b = 64;
   while (b > 16 && (size_t)b * per_thread * sizeof(double) > 65536) b >>= 1;
   *block = b;

@@ -144,29 +144,20 @@ __device__ __forceinline__ double mean_sqrt_det(const double j[3]) {
   return __builtin_sqrt(j[0] * j[2] - j[1] * j[1]);
 }
 
-// Metric-space quality from the averaged tensor j and its sqrt(det j)
-// (numeric.cpp:275-305).
-__device__ __forceinline__ double quality_from_mean(const double* pa,
-                                                    const double* pb,
-                                                    const double* pc,
-                                                    const double j[3],
-                                                    const double sdet) {
-  double coef[6];
-  coef[0] = pb[0] - pa[0];
-  coef[1] = pb[1] - pa[1];
-  coef[2] = pc[0] - pb[0];
-  coef[3] = pc[1] - pb[1];
-  coef[4] = pa[0] - pc[0];
-  coef[5] = pa[1] - pc[1];
-  double perim = 0;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const int i = k * 2, l = i + 1;
-    const double v0 = j[0] * coef[i] + j[1] * coef[l];
-    const double v1 = j[1] * coef[i] + j[2] * coef[l];
-    perim += __builtin_sqrt(coef[i] * v0 + coef[l] * v1);
-  }
-  const double a0 = pb[0] - pa[0];           // numeric.cpp:296-300
+// Metric length of the edge vector (c0, c1) under the averaged tensor j
+// (numeric.cpp:286-292).
+__device__ __forceinline__ double edge_len(const double j[3], const double c0,
+                                           const double c1) {
+  const double v0 = j[0] * c0 + j[1] * c1;
+  const double v1 = j[1] * c0 + j[2] * c1;
+  return __builtin_sqrt(c0 * v0 + c1 * v1);
+}
+
+// Quality from the area, the perimeter and sqrt(det j) (numeric.cpp:296-305).
+__device__ __forceinline__ double quality_tail(const double* pa, const double* pb,
+                                               const double* pc, const double perim,
+                                               const double sdet) {
+  const double a0 = pb[0] - pa[0];
   const double a1 = pc[0] - pa[0];
   const double a2 = pb[1] - pa[1];
   const double a3 = pc[1] - pa[1];
@@ -174,6 +165,39 @@ __device__ __forceinline__ double quality_from_mean(const double* pa,
   const double mean = perim_mean(perim);
   const double norm = (12 * 1.7320508075688772) * cube_rn(mean * (2. - mean));
   return area * norm * sdet / (perim * perim);
+}
+
+// Metric-space quality from the averaged tensor j and its sqrt(det j)
+// (numeric.cpp:275-305).
+__device__ __forceinline__ double quality_from_mean(const double* pa,
+                                                    const double* pb,
+                                                    const double* pc,
+                                                    const double j[3],
+                                                    const double sdet) {
+  double perim = 0;
+  perim += edge_len(j, pb[0] - pa[0], pb[1] - pa[1]);
+  perim += edge_len(j, pc[0] - pb[0], pc[1] - pb[1]);
+  perim += edge_len(j, pa[0] - pc[0], pa[1] - pc[1]);
+  return quality_tail(pa, pb, pc, perim, sdet);
+}
+
+// quality_from_mean of the triangle whose corner r is the point p and whose
+// corners r+1 and r+2 are a and b, with the length lab of the edge b - a
+// (the edge that does not move with p) given.  Whatever r, the reference
+// forms the two other edge vectors as a - p and p - b, and adds the three
+// lengths in corner order (edge k joins corners k and k+1), which this does.
+__device__ __forceinline__ double quality_with_corner(const double pp[3][2], const int r,
+                                                      const double px, const double py,
+                                                      const double2 a, const double2 b,
+                                                      const double j[3], const double sdet,
+                                                      const double lab) {
+  const double la = edge_len(j, a.x - px, a.y - py);  // edge r
+  const double lb = edge_len(j, px - b.x, py - b.y);  // edge r+2
+  double perim = 0;
+  perim += r == 0 ? la : (r == 1 ? lb : lab);
+  perim += r == 0 ? lab : (r == 1 ? la : lb);
+  perim += r == 0 ? lb : (r == 1 ? lab : la);
+  return quality_tail(pp[0], pp[1], pp[2], perim, sdet);
 }
 
 __device__ __forceinline__ double quality_from_mean(const double* pa,

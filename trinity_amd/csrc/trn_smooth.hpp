@@ -162,8 +162,8 @@ __device__ __forceinline__ FastTopo fast_topo_rec(const int32_t i, const VTopo& 
 // are used), and the corner coordinates stay in registers across the
 // tries.  Arithmetic and its order are those of move_vertex_generic.
 // lds holds, per thread with stride `stride`, the averaged tensor of each
-// incident triangle (3 doubles) and its sqrt(det) (1 double): 4 *
-// kVtopoDeg doubles.
+// incident triangle (3 doubles), its sqrt(det) and the metric length of the
+// triangle's edge opposite the vertex: 5 * kVtopoDeg doubles.
 template <class V>
 __device__ __forceinline__ int move_vertex_fast(const int32_t i, const FastTopo& ft,
                                                 const double2 pini, const double T[3],
@@ -203,7 +203,7 @@ __device__ __forceinline__ int move_vertex_fast(const int32_t i, const FastTopo&
   const double4 LN = tensor_log4(TN[0], TN[1], TN[2]);
 
   double2 pa[kVtopoDeg], pb[kVtopoDeg];
-  double* jm = lds;  // [kVtopoDeg][3], then [kVtopoDeg] sqrt(det)
+  double* jm = lds;  // [kVtopoDeg][3], [kVtopoDeg] sqrt(det), [kVtopoDeg] |b - a|
 #pragma unroll
   for (int k = 0; k < kVtopoDeg; ++k) {
     if (k < deg) {
@@ -220,6 +220,7 @@ __device__ __forceinline__ int move_vertex_fast(const int32_t i, const FastTopo&
       jm[(3 * k + 1) * stride] = J[1];
       jm[(3 * k + 2) * stride] = J[2];
       jm[(3 * kVtopoDeg + k) * stride] = mean_sqrt_det(J);
+      jm[(4 * kVtopoDeg + k) * stride] = edge_len(J, pb[k].x - pa[k].x, pb[k].y - pa[k].y);
     } else {
       pa[k] = pb[k] = pini;
     }
@@ -247,7 +248,8 @@ __device__ __forceinline__ int move_vertex_fast(const int32_t i, const FastTopo&
         if (ccw(pp[0], pp[1], pp[2])) {
           const double J[3] = {jm[(3 * k) * stride], jm[(3 * k + 1) * stride],
                                jm[(3 * k + 2) * stride]};
-          qs[k] = quality_from_mean(pp[0], pp[1], pp[2], J, jm[(3 * kVtopoDeg + k) * stride]);
+          qs[k] = quality_with_corner(pp, r, px, py, a, b, J, jm[(3 * kVtopoDeg + k) * stride],
+                                      jm[(4 * kVtopoDeg + k) * stride]);
           valid = qs[k] > q_min;
         }
       }
