@@ -112,9 +112,16 @@ def free_gpu():
 
 
 def run_plain(T, mg, ctx, workload, p, sweeps, steps, warmup, seed, color_seed,
-              locality=False):
+              locality=False, cctx=None, cstream=None):
     """Times `steps` passes (coordinates restored, metric, colouring, sweeps)
-    and returns the wall time per pass and the per-kernel statistics."""
+    and returns the wall time per pass and the per-kernel statistics.
+
+    With cctx (a second context on the non-blocking stream cstream) the
+    colouring, which reads only the topology, runs on cstream while the
+    recovery and metric kernels run on ctx's stream; cstream first waits for
+    all earlier work of ctx's stream, so the colouring never overwrites
+    buffers the previous pass still reads, and the sweeps start after the
+    colouring has returned (trn_color synchronises its stream)."""
     import numpy as np
     import torch
     m = mg.workload(workload, seed=seed)
@@ -130,8 +137,10 @@ def run_plain(T, mg, ctx, workload, p, sweeps, steps, warmup, seed, color_seed,
 
     def step():
         xy.copy_(xy0)
+        if cctx is not None:
+            cstream.wait_stream(torch.cuda.current_stream())
         T.metric_run(ctx, dm, xy, f, 0.8, p, 1e-9, 0.4, tensor=tensor, want_complexity=False)
-        col = T.color(ctx, dm, seed=color_seed, max_colors=64)
+        col = T.color(cctx if cctx is not None else ctx, dm, seed=color_seed, max_colors=64)
         ncolors.append(col.ncolors)
         T.smooth_run(ctx, dm, xy, tensor, qualit, col, sweeps, recompute_qualit=True,
                      counters=False)
@@ -139,23 +148,27 @@ def run_plain(T, mg, ctx, workload, p, sweeps, steps, warmup, seed, color_seed,
     for _ in range(warmup):
         step()
     torch.cuda.synchronize()
-    ctx.set_timing(True)
-    ctx.reset_timing()
+    ctxs = [ctx] + ([cctx] if cctx is not None else [])
+    for c in ctxs:
+        c.set_timing(True)
+        c.reset_timing()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    stats = ctx.timings()
-    ctx.set_timing(False)
+    stats = {}
+    for c in ctxs:
+        stats.update(c.timings())
+        c.set_timing(False)
     ms = (t1 - t0) * 1e3 / steps
     nv, nt = m.nv, m.nt
     dm.close()
     del m, xy0, dm, f, xy, tensor, qualit
     free_gpu()
     return {"nv": nv, "nt": nt, "ms": ms, "stats": stats, "colours": int(np.median(ncolors)),
-            "steps": steps}
+            "steps": steps, "colouring_overlapped": cctx is not None}
 
 
 def summarize(r, sweeps, label=None):
@@ -178,8 +191,10 @@ def summarize(r, sweeps, label=None):
     color_ms = sum(v["ms_per_step"] for n, v in kern.items()
                    if n.startswith("color") or n.startswith("class") or n.startswith("scan")
                    or n == "reduce_max")
-    smooth_ms = sum(kern[n]["ms_per_step"] for n in ("tensor_log", "quality_cache",
-                                                     "smooth_phase", "counter_reduce")
+    smooth_ms = sum(kern[n]["ms_per_step"] for n in ("tensor_log", "pack_records",
+                                                     "quality_cache", "class_live",
+                                                     "smooth_phase", "unpack_records",
+                                                     "counter_reduce")
                     if n in kern)
     stages = {
         "recovery_metric": {"ms": round(metric_ms, 4),
@@ -187,7 +202,8 @@ def summarize(r, sweeps, label=None):
                             "hbm_frac": round(BYTES_METRIC_PER_V * nv / (metric_ms * 1e-3) / 1e9
                                               / HBM_PEAK_GBS, 4),
                             "bytes_per_vertex": BYTES_METRIC_PER_V},
-        "colouring": {"ms": round(color_ms, 4), "colours": r["colours"]},
+        "colouring": {"ms": round(color_ms, 4), "colours": r["colours"],
+                      "overlapped_with_metric": bool(r.get("colouring_overlapped", False))},
         "smoothing": {"ms": round(smooth_ms, 4), "sweeps": sweeps,
                       "Mvertex_moves_per_s": round(nv * sweeps / (smooth_ms * 1e-3) / 1e6, 1),
                       "alg_GBps": round(BYTES_SWEEP_PER_V * nv * sweeps / (smooth_ms * 1e-3) / 1e9, 1)},
@@ -296,6 +312,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline pass (no p sweep, adjoint, 16.8M stress run)")
+    ap.add_argument("--serial-colouring", action="store_true",
+                    help="colour on the pass's stream after the metric instead of on a "
+                         "second stream beside it")
     ap.add_argument("--locality-order", action="store_true",
                     help="visit colour classes along a Z-curve of the coordinates")
     ap.add_argument("--cpu-sample-n", type=int, default=1024)
@@ -323,9 +342,15 @@ def main():
     import trinity_amd as T
     from trinity_amd import meshgen as mg
 
+    import torch
     ctx = T.Context(0)
+    cctx = cstream = None
+    if not args.serial_colouring:
+        cstream = torch.cuda.Stream()  # non-blocking, default priority
+        cctx = T.Context(0, use_torch_stream=False)
+        cctx.set_stream(cstream.cuda_stream)
     r = run_plain(T, mg, ctx, args.workload, args.p, args.sweeps, args.steps, args.warmup,
-                  args.seed, args.color_seed, args.locality_order)
+                  args.seed, args.color_seed, args.locality_order, cctx, cstream)
     ms = r["ms"]
     value = r["nv"] / (ms * 1e-3) / 1e6
     kern, stages = summarize(r, args.sweeps, f"{args.workload} p={args.p}")
@@ -343,12 +368,13 @@ def main():
             if p == args.p:
                 continue
             rp = run_plain(T, mg, ctx, args.workload, p, args.sweeps, 3, 1, args.seed,
-                           args.color_seed)
+                           args.color_seed, False, cctx, cstream)
             _, st = summarize(rp, args.sweeps, f"{args.workload} p={p}")
             extras[f"p{p}"] = {"ms_per_pass": round(rp["ms"], 4),
                                "Mvertices_per_s": round(rp["nv"] / (rp["ms"] * 1e-3) / 1e6, 2),
                                "stages": st}
-        rs = run_plain(T, mg, ctx, "jitter4096:layer", 4, 5, 2, 1, args.seed, args.color_seed)
+        rs = run_plain(T, mg, ctx, "jitter4096:layer", 4, 5, 2, 1, args.seed, args.color_seed,
+                       False, cctx, cstream)
         _, st = summarize(rs, 5, "jitter4096:layer p=4")
         extras["stress_16M"] = {"workload": "jitter4096:layer", "vertices": rs["nv"],
                                 "triangles": rs["nt"], "lp_norm": 4, "sweeps": 5,
@@ -383,7 +409,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "vertices": r["nv"], "triangles": r["nt"],
                    "lp_norm": args.p, "target_factor": 0.8, "h_min": 1e-9, "h_max": 0.4,
-                   "sweeps": args.sweeps, "colouring": "gpu seeded speculative greedy",
+                   "sweeps": args.sweeps, "colouring": "gpu seeded speculative greedy" +
+                   ("" if args.serial_colouring else ", on a second stream beside the metric"),
                    "parallelism": "single GPU"},
         "roofline": {"kernel": dname, "bound": "hbm", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
