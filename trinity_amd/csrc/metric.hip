@@ -49,12 +49,37 @@ __device__ __forceinline__ const int32_t* patch_col(const int32_t* __restrict__ 
   return ell + (int64_t)row[xcd_block()] * kPatchSlab + threadIdx.x;
 }
 
+// Gathers over the stored patches.  A slab with a union (union_n[b] >= 0)
+// first stages the per-triangle terms of its union in LDS with coalesced
+// loads (the union is ascending, so its triangles come in runs), then each
+// vertex sums its patch from LDS through the local indices, in the same
+// patch order; a slab without one gathers from global memory.
 template <bool WITH_GRAD>
 __global__ __launch_bounds__(kGatherB) void k_vtx_grad(
     int32_t nv, const int32_t* __restrict__ patch_n, const int32_t* __restrict__ patch_row,
-    const int32_t* __restrict__ patch_ell, const double* __restrict__ area_t,
+    const int32_t* __restrict__ patch_ell, const int32_t* __restrict__ union_n,
+    const int32_t* __restrict__ union_off, const int32_t* __restrict__ union_ids,
+    const uint16_t* __restrict__ patch_local, int max_union, const double* __restrict__ area_t,
     const double2* __restrict__ ag_t, double2* __restrict__ grad, double* __restrict__ tarea) {
-  const int i = (int)(xcd_block() * kGatherB + threadIdx.x);
+  extern __shared__ double2 lds_g[];  // [max_union] ag, then [max_union] area
+  const uint32_t b = xcd_block();
+  const int i = (int)(b * kGatherB + threadIdx.x);
+  const int nu = union_n[b];
+  if (nu >= 0) {
+    double* lds_a = (double*)(lds_g + max_union);
+    const int32_t* U = union_ids + union_off[b];
+    for (int k = threadIdx.x; k < nu; k += kGatherB) {
+      const int32_t t = U[k];
+      lds_a[k] = area_t[t];
+      if (WITH_GRAD) lds_g[k] = ag_t[t];
+    }
+    __syncthreads();
+    if (i >= nv) return;
+    vtx_grad_one<WITH_GRAD>(i, patch_local + (int64_t)patch_row[b] * kPatchSlab + threadIdx.x,
+                            kPatchSlab, patch_n[i], (const double*)lds_a,
+                            (const double2*)lds_g, grad, tarea);
+    return;
+  }
   if (i >= nv) return;
   vtx_grad_one<WITH_GRAD>(i, patch_col(patch_row, patch_ell), kPatchSlab, patch_n[i], area_t,
                           ag_t, grad, tarea);
@@ -72,10 +97,24 @@ __global__ __launch_bounds__(256) void k_tri_hess(int32_t nt, const int32_t* __r
 template <bool NORMALIZE>
 __global__ __launch_bounds__(kGatherB) void k_vtx_hess(
     int32_t nv, const int32_t* __restrict__ patch_n, const int32_t* __restrict__ patch_row,
-    const int32_t* __restrict__ patch_ell, const double4* __restrict__ ah_t,
+    const int32_t* __restrict__ patch_ell, const int32_t* __restrict__ union_n,
+    const int32_t* __restrict__ union_off, const int32_t* __restrict__ union_ids,
+    const uint16_t* __restrict__ patch_local, const double4* __restrict__ ah_t,
     const double* __restrict__ tarea, double* __restrict__ hess, double* __restrict__ out,
     double ex) {
-  const int i = (int)(xcd_block() * kGatherB + threadIdx.x);
+  extern __shared__ double4 lds_h[];  // [max_union]
+  const uint32_t b = xcd_block();
+  const int i = (int)(b * kGatherB + threadIdx.x);
+  const int nu = union_n[b];
+  if (nu >= 0) {
+    const int32_t* U = union_ids + union_off[b];
+    for (int k = threadIdx.x; k < nu; k += kGatherB) lds_h[k] = ah_t[U[k]];
+    __syncthreads();
+    if (i >= nv) return;
+    vtx_hess_one<NORMALIZE>(i, patch_local + (int64_t)patch_row[b] * kPatchSlab + threadIdx.x,
+                            kPatchSlab, patch_n[i], (const double4*)lds_h, tarea, hess, out, ex);
+    return;
+  }
   if (i >= nv) return;
   vtx_hess_one<NORMALIZE>(i, patch_col(patch_row, patch_ell), kPatchSlab, patch_n[i], ah_t,
                           tarea, hess, out, ex);
@@ -172,30 +211,44 @@ static GradBuffers carve_grad(Scratch& sc, const trn_mesh* m) {
 }
 
 
+// LDS of the gradient gathers: the union's ag (16 B) and area (8 B) terms.
+static size_t grad_shmem(const trn_mesh* m) {
+  return (sizeof(double2) + sizeof(double)) * (size_t)m->max_union;
+}
+
 static void run_gradient(trn_ctx* ctx, const trn_mesh* m, const double* xy, const double* f,
                          double* grad, GradBuffers& b) {
   launch(ctx, "tri_grad", 0.0, k_tri_grad, dim3(nblk(m->nt, 256)), dim3(256), 0, m->nt,
          (const int32_t*)m->tri, (const double2*)xy, f, b.area_t, b.ag_t);
   launch(ctx, "vtx_grad", kBytesGradPerV * m->nv, k_vtx_grad<true>, dim3(nblk(m->nv, kGatherB)),
-         dim3(kGatherB), 0, m->nv, (const int32_t*)m->patch_n, (const int32_t*)m->patch_row,
-         (const int32_t*)m->patch_ell, (const double*)b.area_t, (const double2*)b.ag_t,
-         (double2*)grad, b.tarea);
+         dim3(kGatherB), grad_shmem(m), m->nv, (const int32_t*)m->patch_n,
+         (const int32_t*)m->patch_row, (const int32_t*)m->patch_ell, (const int32_t*)m->union_n,
+         (const int32_t*)m->union_off, (const int32_t*)m->union_ids,
+         (const uint16_t*)m->patch_local, m->max_union, (const double*)b.area_t,
+         (const double2*)b.ag_t, (double2*)grad, b.tarea);
 }
 
 static void run_hessian(trn_ctx* ctx, const trn_mesh* m, const double* xy, const double* grad,
                         double* hess, double* out, bool normalize, double ex, GradBuffers& b) {
   launch(ctx, "tri_hess", 0.0, k_tri_hess, dim3(nblk(m->nt, 256)), dim3(256), 0, m->nt,
          (const int32_t*)m->tri, (const double2*)xy, (const double2*)grad, b.ah_t);
+  const size_t shmem = sizeof(double4) * (size_t)m->max_union;
   if (normalize)
     launch(ctx, "vtx_hess_norm", kBytesHessPerV * m->nv, k_vtx_hess<true>,
-           dim3(nblk(m->nv, kGatherB)), dim3(kGatherB), 0, m->nv, (const int32_t*)m->patch_n,
-           (const int32_t*)m->patch_row, (const int32_t*)m->patch_ell, (const double4*)b.ah_t,
-           (const double*)b.tarea, hess, out, ex);
+           dim3(nblk(m->nv, kGatherB)), dim3(kGatherB), shmem, m->nv,
+           (const int32_t*)m->patch_n, (const int32_t*)m->patch_row,
+           (const int32_t*)m->patch_ell, (const int32_t*)m->union_n,
+           (const int32_t*)m->union_off, (const int32_t*)m->union_ids,
+           (const uint16_t*)m->patch_local, (const double4*)b.ah_t, (const double*)b.tarea, hess,
+           out, ex);
   else
     launch(ctx, "vtx_hess", kBytesHessPerV * m->nv, k_vtx_hess<false>,
-           dim3(nblk(m->nv, kGatherB)), dim3(kGatherB), 0, m->nv, (const int32_t*)m->patch_n,
-           (const int32_t*)m->patch_row, (const int32_t*)m->patch_ell, (const double4*)b.ah_t,
-           (const double*)b.tarea, hess, out, ex);
+           dim3(nblk(m->nv, kGatherB)), dim3(kGatherB), shmem, m->nv,
+           (const int32_t*)m->patch_n, (const int32_t*)m->patch_row,
+           (const int32_t*)m->patch_ell, (const int32_t*)m->union_n,
+           (const int32_t*)m->union_off, (const int32_t*)m->union_ids,
+           (const uint16_t*)m->patch_local, (const double4*)b.ah_t, (const double*)b.tarea, hess,
+           out, ex);
 }
 
 static void run_complexity(trn_ctx* ctx, const trn_mesh* m, const double* xy,
@@ -277,8 +330,10 @@ int trn_hessian(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d, const do
            (const int32_t*)mesh->tri, (const double2*)xy_d, (const double*)nullptr, b.area_t,
            b.ag_t);
     launch(ctx, "vtx_area", 0.0, k_vtx_grad<false>, dim3(nblk(mesh->nv, kGatherB)),
-           dim3(kGatherB), 0, mesh->nv, (const int32_t*)mesh->patch_n,
+           dim3(kGatherB), grad_shmem(mesh), mesh->nv, (const int32_t*)mesh->patch_n,
            (const int32_t*)mesh->patch_row, (const int32_t*)mesh->patch_ell,
+           (const int32_t*)mesh->union_n, (const int32_t*)mesh->union_off,
+           (const int32_t*)mesh->union_ids, (const uint16_t*)mesh->patch_local, mesh->max_union,
            (const double*)b.area_t, (const double2*)b.ag_t, (double2*)nullptr, b.tarea);
     run_hessian(ctx, mesh, xy_d, gradient_d, nullptr, tensor_d, false, 0.0, b);
     check_err(ctx, b);

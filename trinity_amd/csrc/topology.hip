@@ -289,26 +289,128 @@ __global__ __launch_bounds__(kPatchSlab) void k_patch_slab_rows(int32_t nv,
 
 // Stored patches: the ascending 2-ring element set of every vertex, as
 // k_patch_size builds it, written column-wise into its slab, padded with -1.
+// The columns past the last vertex of the last slab are all padding, so
+// every entry of the array is written.
 __global__ __launch_bounds__(kPatchSlab) void k_patch_fill(
     int32_t nv, const int32_t* __restrict__ v2t_off, const int32_t* __restrict__ v2t,
     const int32_t* __restrict__ v2v_off, const int32_t* __restrict__ v2v, int cap,
     const int32_t* __restrict__ row, int32_t* __restrict__ ell) {
   extern __shared__ int lds[];
   const int64_t i = (int64_t)blockIdx.x * kPatchSlab + threadIdx.x;
-  if (i >= nv) return;
   int* s = lds + threadIdx.x;
   int m = 0;
   bool ok = true;
-  for (int32_t k = v2t_off[i]; k < v2t_off[i + 1]; ++k)
-    ok = ok && lds_sorted_insert(s, kPatchSlab, m, cap, v2t[k]);
-  for (int32_t a = v2v_off[i]; a < v2v_off[i + 1] && ok; ++a) {
-    const int32_t j = v2v[a];
-    for (int32_t k = v2t_off[j]; k < v2t_off[j + 1]; ++k)
+  if (i < nv) {
+    for (int32_t k = v2t_off[i]; k < v2t_off[i + 1]; ++k)
       ok = ok && lds_sorted_insert(s, kPatchSlab, m, cap, v2t[k]);
+    for (int32_t a = v2v_off[i]; a < v2v_off[i + 1] && ok; ++a) {
+      const int32_t j = v2v[a];
+      for (int32_t k = v2t_off[j]; k < v2t_off[j + 1]; ++k)
+        ok = ok && lds_sorted_insert(s, kPatchSlab, m, cap, v2t[k]);
+    }
   }
   const int32_t r0 = row[blockIdx.x], w = row[blockIdx.x + 1] - r0;
   int32_t* col = ell + (int64_t)r0 * kPatchSlab + threadIdx.x;
   for (int e = 0; e < w; ++e) col[(int64_t)e * kPatchSlab] = e < m ? s[e * kPatchSlab] : -1;
+}
+
+// Union sizes for the scan: negative (no union) counts as 0.
+__global__ void k_union_sizes(int32_t nslab, const int32_t* __restrict__ n,
+                              int32_t* __restrict__ out) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nslab) out[b] = n[b] > 0 ? n[b] : 0;
+}
+
+// Union of the patches of one slab (one block per slab): the slab's stored
+// entries are sorted in LDS (bitonic), duplicates dropped.  COUNT: writes
+// the union size (or -1 when the slab is larger than kUnionKeys entries or
+// its union larger than kUnionCap).  Otherwise writes the union at
+// union_off[b] and, for every patch entry, its position in the union.
+template <bool COUNT>
+__global__ __launch_bounds__(kPatchSlab) void k_patch_union(
+    int32_t nv, const int32_t* __restrict__ row, const int32_t* __restrict__ ell,
+    int32_t* __restrict__ union_n, const int32_t* __restrict__ union_off,
+    int32_t* __restrict__ union_ids, uint16_t* __restrict__ local) {
+  __shared__ int32_t keys[kUnionKeys];
+  __shared__ int32_t part[kPatchSlab];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int64_t r0 = row[b];
+  const int n = (row[b + 1] - (int)r0) * kPatchSlab;
+  if (n > kUnionKeys || (!COUNT && union_n[b] < 0)) {
+    if (COUNT && tid == 0) union_n[b] = -1;
+    return;
+  }
+  int P = 1;
+  while (P < n) P <<= 1;
+  const int32_t* src = ell + r0 * kPatchSlab;
+  const int64_t v0 = (int64_t)b * kPatchSlab;  // first vertex of the slab
+  for (int k = tid; k < P; k += kPatchSlab) {
+    const int32_t t = (k < n && v0 + (k % kPatchSlab) < nv) ? src[k] : -1;
+    keys[k] = t >= 0 ? t : INT32_MAX;
+  }
+  __syncthreads();
+  for (int size = 2; size <= P; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int k = tid; k < P; k += kPatchSlab) {
+        const int q = k ^ stride;
+        if (q > k) {
+          const bool up = (k & size) == 0;
+          const int32_t x = keys[k], y = keys[q];
+          if ((x > y) == up) {
+            keys[k] = y;
+            keys[q] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  // distinct entries, counted over contiguous chunks so positions follow
+  const int C = (P + kPatchSlab - 1) / kPatchSlab, c0 = tid * C;
+  int cnt = 0;
+  for (int k = c0; k < c0 + C && k < P; ++k)
+    cnt += keys[k] != INT32_MAX && (k == 0 || keys[k] != keys[k - 1]);
+  part[tid] = cnt;
+  __syncthreads();
+  if (tid == 0) {
+    int s = 0;
+    for (int w = 0; w < kPatchSlab; ++w) {
+      const int c = part[w];
+      part[w] = s;
+      s += c;
+    }
+    if (COUNT) union_n[b] = s <= kUnionCap ? s : -1;
+  }
+  __syncthreads();
+  if (COUNT) return;
+  // compact the union into the front of keys (positions only move down)
+  int32_t* U = union_ids + union_off[b];
+  int pos = part[tid];
+  int32_t mine[64];  // a chunk holds at most kUnionKeys / kPatchSlab = 64 keys
+  int nm = 0;
+  for (int k = c0; k < c0 + C && k < P; ++k)
+    if (keys[k] != INT32_MAX && (k == 0 || keys[k] != keys[k - 1])) mine[nm++] = keys[k];
+  __syncthreads();
+  for (int j = 0; j < nm; ++j) {
+    keys[pos + j] = mine[j];
+    U[pos + j] = mine[j];
+  }
+  __syncthreads();
+  const int nu = union_n[b];
+  uint16_t* dst = local + r0 * kPatchSlab;
+  for (int k = tid; k < n; k += kPatchSlab) {
+    const int32_t t = v0 + (k % kPatchSlab) < nv ? src[k] : -1;
+    uint16_t v = 0xFFFF;
+    if (t >= 0) {
+      int lo = 0, hi = nu - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (keys[mid] < t) lo = mid + 1;
+        else hi = mid;
+      }
+      v = (uint16_t)lo;
+    }
+    dst[k] = v;
+  }
 }
 
 // Star codes (see trn_mesh::star).  Corners are located in the sorted 1-ring
@@ -498,6 +600,26 @@ int trn_mesh_create(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
              (const int32_t*)m->v2t, (const int32_t*)m->v2v_off, (const int32_t*)m->v2v,
              m->patch_cap, (const int32_t*)m->patch_row, m->patch_ell);
 
+      // 4c. per-slab unions of the patches (LDS staging of the gathers)
+      TRN_HIP(hipMalloc(&m->union_n, sizeof(int32_t) * (size_t)nslab));
+      TRN_HIP(hipMalloc(&m->union_off, sizeof(int32_t) * (size_t)(nslab + 1)));
+      TRN_HIP(hipMalloc(&m->patch_local, sizeof(uint16_t) * (size_t)(ell_n > 0 ? ell_n : 1)));
+      launch(ctx, "topo_patch_union_count", 0.0, k_patch_union<true>, dim3((unsigned)nslab),
+             dim3(kPatchSlab), 0, nv, (const int32_t*)m->patch_row, (const int32_t*)m->patch_ell,
+             m->union_n, (const int32_t*)nullptr, (int32_t*)nullptr, (uint16_t*)nullptr);
+      launch(ctx, "topo_union_clamp", 0.0, k_union_sizes, dim3(blocks_for(nslab, 256)),
+             dim3(256), 0, (int32_t)nslab, (const int32_t*)m->union_n, m->union_off);
+      reduce_max_i32(ctx, m->union_n, nslab, flags + 7);
+      exclusive_scan_i32(ctx, m->union_off, m->union_off, nslab, true, scan_tmp);
+      copy_async(ctx, h, m->union_off + nslab, sizeof(int32_t));
+      copy_async(ctx, h + 1, flags + 7, sizeof(int32_t));
+      sync(ctx);
+      m->max_union = h[1] > 0 ? h[1] : 0;
+      TRN_HIP(hipMalloc(&m->union_ids, sizeof(int32_t) * (size_t)(h[0] > 0 ? h[0] : 1)));
+      launch(ctx, "topo_patch_union", 0.0, k_patch_union<false>, dim3((unsigned)nslab),
+             dim3(kPatchSlab), 0, nv, (const int32_t*)m->patch_row, (const int32_t*)m->patch_ell,
+             m->union_n, (const int32_t*)m->union_off, m->union_ids, m->patch_local);
+
       // 5. star codes of the incident triangles (smoothing reads corners
       //    through the 1-ring instead of the triangle array)
       TRN_REQUIRE(m->max_deg_v < 256, TRN_ERR_CAPACITY, "1-ring larger than 255 vertices");
@@ -542,6 +664,10 @@ int trn_mesh_destroy(trn_mesh* m) {
     if (m->patch_n) (void)hipFree(m->patch_n);
     if (m->patch_row) (void)hipFree(m->patch_row);
     if (m->patch_ell) (void)hipFree(m->patch_ell);
+    if (m->union_n) (void)hipFree(m->union_n);
+    if (m->union_off) (void)hipFree(m->union_off);
+    if (m->union_ids) (void)hipFree(m->union_ids);
+    if (m->patch_local) (void)hipFree(m->patch_local);
     delete m;
   });
 }

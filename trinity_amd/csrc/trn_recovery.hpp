@@ -60,8 +60,8 @@ __device__ __forceinline__ void tri_grad_one(const int64_t t, const int32_t* __r
 
 // Per-vertex gradient (src/hessian.cpp:75-82) from the patch in the LDS set:
 // total area and area-weighted element gradients summed in patch order.
-template <bool WITH_GRAD>
-__device__ __forceinline__ void vtx_grad_one(const int i, const int* s, const int stride,
+template <bool WITH_GRAD, class Idx>
+__device__ __forceinline__ void vtx_grad_one(const int i, const Idx* s, const int stride,
                                              const int m, const double* __restrict__ area_t,
                                              const double2* __restrict__ ag_t,
                                              double2* __restrict__ grad,
@@ -121,8 +121,8 @@ __device__ __forceinline__ void normalize_local(double& t0, double& t1, double& 
 
 // Per-vertex Hessian (src/hessian.cpp:141-152) from the patch in the LDS set,
 // optionally followed by the local normalisation.
-template <bool NORMALIZE>
-__device__ __forceinline__ void vtx_hess_one(const int i, const int* s, const int stride,
+template <bool NORMALIZE, class Idx>
+__device__ __forceinline__ void vtx_hess_one(const int i, const Idx* s, const int stride,
                                              const int m, const double4* __restrict__ ah_t,
                                              const double* __restrict__ tarea,
                                              double* __restrict__ hess, double* __restrict__ out,

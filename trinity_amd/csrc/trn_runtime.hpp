@@ -105,6 +105,8 @@ struct alignas(128) VRecord {
 static_assert(sizeof(VRecord) == 128, "VRecord is one 128-byte line");
 
 constexpr int kPatchSlab = 64;  // vertices per slab of the stored patches
+constexpr int kUnionKeys = 4096;  // largest slab (rows x kPatchSlab) given a union
+constexpr int kUnionCap = 2048;   // most triangles in one slab's union
 }  // namespace trn
 
 struct trn_mesh {
@@ -140,6 +142,15 @@ struct trn_mesh {
   int32_t* patch_n = nullptr;    // [nv]
   int32_t* patch_row = nullptr;  // [nslab + 1]
   int32_t* patch_ell = nullptr;  // [patch_row[nslab] * kPatchSlab]
+  // Per slab, the ascending union of its vertices' patches (union_n[b]
+  // triangles at union_ids + union_off[b]; union_n[b] < 0: no union, the
+  // slab gathers through patch_ell), and patch_local, shaped like
+  // patch_ell, the position of each patch entry in its slab's union.
+  int32_t* union_n = nullptr;        // [nslab]
+  int32_t* union_off = nullptr;      // [nslab + 1]
+  int32_t* union_ids = nullptr;      // [union_off[nslab]]
+  uint16_t* patch_local = nullptr;   // [patch_row[nslab] * kPatchSlab]
+  int32_t max_union = 0;             // largest union_n
 };
 
 namespace trn {
