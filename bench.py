@@ -118,10 +118,11 @@ def run_plain(T, mg, ctx, workload, p, sweeps, steps, warmup, seed, color_seed,
 
     With cctx (a second context on the non-blocking stream cstream) the
     colouring, which reads only the topology, runs on cstream while the
-    recovery and metric kernels run on ctx's stream; cstream first waits for
-    all earlier work of ctx's stream, so the colouring never overwrites
-    buffers the previous pass still reads, and the sweeps start after the
-    colouring has returned (trn_color synchronises its stream)."""
+    recovery and metric kernels and then the smoothing preparation (record
+    packing, quality cache) run on ctx's stream; cstream first waits for all
+    earlier work of ctx's stream, so the colouring never overwrites buffers
+    the previous pass still reads, and the sweeps start after the colouring
+    has returned (trn_color synchronises its stream)."""
     import numpy as np
     import torch
     m = mg.workload(workload, seed=seed)
@@ -140,10 +141,17 @@ def run_plain(T, mg, ctx, workload, p, sweeps, steps, warmup, seed, color_seed,
         if cctx is not None:
             cstream.wait_stream(torch.cuda.current_stream())
         T.metric_run(ctx, dm, xy, f, 0.8, p, 1e-9, 0.4, tensor=tensor, want_complexity=False)
+        if cctx is not None:
+            # the record packing and the quality cache need the metric, not
+            # the colouring: queue them before waiting for the colouring
+            T.smooth_prepare(ctx, dm, xy, tensor, qualit, recompute_qualit=True)
         col = T.color(cctx if cctx is not None else ctx, dm, seed=color_seed, max_colors=64)
         ncolors.append(col.ncolors)
-        T.smooth_run(ctx, dm, xy, tensor, qualit, col, sweeps, recompute_qualit=True,
-                     counters=False)
+        if cctx is not None:
+            T.smooth_run_prepared(ctx, dm, xy, tensor, qualit, col, sweeps, counters=False)
+        else:
+            T.smooth_run(ctx, dm, xy, tensor, qualit, col, sweeps, recompute_qualit=True,
+                         counters=False)
 
     for _ in range(warmup):
         step()

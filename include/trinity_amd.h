@@ -171,6 +171,21 @@ int trn_smooth_run(trn_ctx* ctx, const trn_mesh* mesh, double* xy_d,
                    double* tensor_d, double* qualit_d, const int32_t* order_d,
                    const int32_t* class_off, int32_t ncolors, int32_t sweeps,
                    int32_t recompute_qualit, int32_t* commits, int32_t* tasks);
+/* trn_smooth_run in two parts, equal to it when called in sequence:
+ * trn_smooth_prepare loads xy_d and tensor_d into the mesh's vertex records
+ * and, if recompute_qualit != 0, rebuilds the quality cache
+ * (Smooth::cacheQuality, src/smoothing.cpp:145-155, :290); it needs no
+ * colouring, so a caller can issue it before the colouring is done.
+ * trn_smooth_run_prepared runs the sweeps on the records and writes xy_d
+ * and tensor_d back.  Between the two, xy_d, tensor_d and qualit_d must not
+ * change and no other smoothing call may use the mesh. */
+int trn_smooth_prepare(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d,
+                       const double* tensor_d, double* qualit_d,
+                       int32_t recompute_qualit);
+int trn_smooth_run_prepared(trn_ctx* ctx, const trn_mesh* mesh, double* xy_d,
+                            double* tensor_d, double* qualit_d, const int32_t* order_d,
+                            const int32_t* class_off, int32_t ncolors, int32_t sweeps,
+                            int32_t* commits, int32_t* tasks);
 
 /* ---- batched form: many meshes of uneven sizes -----------------------------
  * No counterpart in the reference, which processes one mesh per Metrics /

@@ -252,6 +252,26 @@ def test_smoothing_with_gpu_colouring_matches_oracle(ctx, wl, p, sweeps):
     assert normrel(ten, to).max() <= 1e-10
 
 
+def test_prepare_then_sweeps_equals_smooth_run(ctx):
+    """trn_smooth_prepare followed by trn_smooth_run_prepared is
+    trn_smooth_run: identical coordinates, metric, qualities and counters."""
+    import torch
+    m = mg.workload("flip96:fronts+quad")
+    st = oracle.metric_stages(m.xy, m.tri, m.solut, 0.8, 2)
+    dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags)
+    col = T().color(ctx, dm, seed=5, max_colors=16)
+    xa, ta = dev(m.xy), dev(st["metric"])
+    qa = torch.empty(dm.nt, dtype=torch.float64, device="cuda")
+    ca = T().smooth_run(ctx, dm, xa, ta, qa, col, 4)
+    xb, tb = dev(m.xy), dev(st["metric"])
+    qb = torch.empty(dm.nt, dtype=torch.float64, device="cuda")
+    T().smooth_prepare(ctx, dm, xb, tb, qb)
+    cb = T().smooth_run_prepared(ctx, dm, xb, tb, qb, col, 4)
+    assert ca == cb
+    for a, b in ((xa, xb), (ta, tb), (qa, qb)):
+        assert np.array_equal(a.cpu().numpy().view(np.uint64), b.cpu().numpy().view(np.uint64))
+
+
 def test_device_transcendentals_within_one_ulp_of_glibc(ctx):
     rng = np.random.default_rng(0)
     n = 1 << 18

@@ -37,7 +37,8 @@ EXPORTS = [
     "trn_mesh_create", "trn_mesh_destroy", "trn_mesh_info", "trn_mesh_csr",
     "trn_metric_run", "trn_gradient", "trn_hessian", "trn_normalize_local",
     "trn_complexity", "trn_normalize_global", "trn_color", "trn_color_classes",
-    "trn_color_check", "trn_quality_cache", "trn_smooth_run",
+    "trn_color_check", "trn_quality_cache", "trn_smooth_run", "trn_smooth_prepare",
+    "trn_smooth_run_prepared",
     "trn_batch_create", "trn_batch_destroy", "trn_batch_info", "trn_batch_color",
     "trn_batch_set_config", "trn_batch_get_config", "trn_batch_pass", "trn_batch_pass_graph",
     "trn_batch_pass_pipelined", "trn_batch_check", "trn_batch_autotune",
@@ -100,6 +101,9 @@ def _load():
         "trn_quality_cache": ([_vp, _vp, _vp, _vp, _vp], ctypes.c_int),
         "trn_smooth_run": ([_vp, _vp, _vp, _vp, _vp, _vp, _P(_i32), _i32, _i32, _i32, _P(_i32),
                             _P(_i32)], ctypes.c_int),
+        "trn_smooth_prepare": ([_vp, _vp, _vp, _vp, _vp, _i32], ctypes.c_int),
+        "trn_smooth_run_prepared": ([_vp, _vp, _vp, _vp, _vp, _vp, _P(_i32), _i32, _i32,
+                                     _P(_i32), _P(_i32)], ctypes.c_int),
         "trn_batch_create": ([_vp, _i32, _P(_i32), _P(_i32), _vp, _vp, _P(_vp)], ctypes.c_int),
         "trn_batch_destroy": ([_vp], ctypes.c_int),
         "trn_batch_info": ([_vp, _P(_i32), _P(_i32), _P(_vp)], ctypes.c_int),
@@ -405,6 +409,32 @@ def smooth_run(ctx, mesh, xy, tensor, qualit, coloring: Coloring, sweeps: int,
                                 _vp(coloring.order.data_ptr()), off, int(coloring.ncolors),
                                 int(sweeps), int(bool(recompute_qualit)),
                                 commits if counters else None, tasks if counters else None))
+    if counters:
+        return list(commits[:sweeps]), list(tasks[:sweeps])
+    return None
+
+
+def smooth_prepare(ctx, mesh, xy, tensor, qualit, recompute_qualit=True):
+    """First part of smooth_run: loads xy/tensor into the mesh's vertex
+    records and rebuilds the quality cache; needs no colouring."""
+    _check(lib().trn_smooth_prepare(ctx.handle, mesh.handle, _vp(xy.data_ptr()),
+                                    _vp(tensor.data_ptr()), _vp(qualit.data_ptr()),
+                                    int(bool(recompute_qualit))))
+
+
+def smooth_run_prepared(ctx, mesh, xy, tensor, qualit, coloring: Coloring, sweeps: int,
+                        counters=True):
+    """Second part of smooth_run: the sweeps on the prepared records, then
+    xy/tensor written back.  Returns (commits, tasks) when counters=True."""
+    off = (ctypes.c_int32 * (coloring.ncolors + 1))(*[int(v) for v in coloring.class_off])
+    commits = (ctypes.c_int32 * max(sweeps, 1))()
+    tasks = (ctypes.c_int32 * max(sweeps, 1))()
+    _check(lib().trn_smooth_run_prepared(ctx.handle, mesh.handle, _vp(xy.data_ptr()),
+                                         _vp(tensor.data_ptr()), _vp(qualit.data_ptr()),
+                                         _vp(coloring.order.data_ptr()), off,
+                                         int(coloring.ncolors), int(sweeps),
+                                         commits if counters else None,
+                                         tasks if counters else None))
     if counters:
         return list(commits[:sweeps]), list(tasks[:sweeps])
     return None

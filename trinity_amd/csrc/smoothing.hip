@@ -230,11 +230,24 @@ int trn_quality_cache(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d,
   });
 }
 
-int trn_smooth_run(trn_ctx* ctx, const trn_mesh* mesh, double* xy_d, double* tensor_d,
-                   double* qualit_d, const int32_t* order_d, const int32_t* class_off,
-                   int32_t ncolors, int32_t sweeps, int32_t recompute_qualit, int32_t* commits,
-                   int32_t* tasks) {
-  return guard_impl("trn_smooth_run", [&] {
+int trn_smooth_prepare(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d,
+                       const double* tensor_d, double* qualit_d, int32_t recompute_qualit) {
+  return guard_impl("trn_smooth_prepare", [&] {
+    DeviceScope ds(ctx);
+    TRN_REQUIRE(mesh && xy_d && tensor_d && qualit_d, TRN_ERR_ARG, "null argument");
+    const trn_mesh* m = mesh;
+    Scratch sc(ctx, scratch_bytes(sizeof(double4) * (size_t)m->nv));
+    double4* logm = sc.carve<double4>(m->nv);
+    launch(ctx, "pack_records", 256.0 * m->nv, k_pack_records, dim3(nblk(m->nv, 256)),
+           dim3(256), 0, m->nv, (const double2*)xy_d, (const double*)tensor_d, m->vrec, logm);
+    if (recompute_qualit) quality_cache_logs(ctx, m, xy_d, logm, qualit_d);
+  });
+}
+
+int trn_smooth_run_prepared(trn_ctx* ctx, const trn_mesh* mesh, double* xy_d, double* tensor_d,
+                            double* qualit_d, const int32_t* order_d, const int32_t* class_off,
+                            int32_t ncolors, int32_t sweeps, int32_t* commits, int32_t* tasks) {
+  return guard_impl("trn_smooth_run_prepared", [&] {
     DeviceScope ds(ctx);
     TRN_REQUIRE(mesh && xy_d && tensor_d && qualit_d && order_d && class_off, TRN_ERR_ARG,
                 "null argument");
@@ -253,21 +266,16 @@ int trn_smooth_run(trn_ctx* ctx, const trn_mesh* mesh, double* xy_d, double* ten
       max_class = std::max(max_class, class_off[c + 1] - class_off[c]);
     Scratch sc(ctx, scratch_bytes(sizeof(int32_t) * (ncount + nlive)) +
                         scratch_bytes(sizeof(int32_t) * 2 * (size_t)(sweeps + 1)) +
-                        scratch_bytes(sizeof(double4) * (size_t)m->nv) +
                         scratch_bytes((size_t)m->nv) +
                         scratch_bytes(sizeof(int32_t) * (size_t)max_class));
     int32_t* counters = sc.carve<int32_t>(ncount + nlive);
     int32_t* live_count = counters + ncount;
     int32_t* totals = sc.carve<int32_t>(2 * (size_t)(sweeps + 1));
-    double4* logm = sc.carve<double4>(m->nv);
     uint8_t* stale = sc.carve<uint8_t>(m->nv);
     int32_t* live = sc.carve<int32_t>((size_t)max_class);
     VRecord* rec = m->vrec;
     memset_async(ctx, counters, 0, sizeof(int32_t) * (ncount + nlive));
     memset_async(ctx, stale, 0, (size_t)m->nv);
-    launch(ctx, "pack_records", 256.0 * m->nv, k_pack_records, dim3(nblk(m->nv, 256)),
-           dim3(256), 0, m->nv, (const double2*)xy_d, (const double*)tensor_d, rec, logm);
-    if (recompute_qualit) quality_cache_logs(ctx, m, xy_d, logm, qualit_d);
     int block;
     size_t shmem;
     phase_shape(m, &block, &shmem);
@@ -309,6 +317,16 @@ int trn_smooth_run(trn_ctx* ctx, const trn_mesh* mesh, double* xy_d, double* ten
       }
     }
   });
+}
+
+int trn_smooth_run(trn_ctx* ctx, const trn_mesh* mesh, double* xy_d, double* tensor_d,
+                   double* qualit_d, const int32_t* order_d, const int32_t* class_off,
+                   int32_t ncolors, int32_t sweeps, int32_t recompute_qualit, int32_t* commits,
+                   int32_t* tasks) {
+  const int rc = trn_smooth_prepare(ctx, mesh, xy_d, tensor_d, qualit_d, recompute_qualit);
+  if (rc != TRN_OK) return rc;
+  return trn_smooth_run_prepared(ctx, mesh, xy_d, tensor_d, qualit_d, order_d, class_off,
+                                 ncolors, sweeps, commits, tasks);
 }
 
 }  // extern "C"
