@@ -115,6 +115,77 @@ def test_metric_full_mesh_matches_oracle(ctx):
     assert normrel(r["metric"], st["metric"]).max() <= 1e-10
 
 
+def _wheel(n=60):
+    """A hub joined to a rim of n vertices, and an outer ring of n boundary
+    vertices joined to the rim: the hub has n incident triangles (generic
+    move path), every rim vertex's 2-ring holds all n fan triangles and
+    more (a slab too large for an LDS union), the outer slab is small."""
+    ang = 2 * np.pi * np.arange(n) / n
+    rim = np.stack([0.5 + 0.25 * np.cos(ang), 0.5 + 0.25 * np.sin(ang)], 1)
+    out = np.stack([0.5 + 0.45 * np.cos(ang + 0.01), 0.5 + 0.45 * np.sin(ang + 0.01)], 1)
+    xy = np.vstack([[0.5 + 0.013, 0.5 - 0.007], rim, out])  # off-centre hub
+    r = lambda k: 1 + k % n
+    o = lambda k: 1 + n + k % n
+    tri = []
+    for k in range(n):
+        tri += [[0, r(k), r(k + 1)], [r(k), o(k), o(k + 1)], [r(k), o(k + 1), r(k + 1)]]
+    tri = np.array(tri, dtype=np.int32)
+    tags = np.zeros(2 * n + 1, dtype=np.uint8)
+    tags[1 + n:] = mg.TAG_BOUND
+    solut = np.tanh(8 * (xy[:, 0] - 0.45)) + xy[:, 1] ** 2
+    return xy, tri, tags, solut
+
+
+def _renumbered(wl, seed=3):
+    """A workload with its vertices and triangles in random order, so no
+    64-vertex slab or vertex block is spatially compact."""
+    m = mg.workload(wl)
+    rng = np.random.default_rng(seed)
+    pv = rng.permutation(m.nv)            # old -> new vertex id
+    pt = rng.permutation(m.nt)            # new triangle k is old triangle pt[k]
+    xy = np.empty_like(m.xy)
+    xy[pv] = m.xy
+    tags = np.empty_like(m.tags)
+    tags[pv] = m.tags
+    solut = np.empty_like(m.solut)
+    solut[pv] = m.solut
+    tri = pv[m.tri[pt]].astype(np.int32)
+    return xy, tri, tags, solut
+
+
+@pytest.mark.parametrize("case", ["wheel60", "renumbered:jitter80:fronts+quad",
+                                  "renumbered:flip64:fronts"])
+def test_irregular_meshes_match_oracle(ctx, case):
+    """Metric (gradient and Hessian bit-exact) and smoothing (same colouring,
+    counters exact) on a hub of degree 60 (generic move path, patches too
+    large for a slab union) and on randomly renumbered grids (scattered
+    slabs, records and live lists)."""
+    import torch
+    if case == "wheel60":
+        xy, tri, tags, solut = _wheel(60)
+    else:
+        xy, tri, tags, solut = _renumbered(case.split(":", 1)[1])
+    st = oracle.metric_stages(xy, tri, solut, 0.8, 2, 1e-9, 0.4)
+    g = {"params": np.array([0.8, 2, 1e-9, 0.4, 1]), "xy_in": xy, "tri": tri, "tags": tags,
+         "solut": solut}
+    dm, _, r = _metric_on_gpu(ctx, g)
+    assert bits_equal(r["gradient"], st["gradient"])
+    assert bits_equal(r["hessian"], st["hessian"])
+    assert normrel(r["metric"], st["metric"]).max() <= 1e-10
+    col = T().color(ctx, dm, seed=2, max_colors=64)
+    cnp = col.color.cpu().numpy()
+    sweeps = 4
+    xo, to, qo, co, tao = oracle.smooth_run(xy, tri, tags, st["metric"], cnp, col.ncolors, sweeps,
+                                            topo=st["topo"])
+    xg, tg, qg, com, tas = _smooth_gpu(ctx, dm, xy, st["metric"], cnp, col.ncolors, sweeps)
+    assert tas == tao.tolist()
+    assert com == co.tolist()
+    assert normrel(xg, xo).max() <= 1e-12
+    assert normrel(tg, to).max() <= 1e-10
+    if case == "wheel60":
+        assert tas[0] == 61 and com[0] >= 1  # hub and rim are tasks; some moved
+
+
 @pytest.mark.parametrize("wl", ["uniform40:tanh", "jitter100:fronts"])
 def test_colouring_valid_deterministic_seeded(ctx, wl):
     m = mg.workload(wl)
