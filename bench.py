@@ -234,6 +234,21 @@ def read_traffic(kernel, workload):
     return None
 
 
+def read_valu(kernel, workload):
+    """FP64 flops and VALU wave instructions per launch of `kernel` on
+    `workload` from the committed rocprofv3 summary (profiles/valu.txt:
+    'kernel workload fp64_flops valu_wave_instr waves'), or None."""
+    path = os.path.join(ROOT, "profiles", "valu.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 4 and not line.startswith("#") and parts[0] == kernel \
+                and parts[1] == workload:
+            return float(parts[2]), float(parts[3])
+    return None
+
+
 def run_adjoint(T, mg, ctx, workload, steps):
     """Forward (metric + qualities + energy) and backward (dE/dxy, dE/df)
     of the energy sum_t (1 - q_M(t))^2."""
@@ -369,6 +384,17 @@ def main():
     avg_s = dstat["ms"] * 1e-3 / max(dstat["launches"], 1)
     achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
     traffic = read_traffic(dname, args.workload)
+    valu = read_valu(dname, args.workload)
+    fp64 = None
+    if valu and avg_s > 0:
+        flops, winstr = valu
+        fp64 = {"kernel": dname, "achieved": round(flops / avg_s / 1e12, 3),
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(flops / avg_s / 1e12 / FP64_PEAK_TFLOPS, 4),
+                "valu_issue_frac": round(winstr * 4 / (1024 * avg_s * 2.4e9), 3),
+                "note": ("FP64 flops per launch from rocprofv3 F64 VALU counters x 64 lanes "
+                         "(profiles/valu.txt) / HIP-event launch time; valu_issue_frac = VALU "
+                         "wave instructions x 4 cycles / (1024 SIMDs x time x 2.4 GHz)")}
 
     extras = {}
     if not args.no_extras:
@@ -426,6 +452,7 @@ def main():
                      "note": ("achieved = algorithmic bytes per launch / HIP-event launch time; "
                               "traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE and "
                               "WRITE_SIZE (profiles/traffic.txt)")},
+        "fp64_valu": fp64,
         "stages": stages,
         "extras": extras,
         "cpu_baseline": cpu,
