@@ -253,8 +253,12 @@ int trn_color(trn_ctx* ctx, const trn_mesh* mesh, uint64_t seed, int32_t max_col
         launch(ctx, "color_round", 0.0, k_color_round_total, dim3(1), dim3(kRoundSlots), 0,
                (const int32_t*)(slots + (size_t)r * kRoundSlots), totals + r);
       }
+      // the largest colour, valid once no loser is left, comes back with
+      // the convergence check in one synchronisation
+      reduce_max_i32(ctx, color_d, nv, flags + 4);
       copy_async(ctx, h, flags, sizeof(int32_t));
       copy_async(ctx, h + 1, totals + kRoundsPerSync - 1, sizeof(int32_t));
+      copy_async(ctx, h + 2, flags + 4, sizeof(int32_t));
       sync(ctx);
       rounds += kRoundsPerSync;
       TRN_REQUIRE(h[0] == 0, TRN_ERR_PALETTE,
@@ -262,10 +266,7 @@ int trn_color(trn_ctx* ctx, const trn_mesh* mesh, uint64_t seed, int32_t max_col
       if (h[1] == 0) break;
       TRN_REQUIRE(rounds < 100000, TRN_ERR_STATE, "colouring did not converge");
     }
-    reduce_max_i32(ctx, color_d, nv, flags + 4);
-    copy_async(ctx, h, flags + 4, sizeof(int32_t));
-    sync(ctx);
-    const int nc = h[0];
+    const int nc = h[2];
     build_classes(ctx, nv, color_d, mesh->perm, nc, order_d, class_off);
     *ncolors = nc;
   });

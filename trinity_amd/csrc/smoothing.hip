@@ -112,7 +112,7 @@ __global__ __launch_bounds__(64, 3) void k_smooth_phase(
   int commit = 0, task = 0;
   if (idx < n)
     commit = move_vertex_rec(order[idx], rec, qualit, stale, v2t_off, v2t, star, v2v_off, v2v,
-                             lds + threadIdx.x, blockDim.x, &task);
+                             lds + threadIdx.x, blockDim.x, count_d == nullptr, &task);
   // counters are spread over kCounterSlots addresses: one hot address
   // serialises the atomics of every wave in L2
   const uint64_t mc = __ballot(commit), mt = __ballot(task);
@@ -275,7 +275,6 @@ int trn_smooth_run_prepared(trn_ctx* ctx, const trn_mesh* mesh, double* xy_d, do
     int32_t* live = sc.carve<int32_t>((size_t)max_class);
     VRecord* rec = m->vrec;
     memset_async(ctx, counters, 0, sizeof(int32_t) * (ncount + nlive));
-    memset_async(ctx, stale, 0, (size_t)m->nv);
     int block;
     size_t shmem;
     phase_shape(m, &block, &shmem);

@@ -396,6 +396,8 @@ __device__ __forceinline__ int move_vertex(const int i, double2* __restrict__ xy
 // corners rewrites), so such a vertex would fail again with the same
 // result; the sweeps leave it out of the phase (it still counts as a task).
 // A failed move sets stale[i]; a commit clears the flags of the 1-ring.
+// In the first sweep (first != 0) every vertex also clears its own flag
+// before its move, so no flag is read before it was written in the call.
 // Within one phase only the vertex itself and vertices of other colours
 // touch stale[i], apart from concurrent commits that all store 0.
 __device__ __forceinline__ int move_vertex_rec(const int32_t i, VRecord* __restrict__ rec,
@@ -406,8 +408,9 @@ __device__ __forceinline__ int move_vertex_rec(const int32_t i, VRecord* __restr
                                                const int32_t* __restrict__ star,
                                                const int32_t* __restrict__ v2v_off,
                                                const int32_t* __restrict__ v2v, double* lds,
-                                               const int stride, int* task) {
+                                               const int stride, const int first, int* task) {
   const RecVerts v{rec};
+  if (first) stale[i] = 0;
   const double2 pini = rec[i].xy;
   const double T[3] = {rec[i].t[0], rec[i].t[1], rec[i].t[2]};
   const VTopo topo = rec[i].topo;
