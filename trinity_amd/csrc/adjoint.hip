@@ -47,24 +47,6 @@ __global__ __launch_bounds__(256) void k_adj_area(int32_t nt, const int32_t* __r
   if (t < nt) tri_grad_one(t, tri, xy, nullptr, area_t, nullptr);
 }
 
-__global__ __launch_bounds__(kAdjB) void k_adj_tarea(
-    int32_t nv, const int32_t* __restrict__ v2t_off, const int32_t* __restrict__ v2t,
-    const int32_t* __restrict__ v2v_off, const int32_t* __restrict__ v2v,
-    const double* __restrict__ area_t, double* __restrict__ tarea, int32_t cap,
-    int32_t* __restrict__ err) {
-  extern __shared__ int lds[];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nv) return;
-  bool ok;
-  int* s = lds + threadIdx.x;
-  const int m = build_patch(i, v2t_off, v2t, v2v_off, v2v, s, blockDim.x, cap, ok);
-  if (!ok) {
-    atomicOr(err, 1);
-    return;
-  }
-  vtx_grad_one<false>(i, s, blockDim.x, m, area_t, nullptr, nullptr, tarea);
-}
-
 __global__ __launch_bounds__(256) void k_adj_logm(int32_t nv, const double* __restrict__ M,
                                                   double4* __restrict__ logm) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -91,12 +73,18 @@ __global__ __launch_bounds__(256) void k_adj_energy(int32_t nt, const int32_t* _
 }
 
 // out[slot] = sum of partial[0..n) in order, times scale
-__global__ void k_adj_sum(const double* __restrict__ partial, int32_t n, double scale,
-                          double* __restrict__ out) {
-  if (threadIdx.x != 0) return;
+// One block: thread j adds partial[j], partial[j + B], ... in order, then a
+// fixed-shape block tree (wave shuffles, then waves in order) adds the
+// threads' sums, so the result does not depend on timing.
+constexpr int kSumB = 1024;
+__global__ __launch_bounds__(kSumB) void k_adj_sum(const double* __restrict__ partial,
+                                                   int32_t n, double scale,
+                                                   double* __restrict__ out) {
+  __shared__ double red[kSumB / 64];
   double s = 0.;
-  for (int k = 0; k < n; ++k) s += partial[k];
-  *out = s * scale;
+  for (int k = threadIdx.x; k < n; k += kSumB) s += partial[k];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) *out = s * scale;
 }
 
 // ---- backward ----------------------------------------------------------------
@@ -512,18 +500,15 @@ static double adj_forward(trn_ctx* ctx, trn_adjoint* a, const double* xy, const 
   copy_async(ctx, a->scal, ctx->pinned_f64, 2 * sizeof(double));
   launch(ctx, "adj_area", 0.0, k_adj_area, dim3(nblk(m->nt, 256)), dim3(256), 0, m->nt,
          (const int32_t*)m->tri, (const double2*)xy, a->area_t);
-  launch(ctx, "adj_tarea", 0.0, k_adj_tarea, dim3(nblk(m->nv, kAdjB)), dim3(kAdjB),
-         sizeof(int) * kAdjB * (size_t)m->patch_cap, m->nv, (const int32_t*)m->v2t_off,
-         (const int32_t*)m->v2t, (const int32_t*)m->v2v_off, (const int32_t*)m->v2v,
-         (const double*)a->area_t, a->TA, m->patch_cap, a->err);
+  patch_area_sums(ctx, m, (const double*)a->area_t, a->TA);
   launch(ctx, "adj_logm", 0.0, k_adj_logm, dim3(nblk(m->nv, 256)), dim3(256), 0, m->nv,
          (const double*)a->M, a->logm);
   const int eb = (int)std::min<int64_t>(nblk(m->nt, 256), a->npartial);
   launch(ctx, "adj_energy", 0.0, k_adj_energy, dim3(eb), dim3(256), 0, m->nt,
          (const int32_t*)m->tri, (const double2*)xy, (const double4*)a->logm, a->qualit,
          a->partial);
-  launch(ctx, "adj_sum", 0.0, k_adj_sum, dim3(1), dim3(64), 0, (const double*)a->partial, eb, 1.0,
-         a->scal + 3);
+  launch(ctx, "adj_sum", 0.0, k_adj_sum, dim3(1), dim3(kSumB), 0, (const double*)a->partial, eb,
+         1.0, a->scal + 3);
   copy_async(ctx, ctx->pinned_f64 + 2, a->scal + 3, sizeof(double));
   copy_async(ctx, ctx->pinned_i32, a->err, sizeof(int32_t));
   sync(ctx);
@@ -560,7 +545,7 @@ static void adj_backward(trn_ctx* ctx, trn_adjoint* a, double* dxy, double* df) 
          (const double*)a->QB, (const double*)a->Mloc, (const double*)a->scal, emin, emax,
          a->Mlb, a->partial);
   // C adjoint = (d E / d fact) * (d fact / d C) = factb * (-fact / C)
-  launch(ctx, "adj_sum", 0.0, k_adj_sum, dim3(1), dim3(64), 0, (const double*)a->partial, vb,
+  launch(ctx, "adj_sum", 0.0, k_adj_sum, dim3(1), dim3(kSumB), 0, (const double*)a->partial, vb,
          -a->fact / a->C, a->scal + 2);
   launch(ctx, "adj_local", 0.0, k_adj_local, dim3(vb), dim3(256), 0, m->nv,
          (const int32_t*)m->v2t_off, (const int32_t*)m->v2t, (const double*)a->area_t,

@@ -216,6 +216,15 @@ static size_t grad_shmem(const trn_mesh* m) {
   return (sizeof(double2) + sizeof(double)) * (size_t)m->max_union;
 }
 
+void patch_area_sums(trn_ctx* ctx, const trn_mesh* m, const double* area_t, double* tarea) {
+  launch(ctx, "vtx_area", 0.0, k_vtx_grad<false>, dim3(nblk(m->nv, kGatherB)), dim3(kGatherB),
+         grad_shmem(m), m->nv, (const int32_t*)m->patch_n, (const int32_t*)m->patch_row,
+         (const int32_t*)m->patch_ell, (const int32_t*)m->union_n,
+         (const int32_t*)m->union_off, (const int32_t*)m->union_ids,
+         (const uint16_t*)m->patch_local, m->max_union, area_t, (const double2*)nullptr,
+         (double2*)nullptr, tarea);
+}
+
 static void run_gradient(trn_ctx* ctx, const trn_mesh* m, const double* xy, const double* f,
                          double* grad, GradBuffers& b) {
   launch(ctx, "tri_grad", 0.0, k_tri_grad, dim3(nblk(m->nt, 256)), dim3(256), 0, m->nt,

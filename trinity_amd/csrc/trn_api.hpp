@@ -36,6 +36,11 @@ size_t exclusive_scan_scratch(int64_t n);
 // Z-curve order of the vertices from their coordinates (order.hip).
 void spatial_order(trn_ctx* ctx, int32_t nv, const double* xy, int32_t* perm);
 
+// Per-vertex sums of per-triangle areas over the stored 2-ring patches, in
+// patch order (the patch areas of Metrics::computeGradient,
+// src/hessian.cpp:53), from metric.hip's gather.
+void patch_area_sums(trn_ctx* ctx, const trn_mesh* m, const double* area_t, double* tarea);
+
 // Reductions used by several passes.
 void reduce_max_i32(trn_ctx* ctx, const int32_t* in, int64_t n, int32_t* out_d);
 
