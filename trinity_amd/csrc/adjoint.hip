@@ -200,26 +200,61 @@ __device__ __forceinline__ int ring_union(const int32_t n[3], const int32_t* __r
   return m;
 }
 
+// Stored ring unions, built with the adjoint.  SIZE: per triangle, the
+// size of its ring union (err set if one exceeds cap); FILL: the union written
+// column-wise into the triangle's slab, every column and row of the slab
+// written (-1 padding past the union or past the last triangle).
+template <bool FILL>
+__global__ __launch_bounds__(kAdjB) void k_tring_build(
+    int32_t nt, const int32_t* __restrict__ tri, const int32_t* __restrict__ v2v_off,
+    const int32_t* __restrict__ v2v, int32_t cap, int32_t* __restrict__ tr_n,
+    const int32_t* __restrict__ tr_row, int32_t* __restrict__ tr_ell, int32_t* __restrict__ err) {
+  extern __shared__ int lds[];
+  const int64_t t = (int64_t)blockIdx.x * kAdjB + threadIdx.x;
+  int* s = lds + threadIdx.x;
+  int m = 0;
+  bool ok = true;
+  if (t < nt && tri[3 * t] >= 0) {
+    const int32_t n[3] = {tri[3 * t], tri[3 * t + 1], tri[3 * t + 2]};
+    m = ring_union(n, v2v_off, v2v, s, kAdjB, cap, ok);
+  }
+  if (!FILL) {
+    if (!ok) atomicOr(err, 1);
+    if (t < nt) tr_n[t] = ok ? m : 0;
+    return;
+  }
+  const int32_t r0 = tr_row[blockIdx.x], w = tr_row[blockIdx.x + 1] - r0;
+  int32_t* col = tr_ell + (int64_t)r0 * kAdjB + threadIdx.x;
+  for (int e = 0; e < w; ++e) col[(int64_t)e * kAdjB] = (ok && e < m) ? s[e * kAdjB] : -1;
+}
+
+// Rows of each 64-triangle slab: its largest ring union (one wave per slab).
+__global__ __launch_bounds__(kAdjB) void k_tring_rows(int32_t nt, const int32_t* __restrict__ n,
+                                                      int32_t* __restrict__ rows) {
+  const int64_t t = (int64_t)blockIdx.x * kAdjB + threadIdx.x;
+  int m = t < nt ? n[t] : 0;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    const int o = __shfl_xor(m, d, 64);
+    m = o > m ? o : m;
+  }
+  if (threadIdx.x == 0) rows[blockIdx.x] = m;
+}
+
 // B5: Hessian recovery transpose per triangle
 __global__ __launch_bounds__(kAdjB) void k_adj_hess_tri(
     int32_t nt, const int32_t* __restrict__ tri, const double2* __restrict__ xy,
     const double2* __restrict__ G, const double* __restrict__ Sb,
-    const int32_t* __restrict__ v2v_off, const int32_t* __restrict__ v2v, int32_t cap,
-    double* __restrict__ AD, double* __restrict__ GB, int32_t* __restrict__ err) {
-  extern __shared__ int lds[];
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t* __restrict__ tr_n, const int32_t* __restrict__ tr_row,
+    const int32_t* __restrict__ tr_ell, double* __restrict__ AD, double* __restrict__ GB) {
+  const int64_t t = (int64_t)blockIdx.x * kAdjB + threadIdx.x;
   if (t >= nt || tri[3 * t] < 0) return;
   const int32_t n[3] = {tri[3 * t], tri[3 * t + 1], tri[3 * t + 2]};
-  bool ok;
-  int* s = lds + threadIdx.x;
-  const int m = ring_union(n, v2v_off, v2v, s, blockDim.x, cap, ok);
-  if (!ok) {
-    atomicOr(err, 1);
-    return;
-  }
+  const int32_t* s = tr_ell + (int64_t)tr_row[blockIdx.x] * kAdjB + threadIdx.x;
+  const int m = tr_n[t];
   double s0 = 0., s12 = 0., s3 = 0.;
   for (int e = 0; e < m; ++e) {
-    const double* b = Sb + 3 * (int64_t)s[e * blockDim.x];
+    const double* b = Sb + 3 * (int64_t)s[e * kAdjB];
     s0 += b[0];
     s12 += b[1];
     s3 += b[2];
@@ -276,23 +311,17 @@ __global__ __launch_bounds__(kAdjB) void k_adj_grad_tri(
     int32_t nt, const int32_t* __restrict__ tri, const double2* __restrict__ xy,
     const double* __restrict__ f, const double* __restrict__ Mloc,
     const double2* __restrict__ Sgb, const double* __restrict__ TAb,
-    const int32_t* __restrict__ v2v_off, const int32_t* __restrict__ v2v,
-    const double* __restrict__ scal, int32_t cap, const double* __restrict__ AD,
-    double* __restrict__ QB, double* __restrict__ FB, int32_t* __restrict__ err) {
-  extern __shared__ int lds[];
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t* __restrict__ tr_n, const int32_t* __restrict__ tr_row,
+    const int32_t* __restrict__ tr_ell, const double* __restrict__ scal,
+    const double* __restrict__ AD, double* __restrict__ QB, double* __restrict__ FB) {
+  const int64_t t = (int64_t)blockIdx.x * kAdjB + threadIdx.x;
   if (t >= nt || tri[3 * t] < 0) return;
   const int32_t n[3] = {tri[3 * t], tri[3 * t + 1], tri[3 * t + 2]};
-  bool ok;
-  int* s = lds + threadIdx.x;
-  const int m = ring_union(n, v2v_off, v2v, s, blockDim.x, cap, ok);
-  if (!ok) {
-    atomicOr(err, 1);
-    return;
-  }
+  const int32_t* s = tr_ell + (int64_t)tr_row[blockIdx.x] * kAdjB + threadIdx.x;
+  const int m = tr_n[t];
   double sx = 0., sy = 0., tb = 0.;
   for (int e = 0; e < m; ++e) {
-    const int i = s[e * blockDim.x];
+    const int i = s[e * kAdjB];
     const double2 v = Sgb[i];
     sx += v.x;
     sy += v.y;
@@ -454,6 +483,13 @@ struct trn_adjoint {
   double* partial = nullptr;
   int32_t npartial = 0;
   int32_t* err = nullptr;
+  // per triangle, the ascending union of its corners' closed 1-rings (the
+  // vertices whose 2-ring patch holds it), sliced ELL over 64-triangle
+  // slabs like the vertex patches: entry e of triangle t = b * kAdjB + l is
+  // tr_ell[(tr_row[b] + e) * kAdjB + l], e < tr_n[t], -1 padding
+  int32_t* tr_n = nullptr;
+  int32_t* tr_row = nullptr;
+  int32_t* tr_ell = nullptr;
   // smoothing
   double* gxy = nullptr;
   double* xy_try = nullptr;
@@ -534,8 +570,6 @@ static void adj_backward(trn_ctx* ctx, trn_adjoint* a, double* dxy, double* df) 
   TRN_REQUIRE(a->have_tape, TRN_ERR_STATE, "no forward to differentiate");
   const trn_mesh* m = a->mesh;
   const double2* xy = (const double2*)a->xy;
-  const int32_t cap = ring_cap(m);
-  const size_t lds = sizeof(int) * kAdjB * (size_t)cap;
   const double emin = 1 / (a->hmin * a->hmin), emax = 1 / (a->hmax * a->hmax);
   launch(ctx, "adj_quality", 0.0, k_adj_quality, dim3(nblk(m->nt, 256)), dim3(256), 0, m->nt,
          (const int32_t*)m->tri, xy, (const double*)a->M, (const double*)a->qualit, a->QB);
@@ -551,17 +585,18 @@ static void adj_backward(trn_ctx* ctx, trn_adjoint* a, double* dxy, double* df) 
          (const int32_t*)m->v2t_off, (const int32_t*)m->v2t, (const double*)a->area_t,
          (const double*)a->Mloc, (const double*)a->Hs, (const double*)a->TA,
          (const double*)a->scal, metric_exponent(a->p), (const double*)a->Mlb, a->Sb, a->TAb);
-  launch(ctx, "adj_hess_tri", 0.0, k_adj_hess_tri, dim3(nblk(m->nt, kAdjB)), dim3(kAdjB), lds,
+  launch(ctx, "adj_hess_tri", 0.0, k_adj_hess_tri, dim3(nblk(m->nt, kAdjB)), dim3(kAdjB), 0,
          m->nt, (const int32_t*)m->tri, xy, (const double2*)a->G, (const double*)a->Sb,
-         (const int32_t*)m->v2v_off, (const int32_t*)m->v2v, cap, a->AD, a->GB, a->err);
+         (const int32_t*)a->tr_n, (const int32_t*)a->tr_row, (const int32_t*)a->tr_ell, a->AD,
+         a->GB);
   launch(ctx, "adj_grad_vtx", 0.0, k_adj_grad_vtx, dim3(vb), dim3(256), 0, m->nv,
          (const int32_t*)m->v2t_off, (const int32_t*)m->v2t, (const int32_t*)m->star,
          (const double*)a->GB, (const double2*)a->G, (const double*)a->TA, a->TAb, a->Sgb);
-  launch(ctx, "adj_grad_tri", 0.0, k_adj_grad_tri, dim3(nblk(m->nt, kAdjB)), dim3(kAdjB), lds,
+  launch(ctx, "adj_grad_tri", 0.0, k_adj_grad_tri, dim3(nblk(m->nt, kAdjB)), dim3(kAdjB), 0,
          m->nt, (const int32_t*)m->tri, xy, a->f, (const double*)a->Mloc,
-         (const double2*)a->Sgb, (const double*)a->TAb, (const int32_t*)m->v2v_off,
-         (const int32_t*)m->v2v, (const double*)a->scal, cap, (const double*)a->AD, a->QB, a->FB,
-         a->err);
+         (const double2*)a->Sgb, (const double*)a->TAb, (const int32_t*)a->tr_n,
+         (const int32_t*)a->tr_row, (const int32_t*)a->tr_ell, (const double*)a->scal,
+         (const double*)a->AD, a->QB, a->FB);
   launch(ctx, "adj_gather", kBytesBackwardPerV * m->nv, k_adj_gather, dim3(vb), dim3(256), 0,
          m->nv, (const int32_t*)m->v2t_off, (const int32_t*)m->v2t, (const int32_t*)m->star,
          (const double*)a->QB, (const double*)a->FB, (double2*)dxy, df);
@@ -614,6 +649,32 @@ int trn_adjoint_create(trn_ctx* ctx, const trn_mesh* mesh, trn_adjoint** out) {
       adj_alloc(&a->gxy, 2 * nv);
       adj_alloc(&a->xy_try, 2 * nv);
       memset_async(ctx, a->err, 0, sizeof(int32_t));
+      // ring unions of the triangles, stored for the recovery transposes
+      const int32_t cap = ring_cap(mesh);
+      const size_t lds = sizeof(int) * kAdjB * (size_t)cap;
+      const int64_t nslab = (int64_t)nblk(nt, kAdjB);
+      adj_alloc(&a->tr_n, nt);
+      adj_alloc(&a->tr_row, (size_t)nslab + 1);
+      launch(ctx, "adj_tring_size", 0.0, k_tring_build<false>, dim3((unsigned)nslab),
+             dim3(kAdjB), lds, mesh->nt, (const int32_t*)mesh->tri,
+             (const int32_t*)mesh->v2v_off, (const int32_t*)mesh->v2v, cap, a->tr_n,
+             (const int32_t*)nullptr, (int32_t*)nullptr, a->err);
+      Scratch sc(ctx, exclusive_scan_scratch(nslab));
+      int32_t* tmp = sc.carve<int32_t>(exclusive_scan_scratch(nslab) / 4);
+      launch(ctx, "adj_tring_rows", 0.0, k_tring_rows, dim3((unsigned)nslab), dim3(kAdjB), 0,
+             mesh->nt, (const int32_t*)a->tr_n, a->tr_row);
+      exclusive_scan_i32(ctx, a->tr_row, a->tr_row, nslab, true, tmp);
+      copy_async(ctx, ctx->pinned_i32, a->tr_row + nslab, sizeof(int32_t));
+      copy_async(ctx, ctx->pinned_i32 + 1, a->err, sizeof(int32_t));
+      sync(ctx);
+      TRN_REQUIRE(ctx->pinned_i32[1] == 0, TRN_ERR_CAPACITY,
+                  "a triangle's ring union exceeds " + std::to_string(cap) + " vertices");
+      const int64_t rows = ctx->pinned_i32[0];
+      adj_alloc(&a->tr_ell, (size_t)std::max<int64_t>(rows, 1) * kAdjB);
+      launch(ctx, "adj_tring_fill", 0.0, k_tring_build<true>, dim3((unsigned)nslab),
+             dim3(kAdjB), lds, mesh->nt, (const int32_t*)mesh->tri,
+             (const int32_t*)mesh->v2v_off, (const int32_t*)mesh->v2v, cap, a->tr_n,
+             (const int32_t*)a->tr_row, a->tr_ell, a->err);
       sync(ctx);
     } catch (...) {
       trn_adjoint_destroy(a);
@@ -633,6 +694,7 @@ int trn_adjoint_destroy(trn_adjoint* a) {
     adj_free(a->Mlb); adj_free(a->Sb); adj_free(a->TAb); adj_free(a->Sgb); adj_free(a->QB);
     adj_free(a->GB); adj_free(a->AD); adj_free(a->FB); adj_free(a->partial); adj_free(a->err);
     adj_free(a->gxy); adj_free(a->xy_try);
+    adj_free(a->tr_n); adj_free(a->tr_row); adj_free(a->tr_ell);
     delete a;
   });
 }
