@@ -14,12 +14,19 @@
 // Two adjacent winners of one round hold different tentative colours and
 // every tentative colour avoids all committed neighbours, so the colouring
 // is proper after every round; the top-priority vertex of every conflict
-// group commits, so each round makes progress.
+// group commits, so each round makes progress.  In the first round every
+// tentative colour is 1, so that round is the resolve kernel alone.  A
+// round reads the previous round's loser count itself and returns at once
+// when it is zero, so a batch of rounds is queued without host checks.
 //
 // The colour classes (Partit::task.subset/cardin/nb.parts, :171-194) are a
 // stable counting sort: per-block colour histograms, one exclusive scan in
-// colour-major order, and a scatter that ranks equal colours inside a wave
-// with ballots, so each class lists its vertices in ascending order.
+// colour-major order (a single block, which also gathers the convergence
+// check, the number of colours and the class offsets for the host), and a
+// scatter that ranks equal colours inside a wave with ballots, so each
+// class lists its vertices in ascending order.  The class build is queued
+// behind the batch of rounds, so a colouring that converges within one
+// batch costs one host synchronisation.
 #include <string>
 #include <vector>
 
@@ -27,7 +34,9 @@
 
 namespace trn {
 
-constexpr int kClassB = 256;
+constexpr int kClassB = 1024;  // vertices per block of the class build
+constexpr int kScanB = 1024;   // threads of the single-block scan
+constexpr int kMaxRows = 64;   // colours a class build handles
 
 // splitmix64 finaliser (Steele, Lea, Flood 2014) as the seeded priority.
 __device__ __forceinline__ uint32_t color_priority(uint64_t seed, uint32_t v) {
@@ -41,13 +50,28 @@ __device__ __forceinline__ uint32_t color_priority(uint64_t seed, uint32_t v) {
 constexpr int kRoundSlots = 64;   // per-round loser counters, spread over addresses
 constexpr int kRoundsPerSync = 8;  // rounds queued between two host checks
 
+// Losers left by the previous round: the sum of its kRoundSlots counters.
+// The addresses are the same in every thread, so the loads are scalar
+// (one scalar-cache line per CU) and no barrier is needed.
+// prev_slots == nullptr: no previous round in this batch, taken as "losers
+// left".
+__device__ __forceinline__ int prev_losers(const int32_t* __restrict__ prev_slots) {
+  if (!prev_slots) return 1;
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < kRoundSlots; ++k) s += prev_slots[k];
+  return s;
+}
+
 // Rounds are queued in batches; a round whose predecessor left no loser
-// returns at once (prev_total == 0), so over-queued rounds cost a launch.
-__global__ void k_color_tent(int32_t nv, const int32_t* __restrict__ v2v_off,
-                             const int32_t* __restrict__ v2v, const int32_t* __restrict__ color,
-                             int32_t* __restrict__ tent, int max_colors, int32_t* fail,
-                             const int32_t* __restrict__ prev_total) {
-  if (prev_total && *prev_total == 0) return;
+// returns at once, so over-queued rounds cost a launch each.
+__global__ __launch_bounds__(256) void k_color_tent(int32_t nv, const int32_t* __restrict__ v2v_off,
+                                                    const int32_t* __restrict__ v2v,
+                                                    const int32_t* __restrict__ color,
+                                                    int32_t* __restrict__ tent, int max_colors,
+                                                    int32_t* fail,
+                                                    const int32_t* __restrict__ prev_slots) {
+  if (prev_losers(prev_slots) == 0) return;
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nv) return;
   const int32_t b = v2v_off[v], e = v2v_off[v + 1];
@@ -70,23 +94,40 @@ __global__ void k_color_tent(int32_t nv, const int32_t* __restrict__ v2v_off,
   tent[v] = c;
 }
 
+// first != 0: the first round of the colouring.  No vertex is coloured
+// yet, so every vertex with a neighbour would take colour 1 (max_colors >=
+// 1): the tentative kernel is skipped, tent[] is not read, and every colour
+// is written (0 for losers and isolated vertices), which also clears the
+// colours of an earlier call.
 __global__ __launch_bounds__(256) void k_color_resolve(int32_t nv,
                                                        const int32_t* __restrict__ v2v_off,
                                                        const int32_t* __restrict__ v2v,
                                                        int32_t* __restrict__ color,
                                                        const int32_t* __restrict__ tent,
-                                                       uint64_t seed, int32_t* remaining,
-                                                       const int32_t* __restrict__ prev_total) {
-  if (prev_total && *prev_total == 0) return;
+                                                       uint64_t seed, int first,
+                                                       int32_t* remaining,
+                                                       const int32_t* __restrict__ prev_slots) {
+  if (prev_losers(prev_slots) == 0) return;
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int lose = 0;
   if (v < nv) {
-    const int32_t tv = tent[v];
+    int32_t tv, b = 0, e = 0;
+    if (first) {
+      b = v2v_off[v];
+      e = v2v_off[v + 1];
+      tv = b != e ? 1 : 0;
+    } else {
+      tv = tent[v];
+      if (tv != 0) {
+        b = v2v_off[v];
+        e = v2v_off[v + 1];
+      }
+    }
     if (tv != 0) {
       const uint32_t pv = color_priority(seed, (uint32_t)v);
-      for (int32_t a = v2v_off[v]; a < v2v_off[v + 1]; ++a) {
+      for (int32_t a = b; a < e; ++a) {
         const int32_t u = v2v[a];
-        if (tent[u] == tv) {
+        if ((first ? 1 : tent[u]) == tv) {
           const uint32_t pu = color_priority(seed, (uint32_t)u);
           if (pu > pv || (pu == pv && u > v)) {
             lose = 1;
@@ -96,20 +137,12 @@ __global__ __launch_bounds__(256) void k_color_resolve(int32_t nv,
       }
       if (!lose) color[v] = tv;
     }
+    if (first && (tv == 0 || lose)) color[v] = 0;
   }
   // one atomic per block, spread over kRoundSlots addresses (a single hot
   // address serialises in L2)
   const int losers = __syncthreads_count(lose);
   if (threadIdx.x == 0 && losers) atomicAdd(&remaining[blockIdx.x & (kRoundSlots - 1)], losers);
-}
-
-// total[0] = sum of the round's slots, or 0 when the previous round was
-// already empty (its slots were never written).
-__global__ void k_color_round_total(const int32_t* __restrict__ slots, int32_t* __restrict__ total) {
-  int s = slots[threadIdx.x];
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if (threadIdx.x == 0) total[0] = s;
 }
 
 __global__ void k_color_check(int32_t nv, const int32_t* __restrict__ v2v_off,
@@ -180,36 +213,113 @@ __global__ __launch_bounds__(kClassB) void k_class_scatter(int32_t nv, const int
   }
 }
 
-__global__ void k_class_offsets(const int32_t* __restrict__ off, int ncolors, int nblocks,
-                                int32_t* __restrict__ out) {
-  const int k = threadIdx.x;
-  if (k <= ncolors) out[k] = off[(int64_t)k * nblocks];
+// Exclusive scan, in one block, of the colour-major per-block counts cnt
+// (rows x nblocks) into off (rows * nblocks + 1 entries), and the status
+// words the host reads back in one copy:
+//   status[0]     palette failure flag (0 without fail),
+//   status[1]     losers of the last queued round (0 without last_slots),
+//   status[2]     colours in use: the largest c <= rows with a vertex,
+//   status[3 + c] first position of colour c + 1 in the class order, for
+//                 c = 0..rows (status[3 + rows] = number of listed vertices).
+// Each thread scans a contiguous chunk; the chunk totals are scanned in LDS.
+__global__ __launch_bounds__(kScanB) void k_class_scan(const int32_t* __restrict__ cnt, int rows,
+                                                       int64_t nblocks, int32_t* __restrict__ off,
+                                                       const int32_t* __restrict__ fail,
+                                                       const int32_t* __restrict__ last_slots,
+                                                       int32_t* __restrict__ status) {
+  __shared__ int part[kScanB];
+  __shared__ int coff[kMaxRows + 1];
+  const int t = threadIdx.x;
+  const int64_t n = (int64_t)rows * nblocks;
+  const int64_t per = (n + kScanB - 1) / kScanB;
+  const int64_t b0 = per * t < n ? per * t : n;
+  const int64_t b1 = b0 + per < n ? b0 + per : n;
+  if (t <= rows) coff[t] = 0;
+  int s = 0;
+  for (int64_t k = b0; k < b1; ++k) s += cnt[k];
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < kScanB; o <<= 1) {  // inclusive scan of the chunk totals
+    const int x = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += x;
+    __syncthreads();
+  }
+  int run = t ? part[t - 1] : 0;
+  // the first row start at or after b0; rows start at multiples of nblocks
+  int64_t next = nblocks ? (b0 + nblocks - 1) / nblocks * nblocks : n;
+  for (int64_t k = b0; k < b1; ++k) {
+    if (k == next) {
+      coff[k / nblocks] = run;
+      next += nblocks;
+    }
+    const int c = cnt[k];
+    off[k] = run;
+    run += c;
+  }
+  if (t == kScanB - 1) {
+    off[n] = run;
+    coff[rows] = run;
+  }
+  __syncthreads();
+  if (t <= rows) status[3 + t] = coff[t];
+  if (t == 0) {
+    int nc = 0;
+    for (int c = 0; c < rows; ++c)
+      if (coff[c + 1] > coff[c]) nc = c + 1;
+    status[0] = fail ? *fail : 0;
+    status[2] = nc;
+  }
+  if (t < kRoundSlots) {
+    int l = last_slots ? last_slots[t] : 0;
+#pragma unroll
+    for (int o = kRoundSlots / 2; o > 0; o >>= 1) l += __shfl_xor(l, o, kRoundSlots);
+    if (t == 0) status[1] = l;
+  }
 }
 
 static unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-// Builds order_d and class_off (host) for colours 1..ncolors, visiting the
-// vertices in perm order when given.
-static void build_classes(trn_ctx* ctx, int32_t nv, const int32_t* color, const int32_t* perm,
-                          int ncolors, int32_t* order, int32_t* class_off) {
+constexpr int kStatusWords = 3 + kMaxRows + 1;
+
+struct ClassBufs {
+  int32_t* cnt;
+  int32_t* off;
+  int32_t* status;
+};
+
+static size_t class_bytes(int32_t nv, int rows) {
+  const int64_t n = (int64_t)nblk(nv, kClassB) * rows;
+  return 2 * scratch_bytes(sizeof(int32_t) * (size_t)(n + 1)) +
+         scratch_bytes(sizeof(int32_t) * kStatusWords);
+}
+
+static ClassBufs carve_classes(Scratch& sc, int32_t nv, int rows) {
+  const int64_t n = (int64_t)nblk(nv, kClassB) * rows;
+  ClassBufs b;
+  b.cnt = sc.carve<int32_t>((size_t)n + 1);
+  b.off = sc.carve<int32_t>((size_t)n + 1);
+  b.status = sc.carve<int32_t>(kStatusWords);
+  return b;
+}
+
+// Queues the class build for colours 1..rows (per-block counts, the scan,
+// which also fills the status words, and the scatter into order, visiting
+// the vertices in perm order when given) and the copy of the status words
+// to ctx->pinned_i32.  The caller synchronises before reading them.
+static void queue_classes(trn_ctx* ctx, int32_t nv, const int32_t* color, const int32_t* perm,
+                          int rows, const int32_t* fail, const int32_t* last_slots,
+                          int32_t* order, const ClassBufs& b) {
   const int64_t nb = nblk(nv, kClassB);
-  const int64_t n = nb * ncolors;
-  Scratch sc(ctx, scratch_bytes(sizeof(int32_t) * (size_t)(n + 1)) * 2 +
-                      exclusive_scan_scratch(n) + scratch_bytes(sizeof(int32_t) * 80));
-  int32_t* cnt = sc.carve<int32_t>(n + 1);
-  int32_t* off = sc.carve<int32_t>(n + 1);
-  int32_t* tmp = sc.carve<int32_t>(exclusive_scan_scratch(n) / 4);
-  int32_t* offs = sc.carve<int32_t>(80);
-  launch(ctx, "class_count", 4.0 * nv, k_class_count, dim3((unsigned)nb), dim3(kClassB), 0, nv,
-         color, perm, ncolors, cnt);
-  exclusive_scan_i32(ctx, cnt, off, n, true, tmp);
-  launch(ctx, "class_scatter", 8.0 * nv, k_class_scatter, dim3((unsigned)nb), dim3(kClassB), 0,
-         nv, color, perm, ncolors, (const int32_t*)off, order);
-  launch(ctx, "class_offsets", 0.0, k_class_offsets, dim3(1), dim3(128), 0, (const int32_t*)off,
-         ncolors, (int)nb, offs);
-  copy_async(ctx, ctx->pinned_i32, offs, sizeof(int32_t) * (size_t)(ncolors + 1));
-  sync(ctx);
-  for (int k = 0; k <= ncolors; ++k) class_off[k] = ctx->pinned_i32[k];
+  if (nb > 0)
+    launch(ctx, "class_count", 4.0 * nv, k_class_count, dim3((unsigned)nb), dim3(kClassB), 0, nv,
+           color, perm, rows, b.cnt);
+  launch(ctx, "class_scan", 0.0, k_class_scan, dim3(1), dim3(kScanB), 0, (const int32_t*)b.cnt,
+         rows, nb, b.off, fail, last_slots, b.status);
+  if (nb > 0)
+    launch(ctx, "class_scatter", 8.0 * nv, k_class_scatter, dim3((unsigned)nb), dim3(kClassB), 0,
+           nv, color, perm, rows, (const int32_t*)b.off, order);
+  copy_async(ctx, ctx->pinned_i32, b.status, sizeof(int32_t) * (size_t)(3 + rows + 1));
 }
 
 }  // namespace trn
@@ -224,41 +334,41 @@ int trn_color(trn_ctx* ctx, const trn_mesh* mesh, uint64_t seed, int32_t max_col
     DeviceScope ds(ctx);
     TRN_REQUIRE(mesh != nullptr && color_d && order_d && class_off && ncolors, TRN_ERR_ARG,
                 "null argument");
-    TRN_REQUIRE(max_colors >= 1 && max_colors <= 64, TRN_ERR_ARG, "max_colors must be in 1..64");
+    TRN_REQUIRE(max_colors >= 1 && max_colors <= kMaxRows, TRN_ERR_ARG,
+                "max_colors must be in 1..64");
     const int32_t nv = mesh->nv;
+    // first fit gives a vertex at most (1-ring size + 1) colours
+    const int rows = max_colors < mesh->max_deg_v + 1 ? max_colors : mesh->max_deg_v + 1;
     const size_t nslots = (size_t)kRoundsPerSync * kRoundSlots;
-    Scratch sc(ctx, scratch_bytes(sizeof(int32_t) * (size_t)nv) + scratch_bytes(64) +
-                        scratch_bytes(sizeof(int32_t) * nslots) +
-                        scratch_bytes(sizeof(int32_t) * kRoundsPerSync));
+    Scratch sc(ctx, scratch_bytes(sizeof(int32_t) * (size_t)nv) +
+                        scratch_bytes(sizeof(int32_t) * (16 + nslots)) + class_bytes(nv, rows));
     int32_t* tent = sc.carve<int32_t>(nv);
-    int32_t* flags = sc.carve<int32_t>(16);
-    int32_t* slots = sc.carve<int32_t>(nslots);
-    int32_t* totals = sc.carve<int32_t>(kRoundsPerSync);
-    memset_async(ctx, color_d, 0, sizeof(int32_t) * (size_t)nv);
-    memset_async(ctx, flags, 0, sizeof(int32_t) * 16);
-    int32_t* h = ctx->pinned_i32;
+    int32_t* flags = sc.carve<int32_t>(16 + nslots);  // [0] palette failure, then the slots
+    int32_t* slots = flags + 16;
+    const ClassBufs cb = carve_classes(sc, nv, rows);
+    memset_async(ctx, flags, 0, sizeof(int32_t) * (16 + nslots));
+    const int32_t* h = ctx->pinned_i32;
     int rounds = 0;
     for (;;) {
-      memset_async(ctx, slots, 0, sizeof(int32_t) * nslots);
+      if (rounds) memset_async(ctx, slots, 0, sizeof(int32_t) * nslots);
       for (int r = 0; r < kRoundsPerSync; ++r) {
-        const int32_t* prev = r ? totals + r - 1 : nullptr;
-        launch(ctx, "color_tent", 4.0 * nv + 4.0 * (double)mesh->nnz_v2v, k_color_tent,
-               dim3(nblk(nv, 256)), dim3(256), 0, nv, (const int32_t*)mesh->v2v_off,
-               (const int32_t*)mesh->v2v, (const int32_t*)color_d, tent, (int)max_colors, flags,
-               prev);
+        const int32_t* prev = r ? slots + (size_t)(r - 1) * kRoundSlots : nullptr;
+        const int first = rounds == 0 && r == 0;
+        if (!first)
+          launch(ctx, "color_tent", 4.0 * nv + 4.0 * (double)mesh->nnz_v2v, k_color_tent,
+                 dim3(nblk(nv, 256)), dim3(256), 0, nv, (const int32_t*)mesh->v2v_off,
+                 (const int32_t*)mesh->v2v, (const int32_t*)color_d, tent, (int)max_colors,
+                 flags, prev);
         launch(ctx, "color_resolve", 4.0 * nv + 4.0 * (double)mesh->nnz_v2v, k_color_resolve,
                dim3(nblk(nv, 256)), dim3(256), 0, nv, (const int32_t*)mesh->v2v_off,
-               (const int32_t*)mesh->v2v, color_d, (const int32_t*)tent, seed,
+               (const int32_t*)mesh->v2v, color_d, (const int32_t*)tent, seed, first,
                slots + (size_t)r * kRoundSlots, prev);
-        launch(ctx, "color_round", 0.0, k_color_round_total, dim3(1), dim3(kRoundSlots), 0,
-               (const int32_t*)(slots + (size_t)r * kRoundSlots), totals + r);
       }
-      // the largest colour, valid once no loser is left, comes back with
-      // the convergence check in one synchronisation
-      reduce_max_i32(ctx, color_d, nv, flags + 4);
-      copy_async(ctx, h, flags, sizeof(int32_t));
-      copy_async(ctx, h + 1, totals + kRoundsPerSync - 1, sizeof(int32_t));
-      copy_async(ctx, h + 2, flags + 4, sizeof(int32_t));
+      // the class build is queued with the batch: when the batch has
+      // coloured every vertex (the usual case) one synchronisation returns
+      // the convergence check, the number of colours and the class offsets
+      queue_classes(ctx, nv, color_d, mesh->perm, rows, flags,
+                    slots + (size_t)(kRoundsPerSync - 1) * kRoundSlots, order_d, cb);
       sync(ctx);
       rounds += kRoundsPerSync;
       TRN_REQUIRE(h[0] == 0, TRN_ERR_PALETTE,
@@ -267,7 +377,7 @@ int trn_color(trn_ctx* ctx, const trn_mesh* mesh, uint64_t seed, int32_t max_col
       TRN_REQUIRE(rounds < 100000, TRN_ERR_STATE, "colouring did not converge");
     }
     const int nc = h[2];
-    build_classes(ctx, nv, color_d, mesh->perm, nc, order_d, class_off);
+    for (int k = 0; k <= nc; ++k) class_off[k] = h[3 + k];
     *ncolors = nc;
   });
 }
@@ -277,9 +387,14 @@ int trn_color_classes(trn_ctx* ctx, const trn_mesh* mesh, int32_t nv, const int3
   return guard_impl("trn_color_classes", [&] {
     DeviceScope ds(ctx);
     TRN_REQUIRE(nv >= 0 && color_d && order_d && class_off, TRN_ERR_ARG, "null argument");
-    TRN_REQUIRE(ncolors >= 1 && ncolors <= 64, TRN_ERR_ARG, "ncolors must be in 1..64");
+    TRN_REQUIRE(ncolors >= 1 && ncolors <= kMaxRows, TRN_ERR_ARG, "ncolors must be in 1..64");
     TRN_REQUIRE(!mesh || mesh->nv == nv, TRN_ERR_ARG, "nv differs from the mesh");
-    build_classes(ctx, nv, color_d, mesh ? mesh->perm : nullptr, ncolors, order_d, class_off);
+    Scratch sc(ctx, class_bytes(nv, ncolors));
+    const ClassBufs cb = carve_classes(sc, nv, ncolors);
+    queue_classes(ctx, nv, color_d, mesh ? mesh->perm : nullptr, ncolors, nullptr, nullptr,
+                  order_d, cb);
+    sync(ctx);
+    for (int k = 0; k <= ncolors; ++k) class_off[k] = ctx->pinned_i32[3 + k];
   });
 }
 
