@@ -124,11 +124,15 @@ __global__ __launch_bounds__(64, 3) void k_smooth_phase(
 }
 
 // Live list of a colour class: the vertices not marked stale, appended one
-// block of kLiveB class entries at a time with one atomic per block (a
-// block's vertices keep their class order, so each phase block still covers
-// a compact region; the order of the blocks does not change the result, a
-// class being an independent set).  Stale vertices are counted as tasks.
+// block of kLiveB * kLivePer class entries at a time with one atomic per
+// block (a single counter takes every block's atomic in turn, so fewer,
+// larger blocks finish sooner).  A block's vertices keep their class order,
+// so each phase block still covers a compact region; the order of the
+// blocks does not change the result, a class being an independent set.
+// Stale vertices are counted as tasks.
 constexpr int kLiveB = 1024;
+constexpr int kLivePer = 4;
+constexpr int kLiveGroups = kLivePer * (kLiveB / 64);  // (entry round, wave) groups
 
 __global__ __launch_bounds__(kLiveB) void k_class_live(int32_t count,
                                                        const int32_t* __restrict__ order,
@@ -136,35 +140,45 @@ __global__ __launch_bounds__(kLiveB) void k_class_live(int32_t count,
                                                        int32_t* __restrict__ live,
                                                        int32_t* __restrict__ live_count,
                                                        int32_t* __restrict__ counters) {
-  __shared__ int32_t wave_live[kLiveB / 64], wave_skip[kLiveB / 64], base;
-  const int64_t idx = (int64_t)xcd_block() * kLiveB + threadIdx.x;
-  int32_t i = 0;
-  bool is_live = false, skipped = false;
-  if (idx < count) {
-    i = order[idx];
-    skipped = stale[i] != 0;
-    is_live = !skipped;
-  }
-  const uint64_t ml = __ballot(is_live), ms = __ballot(skipped);
+  __shared__ int32_t grp_live[kLiveGroups], wave_skip[kLiveB / 64], base;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) {
-    wave_live[wave] = (int32_t)__popcll(ml);
-    wave_skip[wave] = (int32_t)__popcll(ms);
+  const int64_t first = (int64_t)xcd_block() * (kLiveB * kLivePer) + threadIdx.x;
+  const uint64_t below = (1ull << lane) - 1;
+  int32_t v[kLivePer];
+  uint64_t ml[kLivePer];
+  int nskip = 0;
+#pragma unroll
+  for (int j = 0; j < kLivePer; ++j) {  // entry first + j * kLiveB: class order is (j, thread)
+    const int64_t idx = first + (int64_t)j * kLiveB;
+    bool is_live = false, skipped = false;
+    v[j] = 0;
+    if (idx < count) {
+      v[j] = order[idx];
+      skipped = stale[v[j]] != 0;
+      is_live = !skipped;
+    }
+    ml[j] = __ballot(is_live);
+    nskip += (int)__popcll(__ballot(skipped));
+    if (lane == 0) grp_live[j * (kLiveB / 64) + wave] = (int32_t)__popcll(ml[j]);
   }
+  if (lane == 0) wave_skip[wave] = nskip;
   __syncthreads();
   if (threadIdx.x == 0) {
     int32_t nl = 0, nsk = 0;
-    for (int w = 0; w < kLiveB / 64; ++w) {
-      const int32_t c = wave_live[w];
-      wave_live[w] = nl;  // exclusive prefix
+    for (int g = 0; g < kLiveGroups; ++g) {
+      const int32_t c = grp_live[g];
+      grp_live[g] = nl;  // exclusive prefix in class order
       nl += c;
-      nsk += wave_skip[w];
     }
+    for (int w = 0; w < kLiveB / 64; ++w) nsk += wave_skip[w];
     base = nl ? atomicAdd(live_count, nl) : 0;
     if (nsk) atomicAdd(&counters[2 * (blockIdx.x & (kCounterSlots - 1)) + 1], nsk);
   }
   __syncthreads();
-  if (is_live) live[base + wave_live[wave] + (int32_t)__popcll(ml & ((1ull << lane) - 1))] = i;
+#pragma unroll
+  for (int j = 0; j < kLivePer; ++j)
+    if ((ml[j] >> lane) & 1)
+      live[base + grp_live[j * (kLiveB / 64) + wave] + (int32_t)__popcll(ml[j] & below)] = v[j];
 }
 
 // counters[s][slot][2] -> out[s][2]
@@ -287,8 +301,9 @@ int trn_smooth_run_prepared(trn_ctx* ctx, const trn_mesh* mesh, double* xy_d, do
         const int32_t* count_d = nullptr;
         if (s > 0) {  // nothing is stale before the first sweep
           int32_t* lc = live_count + (size_t)s * (ncolors + 1) + c;
-          launch(ctx, "class_live", 5.0 * count, k_class_live, dim3(nblk(count, kLiveB)),
-                 dim3(kLiveB), 0, count, list, (const uint8_t*)stale, live, lc, cs);
+          launch(ctx, "class_live", 5.0 * count, k_class_live,
+                 dim3(nblk(count, kLiveB * kLivePer)), dim3(kLiveB), 0, count, list,
+                 (const uint8_t*)stale, live, lc, cs);
           list = live;
           count_d = lc;
         }
