@@ -22,6 +22,10 @@ JSON line also carries (under "extras") the same pass with L^1 and L^4, the
 forward/backward on the 1024^2 grid, and gradient-based against Laplacian
 smoothing scored by the same energy.
 
+Inside the timed region only the dominant kernel's launches (smooth_phase)
+carry HIP events, for the roofline figures; the per-kernel table and the
+stage breakdown come from a few further passes with every launch bracketed.
+
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
@@ -112,9 +116,15 @@ def free_gpu():
 
 
 def run_plain(T, mg, ctx, workload, p, sweeps, steps, warmup, seed, color_seed,
-              locality=False, cctx=None, cstream=None):
+              locality=False, cctx=None, cstream=None, stat_steps=3,
+              timed_kernel="smooth_phase"):
     """Times `steps` passes (coordinates restored, metric, colouring, sweeps)
     and returns the wall time per pass and the per-kernel statistics.
+
+    Inside the timed region only the launches of `timed_kernel` are
+    bracketed with HIP events (each event pair costs a few microseconds of
+    the pass); the statistics of every kernel come from `stat_steps` further
+    passes with every launch bracketed, run after the timed region.
 
     With cctx (a second context on the non-blocking stream cstream) the
     colouring, which reads only the topology, runs on cstream while the
@@ -158,31 +168,40 @@ def run_plain(T, mg, ctx, workload, p, sweeps, steps, warmup, seed, color_seed,
     torch.cuda.synchronize()
     ctxs = [ctx] + ([cctx] if cctx is not None else [])
     for c in ctxs:
-        c.set_timing(True)
+        c.set_timing(False)
         c.reset_timing()
+    ctx.set_timing(True, only=timed_kernel)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    timed_stats = ctx.timings()
+    ms = (t1 - t0) * 1e3 / steps
+    for c in ctxs:
+        c.set_timing(True, only=None)
+        c.reset_timing()
+    for _ in range(stat_steps):
+        step()
+    torch.cuda.synchronize()
     stats = {}
     for c in ctxs:
         stats.update(c.timings())
         c.set_timing(False)
-    ms = (t1 - t0) * 1e3 / steps
     nv, nt = m.nv, m.nt
     dm.close()
     del m, xy0, dm, f, xy, tensor, qualit
     free_gpu()
-    return {"nv": nv, "nt": nt, "ms": ms, "stats": stats, "colours": int(np.median(ncolors)),
-            "steps": steps, "colouring_overlapped": cctx is not None}
+    return {"nv": nv, "nt": nt, "ms": ms, "stats": stats, "timed_stats": timed_stats,
+            "colours": int(np.median(ncolors)), "steps": steps, "stat_steps": stat_steps,
+            "colouring_overlapped": cctx is not None}
 
 
 def summarize(r, sweeps, label=None):
     """Per-kernel averages and per-stage rates from the HIP-event timings of
-    run_plain (the same timed region)."""
-    steps, stats, nv = r["steps"], r["stats"], r["nv"]
+    run_plain's instrumented passes (every launch bracketed)."""
+    steps, stats, nv = r["stat_steps"], r["stats"], r["nv"]
     kern = {}
     for name, s in stats.items():
         kern[name] = {"ms_per_step": s["ms"] / steps, "launches": s["launches"],
@@ -379,7 +398,8 @@ def main():
     kern, stages = summarize(r, args.sweeps, f"{args.workload} p={args.p}")
     stats = r["stats"]
     dname = max(kern.items(), key=lambda kv: kv[1]["ms_per_step"])[0]
-    dstat = stats[dname]
+    # the dominant kernel's launches carry HIP events inside the timed region
+    dstat = r["timed_stats"].get(dname, stats[dname])
     bytes_per_launch = dstat["bytes"] / max(dstat["launches"], 1)
     avg_s = dstat["ms"] * 1e-3 / max(dstat["launches"], 1)
     achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
@@ -402,13 +422,13 @@ def main():
             if p == args.p:
                 continue
             rp = run_plain(T, mg, ctx, args.workload, p, args.sweeps, 3, 1, args.seed,
-                           args.color_seed, False, cctx, cstream)
+                           args.color_seed, False, cctx, cstream, stat_steps=2)
             _, st = summarize(rp, args.sweeps, f"{args.workload} p={p}")
             extras[f"p{p}"] = {"ms_per_pass": round(rp["ms"], 4),
                                "Mvertices_per_s": round(rp["nv"] / (rp["ms"] * 1e-3) / 1e6, 2),
                                "stages": st}
         rs = run_plain(T, mg, ctx, "jitter4096:layer", 4, 5, 2, 1, args.seed, args.color_seed,
-                       False, cctx, cstream)
+                       False, cctx, cstream, stat_steps=1)
         _, st = summarize(rs, 5, "jitter4096:layer p=4")
         extras["stress_16M"] = {"workload": "jitter4096:layer", "vertices": rs["nv"],
                                 "triangles": rs["nt"], "lp_norm": 4, "sweeps": 5,
@@ -449,7 +469,8 @@ def main():
         "roofline": {"kernel": dname, "bound": "hbm", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": traffic,
-                     "note": ("achieved = algorithmic bytes per launch / HIP-event launch time; "
+                     "note": ("achieved = algorithmic bytes per launch / HIP-event launch time "
+                              "(events on this kernel's launches inside the timed region); "
                               "traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE and "
                               "WRITE_SIZE (profiles/traffic.txt)")},
         "fp64_valu": fp64,

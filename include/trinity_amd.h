@@ -62,6 +62,12 @@ int trn_ctx_use_own_stream(trn_ctx* ctx);
 int trn_ctx_synchronize(trn_ctx* ctx);
 /* Per-kernel timing with hipEvents around every launch (off by default). */
 int trn_ctx_set_timing(trn_ctx* ctx, int enable);
+/* With timing on, time only the launches of the statistic `name` (as
+ * trn_ctx_timing_get reports it, e.g. "smooth_phase"); NULL or "" times
+ * every launch (the default).  The events cost a few microseconds per timed
+ * launch, so a caller that wants one kernel's durations can leave the others
+ * unbracketed. */
+int trn_ctx_set_timing_only(trn_ctx* ctx, const char* name);
 int trn_ctx_reset_timing(trn_ctx* ctx);
 /* Number of timed kernels; then, per index, name, total ms, launches and
  * algorithmic bytes credited (synchronises the stream). */

@@ -33,6 +33,7 @@ TRN_ERR_STATE = -5
 EXPORTS = [
     "trn_version", "trn_last_error", "trn_ctx_create", "trn_ctx_destroy",
     "trn_ctx_set_stream", "trn_ctx_use_own_stream", "trn_ctx_synchronize", "trn_ctx_set_timing",
+    "trn_ctx_set_timing_only",
     "trn_ctx_reset_timing", "trn_ctx_timing_count", "trn_ctx_timing_get",
     "trn_mesh_create", "trn_mesh_destroy", "trn_mesh_info", "trn_mesh_csr",
     "trn_metric_run", "trn_gradient", "trn_hessian", "trn_normalize_local",
@@ -79,6 +80,7 @@ def _load():
         "trn_ctx_use_own_stream": ([_vp], ctypes.c_int),
         "trn_ctx_synchronize": ([_vp], ctypes.c_int),
         "trn_ctx_set_timing": ([_vp, ctypes.c_int], ctypes.c_int),
+        "trn_ctx_set_timing_only": ([_vp, ctypes.c_char_p], ctypes.c_int),
         "trn_ctx_reset_timing": ([_vp], ctypes.c_int),
         "trn_ctx_timing_count": ([_vp, _P(_i32)], ctypes.c_int),
         "trn_ctx_timing_get": ([_vp, _i32, _P(ctypes.c_char_p), _P(_f64), _P(_i64), _P(_f64)],
@@ -195,7 +197,10 @@ class Context:
     def synchronize(self) -> None:
         _check(lib().trn_ctx_synchronize(self.handle))
 
-    def set_timing(self, on: bool) -> None:
+    def set_timing(self, on: bool, only: str | None = None) -> None:
+        """Per-kernel HIP-event timing; `only` restricts it to the launches
+        of one statistic (e.g. "smooth_phase")."""
+        _check(lib().trn_ctx_set_timing_only(self.handle, only.encode() if only else None))
         _check(lib().trn_ctx_set_timing(self.handle, int(bool(on))))
 
     def reset_timing(self) -> None:

@@ -78,7 +78,7 @@ void timing_end(trn_ctx* ctx, int stat, hipEvent_t start, hipEvent_t stop,
 }
 
 unsigned long long* timing_units(trn_ctx* ctx, const char* name, double bytes_per_unit) {
-  if (!ctx->timing) return nullptr;
+  if (!timed(ctx, name)) return nullptr;
   const int st = stat_index(ctx, name);
   if (st >= kTimingUnitSlots) return nullptr;
   if (!ctx->unit_dev) {
@@ -236,6 +236,13 @@ int trn_ctx_set_timing(trn_ctx* ctx, int enable) {
   return guard_impl("trn_ctx_set_timing", [&] {
     TRN_REQUIRE(ctx != nullptr, TRN_ERR_ARG, "null context");
     ctx->timing = enable != 0;
+  });
+}
+
+int trn_ctx_set_timing_only(trn_ctx* ctx, const char* name) {
+  return guard_impl("trn_ctx_set_timing_only", [&] {
+    TRN_REQUIRE(ctx != nullptr, TRN_ERR_ARG, "null context");
+    ctx->timing_only = name ? name : "";
   });
 }
 

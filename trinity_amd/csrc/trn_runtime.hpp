@@ -65,6 +65,7 @@ struct trn_ctx {
   hipStream_t own_stream = nullptr;  // the context's own (default priority)
   trn::Workspace ws;
   bool timing = false;
+  std::string timing_only;           // non-empty: time only launches of this name
   std::vector<trn::KernelStat> stats;
   std::vector<trn::PendingTiming> pending;
   std::vector<hipEvent_t> event_pool;
@@ -160,28 +161,35 @@ int stat_index(trn_ctx* ctx, const char* name);
 // Work of a launch that only the device knows (e.g. the length of a list it
 // reads): with timing on, returns a device counter the kernel adds its work
 // units to; they are turned into bytes (units x bytes_per_unit) when the
-// timings are collected.  nullptr with timing off.
+// timings are collected.  nullptr when launches of `name` are not timed.
 constexpr int kTimingUnitSlots = 64;
 unsigned long long* timing_units(trn_ctx* ctx, const char* name, double bytes_per_unit);
 void timing_begin(trn_ctx* ctx, int stat, hipEvent_t* start, hipEvent_t* stop);
 void timing_end(trn_ctx* ctx, int stat, hipEvent_t start, hipEvent_t stop,
                 double bytes);
 
-// Launches `kernel` on ctx->stream; when timing is on, brackets it with
-// events and credits `bytes` algorithmic bytes to the named statistic.
+// Whether launches of the statistic `name` are timed: timing on, and no
+// name filter or this name.
+inline bool timed(const trn_ctx* ctx, const char* name) {
+  return ctx->timing && (ctx->timing_only.empty() || ctx->timing_only == name);
+}
+
+// Launches `kernel` on ctx->stream; when its launches are timed, brackets it
+// with events and credits `bytes` algorithmic bytes to the named statistic.
 template <typename K, typename... Args>
 inline void launch(trn_ctx* ctx, const char* name, double bytes, K kernel,
                    dim3 grid, dim3 block, size_t shmem, Args... args) {
   if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int st = -1;
-  if (ctx->timing) {
+  const bool t = timed(ctx, name);
+  if (t) {
     st = stat_index(ctx, name);
     timing_begin(ctx, st, &e0, &e1);
   }
   hipLaunchKernelGGL(kernel, grid, block, shmem, ctx->stream, args...);
   TRN_HIP(hipGetLastError());
-  if (ctx->timing) timing_end(ctx, st, e0, e1, bytes);
+  if (t) timing_end(ctx, st, e0, e1, bytes);
 }
 
 // Scratch slices from the context workspace.  An entry point sizes all of
