@@ -235,6 +235,73 @@ def test_colouring_reports_palette_overflow(ctx):
     assert e.value.code == T().TRN_ERR_PALETTE
 
 
+def test_colouring_overwrites_earlier_colours(ctx):
+    """The first round writes every vertex's colour, so colour and order
+    buffers holding an earlier call's (or any) values give the same result
+    as fresh ones."""
+    import ctypes
+    import torch
+    m = mg.workload("jitter40:fronts")
+    dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags)
+    fresh = T().color(ctx, dm, seed=9)
+    col = torch.full((m.nv,), 5, dtype=torch.int32, device="cuda")
+    order = torch.full((m.nv,), -1, dtype=torch.int32, device="cuda")
+    off = (ctypes.c_int32 * 80)()
+    nc = ctypes.c_int32()
+    rc = T().lib().trn_color(ctx.handle, dm.handle, 9, 64, ctypes.c_void_p(col.data_ptr()),
+                             ctypes.c_void_p(order.data_ptr()), off, ctypes.byref(nc))
+    assert rc == 0
+    assert nc.value == fresh.ncolors
+    assert list(off[:nc.value + 1]) == list(fresh.class_off)
+    assert torch.equal(col, fresh.color) and torch.equal(order, fresh.order)
+
+
+def test_colouring_leaves_isolated_vertices_out(ctx):
+    """Vertices in no triangle keep colour 0 and appear in no class (the
+    reference's preProcess skips them; partition.cpp colours only vertices
+    with neighbours)."""
+    m = mg.workload("uniform12:tanh")
+    extra = 7
+    nv = m.nv + extra
+    tags = np.concatenate([m.tags, np.zeros(extra, dtype=m.tags.dtype)])
+    dm = T().DeviceMesh(ctx, nv, m.tri, tags)
+    col = T().color(ctx, dm, seed=2)
+    cv = col.color.cpu().numpy()
+    assert (cv[m.nv:] == 0).all() and (cv[:m.nv] >= 1).all()
+    assert col.class_off[0] == 0 and col.class_off[-1] == m.nv
+    listed = np.sort(col.order.cpu().numpy()[:m.nv])
+    assert np.array_equal(listed, np.arange(m.nv))
+    assert T().color_check(ctx, dm, col.color, col.ncolors) == (0, 0)
+
+
+def test_timing_filter_brackets_only_the_named_kernel(ctx):
+    import torch
+    m = mg.workload("jitter40:fronts")
+    dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags)
+    xy = dev(m.xy)
+    f = dev(m.solut)
+    ten = torch.empty((m.nv, 3), dtype=torch.float64, device="cuda")
+    q = torch.empty(m.nt, dtype=torch.float64, device="cuda")
+    T().metric_run(ctx, dm, xy, f, 0.8, 2, 1e-9, 0.4, tensor=ten, want_complexity=False)
+    col = T().color(ctx, dm, seed=1)
+    try:
+        ctx.set_timing(True, only="smooth_phase")
+        ctx.reset_timing()
+        T().smooth_run(ctx, dm, xy.clone(), ten.clone(), q, col, 2)
+        only = ctx.timings()
+        ctx.set_timing(True)
+        ctx.reset_timing()
+        T().smooth_run(ctx, dm, xy.clone(), ten.clone(), q, col, 2)
+        every = ctx.timings()
+    finally:
+        ctx.set_timing(False)
+    assert set(only) == {"smooth_phase"}
+    assert only["smooth_phase"]["launches"] == 2 * col.ncolors
+    assert only["smooth_phase"]["bytes"] > 0
+    assert {"smooth_phase", "pack_records", "unpack_records"} <= set(every)
+    assert every["smooth_phase"]["launches"] == only["smooth_phase"]["launches"]
+
+
 def test_colour_checker_flags_conflicts(ctx):
     m = mg.workload("uniform8:tanh")
     dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags)
