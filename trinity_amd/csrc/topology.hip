@@ -314,44 +314,48 @@ __global__ __launch_bounds__(kPatchSlab) void k_patch_fill(
   for (int e = 0; e < w; ++e) col[(int64_t)e * kPatchSlab] = e < m ? s[e * kPatchSlab] : -1;
 }
 
-// Union sizes for the scan: negative (no union) counts as 0.
-__global__ void k_union_sizes(int32_t nslab, const int32_t* __restrict__ n,
-                              int32_t* __restrict__ out) {
+// Start of each slab's union: the slab's own ELL block (patch_row[b] *
+// kPatchSlab), which holds every entry of the slab and so any union of them.
+__global__ void k_union_offsets(int32_t nslab, const int32_t* __restrict__ row,
+                                int32_t* __restrict__ out) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < nslab) out[b] = n[b] > 0 ? n[b] : 0;
+  if (b <= nslab) out[b] = row[b] * kPatchSlab;
 }
 
-// Union of the patches of one slab (one block per slab): the slab's stored
-// entries are sorted in LDS (bitonic), duplicates dropped.  COUNT: writes
-// the union size (or -1 when the slab is larger than kUnionKeys entries or
-// its union larger than kUnionCap).  Otherwise writes the union at
-// union_off[b] and, for every patch entry, its position in the union.
-template <bool COUNT>
-__global__ __launch_bounds__(kPatchSlab) void k_patch_union(
+// Union of the patches of one slab (one block of kUnionB threads per slab):
+// the slab's stored entries are sorted in LDS (bitonic), duplicates dropped.
+// Writes the union size (or -1 when the slab is larger than kUnionKeys
+// entries or its union larger than kUnionCap), the union at union_off[b] =
+// the slab's ELL offset, and, for every patch entry, its position in the
+// union.
+constexpr int kUnionB = 256;
+
+__global__ __launch_bounds__(kUnionB) void k_patch_union(
     int32_t nv, const int32_t* __restrict__ row, const int32_t* __restrict__ ell,
     int32_t* __restrict__ union_n, const int32_t* __restrict__ union_off,
     int32_t* __restrict__ union_ids, uint16_t* __restrict__ local) {
   __shared__ int32_t keys[kUnionKeys];
-  __shared__ int32_t part[kPatchSlab];
+  __shared__ int32_t part[kUnionB];
+  __shared__ int32_t total;
   const int b = blockIdx.x, tid = threadIdx.x;
   const int64_t r0 = row[b];
   const int n = (row[b + 1] - (int)r0) * kPatchSlab;
-  if (n > kUnionKeys || (!COUNT && union_n[b] < 0)) {
-    if (COUNT && tid == 0) union_n[b] = -1;
+  if (n > kUnionKeys) {
+    if (tid == 0) union_n[b] = -1;
     return;
   }
   int P = 1;
   while (P < n) P <<= 1;
   const int32_t* src = ell + r0 * kPatchSlab;
   const int64_t v0 = (int64_t)b * kPatchSlab;  // first vertex of the slab
-  for (int k = tid; k < P; k += kPatchSlab) {
+  for (int k = tid; k < P; k += kUnionB) {
     const int32_t t = (k < n && v0 + (k % kPatchSlab) < nv) ? src[k] : -1;
     keys[k] = t >= 0 ? t : INT32_MAX;
   }
   __syncthreads();
   for (int size = 2; size <= P; size <<= 1)
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int k = tid; k < P; k += kPatchSlab) {
+      for (int k = tid; k < P; k += kUnionB) {
         const int q = k ^ stride;
         if (q > k) {
           const bool up = (k & size) == 0;
@@ -365,39 +369,37 @@ __global__ __launch_bounds__(kPatchSlab) void k_patch_union(
       __syncthreads();
     }
   // distinct entries, counted over contiguous chunks so positions follow
-  const int C = (P + kPatchSlab - 1) / kPatchSlab, c0 = tid * C;
-  int cnt = 0;
+  const int C = (P + kUnionB - 1) / kUnionB, c0 = tid * C;
+  int32_t mine[kUnionKeys / kUnionB];  // a chunk holds at most this many keys
+  int nm = 0;
   for (int k = c0; k < c0 + C && k < P; ++k)
-    cnt += keys[k] != INT32_MAX && (k == 0 || keys[k] != keys[k - 1]);
-  part[tid] = cnt;
+    if (keys[k] != INT32_MAX && (k == 0 || keys[k] != keys[k - 1])) mine[nm++] = keys[k];
+  part[tid] = nm;
   __syncthreads();
   if (tid == 0) {
     int s = 0;
-    for (int w = 0; w < kPatchSlab; ++w) {
+    for (int w = 0; w < kUnionB; ++w) {
       const int c = part[w];
       part[w] = s;
       s += c;
     }
-    if (COUNT) union_n[b] = s <= kUnionCap ? s : -1;
+    total = s;
+    union_n[b] = s <= kUnionCap ? s : -1;
   }
   __syncthreads();
-  if (COUNT) return;
-  // compact the union into the front of keys (positions only move down)
+  const int nu = total;
+  if (nu > kUnionCap) return;
+  // compact the union into the front of keys (positions only move down;
+  // every chunk was read into registers before the barrier above)
   int32_t* U = union_ids + union_off[b];
-  int pos = part[tid];
-  int32_t mine[64];  // a chunk holds at most kUnionKeys / kPatchSlab = 64 keys
-  int nm = 0;
-  for (int k = c0; k < c0 + C && k < P; ++k)
-    if (keys[k] != INT32_MAX && (k == 0 || keys[k] != keys[k - 1])) mine[nm++] = keys[k];
-  __syncthreads();
+  const int pos = part[tid];
   for (int j = 0; j < nm; ++j) {
     keys[pos + j] = mine[j];
     U[pos + j] = mine[j];
   }
   __syncthreads();
-  const int nu = union_n[b];
   uint16_t* dst = local + r0 * kPatchSlab;
-  for (int k = tid; k < n; k += kPatchSlab) {
+  for (int k = tid; k < n; k += kUnionB) {
     const int32_t t = v0 + (k % kPatchSlab) < nv ? src[k] : -1;
     uint16_t v = 0xFFFF;
     if (t >= 0) {
@@ -600,25 +602,21 @@ int trn_mesh_create(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
              (const int32_t*)m->v2t, (const int32_t*)m->v2v_off, (const int32_t*)m->v2v,
              m->patch_cap, (const int32_t*)m->patch_row, m->patch_ell);
 
-      // 4c. per-slab unions of the patches (LDS staging of the gathers)
+      // 4c. per-slab unions of the patches (LDS staging of the gathers),
+      //     each stored in its slab's ELL block: one sort per slab
       TRN_HIP(hipMalloc(&m->union_n, sizeof(int32_t) * (size_t)nslab));
       TRN_HIP(hipMalloc(&m->union_off, sizeof(int32_t) * (size_t)(nslab + 1)));
+      TRN_HIP(hipMalloc(&m->union_ids, sizeof(int32_t) * (size_t)(ell_n > 0 ? ell_n : 1)));
       TRN_HIP(hipMalloc(&m->patch_local, sizeof(uint16_t) * (size_t)(ell_n > 0 ? ell_n : 1)));
-      launch(ctx, "topo_patch_union_count", 0.0, k_patch_union<true>, dim3((unsigned)nslab),
-             dim3(kPatchSlab), 0, nv, (const int32_t*)m->patch_row, (const int32_t*)m->patch_ell,
-             m->union_n, (const int32_t*)nullptr, (int32_t*)nullptr, (uint16_t*)nullptr);
-      launch(ctx, "topo_union_clamp", 0.0, k_union_sizes, dim3(blocks_for(nslab, 256)),
-             dim3(256), 0, (int32_t)nslab, (const int32_t*)m->union_n, m->union_off);
+      launch(ctx, "topo_union_offsets", 0.0, k_union_offsets, dim3(blocks_for(nslab + 1, 256)),
+             dim3(256), 0, (int32_t)nslab, (const int32_t*)m->patch_row, m->union_off);
+      launch(ctx, "topo_patch_union", 0.0, k_patch_union, dim3((unsigned)nslab), dim3(kUnionB),
+             0, nv, (const int32_t*)m->patch_row, (const int32_t*)m->patch_ell, m->union_n,
+             (const int32_t*)m->union_off, m->union_ids, m->patch_local);
       reduce_max_i32(ctx, m->union_n, nslab, flags + 7);
-      exclusive_scan_i32(ctx, m->union_off, m->union_off, nslab, true, scan_tmp);
-      copy_async(ctx, h, m->union_off + nslab, sizeof(int32_t));
       copy_async(ctx, h + 1, flags + 7, sizeof(int32_t));
       sync(ctx);
       m->max_union = h[1] > 0 ? h[1] : 0;
-      TRN_HIP(hipMalloc(&m->union_ids, sizeof(int32_t) * (size_t)(h[0] > 0 ? h[0] : 1)));
-      launch(ctx, "topo_patch_union", 0.0, k_patch_union<false>, dim3((unsigned)nslab),
-             dim3(kPatchSlab), 0, nv, (const int32_t*)m->patch_row, (const int32_t*)m->patch_ell,
-             m->union_n, (const int32_t*)m->union_off, m->union_ids, m->patch_local);
 
       // 5. star codes of the incident triangles (smoothing reads corners
       //    through the 1-ring instead of the triangle array)

@@ -144,12 +144,14 @@ struct trn_mesh {
   int32_t* patch_row = nullptr;  // [nslab + 1]
   int32_t* patch_ell = nullptr;  // [patch_row[nslab] * kPatchSlab]
   // Per slab, the ascending union of its vertices' patches (union_n[b]
-  // triangles at union_ids + union_off[b]; union_n[b] < 0: no union, the
-  // slab gathers through patch_ell), and patch_local, shaped like
-  // patch_ell, the position of each patch entry in its slab's union.
+  // triangles at union_ids + union_off[b], union_off[b] = patch_row[b] *
+  // kPatchSlab: the slab's ELL block, which can hold any union of its
+  // entries; union_n[b] < 0: no union, the slab gathers through
+  // patch_ell), and patch_local, shaped like patch_ell, the position of
+  // each patch entry in its slab's union.
   int32_t* union_n = nullptr;        // [nslab]
   int32_t* union_off = nullptr;      // [nslab + 1]
-  int32_t* union_ids = nullptr;      // [union_off[nslab]]
+  int32_t* union_ids = nullptr;      // [patch_row[nslab] * kPatchSlab]
   uint16_t* patch_local = nullptr;   // [patch_row[nslab] * kPatchSlab]
   int32_t max_union = 0;             // largest union_n
 };
