@@ -18,7 +18,8 @@ Default workload: 2048^2 jittered grid (4,198,401 vertices, 8,388,608
 triangles), sum of 3 seeded tanh fronts, L^2 metric with target factor 0.8,
 h_min=1e-9, h_max=0.4, GPU colouring, 3 sweeps.  Unless --no-extras, the
 JSON line also carries (under "extras") the same pass with L^1 and L^4, the
-16.8M-vertex boundary-layer pass (4096^2, L^4, 5 sweeps), the adjoint
+16.8M-vertex boundary-layer pass (4096^2, L^4, 5 sweeps), the drop-in's
+per-call sequence from host arrays (copies and mesh creation included), the adjoint
 forward/backward on the 1024^2 grid, and gradient-based against Laplacian
 smoothing scored by the same energy.
 
@@ -268,6 +269,68 @@ def read_valu(kernel, workload):
     return None
 
 
+def run_host_boundary(T, mg, ctx, workload, p, sweeps, reps=3, seed=1234, color_seed=1):
+    """The drop-in shims' per-call sequence from host arrays: triangles,
+    tags, coordinates and field copied to the device, the device mesh built
+    (CSR, stored patches, unions, records), the pass run on one stream, and
+    coordinates and metric copied back.  Median wall time of `reps` calls
+    after one untimed call."""
+    import numpy as np
+    import torch
+    m = mg.workload(workload, seed=seed)
+    tri = np.ascontiguousarray(m.tri.reshape(-1), dtype=np.int32)
+    tags = np.ascontiguousarray(m.tags, dtype=np.uint8)
+    xy_h = np.ascontiguousarray(m.xy, dtype=np.float64)
+    f_h = np.ascontiguousarray(m.solut, dtype=np.float64)
+    # results go back into existing host arrays, as the shims write the
+    # reference's own (already touched) coordinate and metric arrays
+    xy_back = torch.from_numpy(np.zeros_like(xy_h))
+    t_back = torch.from_numpy(np.zeros((m.nv, 3), dtype=np.float64))
+    times, parts = [], []
+    for r in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xy = torch.from_numpy(xy_h).cuda()
+        f = torch.from_numpy(f_h).cuda()
+        tri_d, tags_d = torch.from_numpy(tri).cuda(), torch.from_numpy(tags).cuda()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        dm = T.DeviceMesh(ctx, m.nv, tri_d, tags_d)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        tensor = torch.empty((m.nv, 3), dtype=torch.float64, device="cuda")
+        qualit = torch.empty(m.nt, dtype=torch.float64, device="cuda")
+        T.metric_run(ctx, dm, xy, f, 0.8, p, 1e-9, 0.4, tensor=tensor, want_complexity=False)
+        col = T.color(ctx, dm, seed=color_seed, max_colors=64)
+        T.smooth_run(ctx, dm, xy, tensor, qualit, col, sweeps, recompute_qualit=True,
+                     counters=False)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        xy_back.copy_(xy)
+        t_back.copy_(tensor)
+        t4 = time.perf_counter()
+        dm.close()
+        del xy, f, tri_d, tags_d, tensor, qualit, col
+        if r:
+            times.append((t4 - t0) * 1e3)
+            parts.append([(t1 - t0) * 1e3, (t2 - t1) * 1e3, (t3 - t2) * 1e3, (t4 - t3) * 1e3])
+    ms = float(np.median(times))
+    pm = np.median(np.array(parts), axis=0)
+    nv = m.nv
+    del m
+    free_gpu()
+    log(f"host boundary {workload}: {ms:.2f} ms per call, {nv / (ms * 1e-3) / 1e6:.1f} "
+        f"Mvertices/s (copy in {pm[0]:.2f}, mesh {pm[1]:.2f}, pass {pm[2]:.2f}, "
+        f"copy out {pm[3]:.2f} ms)")
+    return {"workload": workload, "vertices": nv, "ms_per_call": round(ms, 3),
+            "Mvertices_per_s": round(nv / (ms * 1e-3) / 1e6, 2),
+            "ms_copy_in": round(float(pm[0]), 3), "ms_mesh_create": round(float(pm[1]), 3),
+            "ms_pass": round(float(pm[2]), 3), "ms_copy_out": round(float(pm[3]), 3),
+            "includes": "host-to-device copies of triangles, tags, xy and f (pageable), device "
+                        "mesh creation, the pass on one stream, device-to-host copies of xy and "
+                        "the metric into existing host arrays"}
+
+
 def run_adjoint(T, mg, ctx, workload, steps):
     """Forward (metric + qualities + energy) and backward (dE/dxy, dE/df)
     of the energy sum_t (1 - q_M(t))^2."""
@@ -435,6 +498,9 @@ def main():
                                 "ms_per_pass": round(rs["ms"], 3),
                                 "Mvertices_per_s": round(rs["nv"] / (rs["ms"] * 1e-3) / 1e6, 2),
                                 "stages": st}
+        extras["host_boundary"] = run_host_boundary(T, mg, ctx, args.workload, args.p,
+                                                    args.sweeps, seed=args.seed,
+                                                    color_seed=args.color_seed)
         extras["adjoint"] = run_adjoint(T, mg, ctx, "jitter1024:fronts+quad", 5)
         extras["smoothing_modes"] = compare_smoothing(T, mg, ctx, "jitter256:fronts+quad", 5)
     extras["batched"] = {"status": "switched off",
