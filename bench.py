@@ -493,6 +493,13 @@ def main():
     ms = r["ms"]
     value = r["nv"] / (ms * 1e-3) / 1e6
     kern, stages = summarize(r, args.sweeps, f"{args.workload} p={args.p}")
+    # whole pass against its compulsory bytes: recovery + metric 368, colouring
+    # 32 and 196 per sweep, per vertex
+    pass_bpv = BYTES_METRIC_PER_V + 32 + BYTES_SWEEP_PER_V * args.sweeps
+    pass_gbps = pass_bpv * r["nv"] / (ms * 1e-3) / 1e9
+    stages["pass"] = {"ms": round(ms, 4), "bytes_per_vertex": pass_bpv,
+                      "alg_GBps": round(pass_gbps, 1),
+                      "hbm_frac": round(pass_gbps / HBM_PEAK_GBS, 4)}
     stats = r["stats"]
     dname = max(kern.items(), key=lambda kv: kv[1]["ms_per_step"])[0]
     # the dominant kernel's launches carry HIP events inside the timed region
