@@ -162,6 +162,18 @@ int trn_color_check(trn_ctx* ctx, const trn_mesh* mesh, const int32_t* color_d,
  * Mesh::computeQuality(t) (src/mesh.cpp:626-644, numeric.cpp:251-306). */
 int trn_quality_cache(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d,
                       const double* tensor_d, double* qualit_d);
+/* Replaces Mesh::computeQuality(double quality[3]) (src/mesh.cpp:647-681):
+ * rebuilds qualit_d as trn_quality_cache does, then folds the active
+ * triangles' qualities into the host array as the reference does:
+ * quality[0] = min(quality[0], min q), quality[1] = max(quality[1], max q),
+ * quality[2] = (quality[2] + sum q) / count (the reference starts its
+ * partial extremes at 3 and 0).  *active (may be NULL) receives the count
+ * (Mesh::nb.activ_elem).  The sum is a fixed-order tree, so the mean may
+ * differ from a given OpenMP run of the reference in the last bits, as
+ * runs of the reference with different thread counts do.  Synchronises. */
+int trn_quality_stats(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d,
+                      const double* tensor_d, double* qualit_d, double quality[3],
+                      int64_t* active);
 /* Replaces Smooth::run (src/smoothing.cpp:274-302) after its colouring:
  * `sweeps` rounds (Smooth's `level`) of Smooth::movePoints (:158-180), each
  * visiting colours 1..ncolors in ascending order and applying

@@ -302,6 +302,32 @@ def test_timing_filter_brackets_only_the_named_kernel(ctx):
     assert every["smooth_phase"]["launches"] == only["smooth_phase"]["launches"]
 
 
+def test_quality_stats_match_mesh_compute_quality(ctx):
+    """Mesh::computeQuality(double[3]) (mesh.cpp:647-681): qualities of the
+    active triangles (a deleted one is skipped), folded into the caller's
+    array as min, max and (initial + sum) / count."""
+    import torch
+    m = mg.workload("jitter64:fronts")
+    tri = m.tri.copy()
+    tri[17] = -1  # deleted triangle (Mesh::isActiveElem)
+    active = tri[:, 0] >= 0
+    dm = T().DeviceMesh(ctx, m.nv, tri, m.tags)
+    metric = oracle.metric_stages(m.xy, m.tri, m.solut)["metric"]
+    xy, ten = dev(m.xy), dev(metric)
+    (qmin, qmax, qmean), n, qc = T().quality_stats(ctx, dm, xy, ten)
+    cache = T().quality_cache(ctx, dm, xy, ten)
+    assert n == int(active.sum())
+    assert bits_equal(qc.cpu().numpy()[active], cache.cpu().numpy()[active])
+    ref = oracle.quality_cache(m.xy, m.tri, metric)[active]
+    assert abs(qmin - ref.min()) <= 1e-10 * ref.min()
+    assert abs(qmax - ref.max()) <= 1e-10 * ref.max()
+    assert abs(qmean - ref.mean()) <= 1e-12 * ref.mean()
+    # folding into existing values
+    (a, b, c), n2, _ = T().quality_stats(ctx, dm, xy, ten, quality=(0.25, 2.0, 10.0))
+    assert n2 == n and a == min(0.25, qmin) and b == 2.0
+    assert abs(c - (10.0 + qmean * n) / n) <= 1e-12 * c
+
+
 def test_colour_checker_flags_conflicts(ctx):
     m = mg.workload("uniform8:tanh")
     dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags)

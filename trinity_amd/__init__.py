@@ -38,7 +38,8 @@ EXPORTS = [
     "trn_mesh_create", "trn_mesh_destroy", "trn_mesh_info", "trn_mesh_csr",
     "trn_metric_run", "trn_gradient", "trn_hessian", "trn_normalize_local",
     "trn_complexity", "trn_normalize_global", "trn_color", "trn_color_classes",
-    "trn_color_check", "trn_quality_cache", "trn_smooth_run", "trn_smooth_prepare",
+    "trn_color_check", "trn_quality_cache", "trn_quality_stats", "trn_smooth_run",
+    "trn_smooth_prepare",
     "trn_smooth_run_prepared",
     "trn_batch_create", "trn_batch_destroy", "trn_batch_info", "trn_batch_color",
     "trn_batch_set_config", "trn_batch_get_config", "trn_batch_pass", "trn_batch_pass_graph",
@@ -101,6 +102,7 @@ def _load():
         "trn_color_classes": ([_vp, _vp, _i32, _vp, _i32, _vp, _P(_i32)], ctypes.c_int),
         "trn_color_check": ([_vp, _vp, _vp, _i32, _P(_i64), _P(_i64)], ctypes.c_int),
         "trn_quality_cache": ([_vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+        "trn_quality_stats": ([_vp, _vp, _vp, _vp, _vp, _P(_f64), _P(_i64)], ctypes.c_int),
         "trn_smooth_run": ([_vp, _vp, _vp, _vp, _vp, _vp, _P(_i32), _i32, _i32, _i32, _P(_i32),
                             _P(_i32)], ctypes.c_int),
         "trn_smooth_prepare": ([_vp, _vp, _vp, _vp, _vp, _i32], ctypes.c_int),
@@ -400,6 +402,20 @@ def quality_cache(ctx, mesh, xy, tensor, out=None):
     _check(lib().trn_quality_cache(ctx.handle, mesh.handle, _vp(xy.data_ptr()),
                                    _vp(tensor.data_ptr()), _vp(out.data_ptr())))
     return out
+
+
+def quality_stats(ctx, mesh, xy, tensor, out=None, quality=(3.0, 0.0, 0.0)):
+    """Mesh::computeQuality(double[3]): rebuilds the quality cache and
+    returns ((min, max, mean) folded into `quality` as the reference folds
+    into its array, active triangle count, cache)."""
+    torch = _torch()
+    out = out if out is not None else torch.empty(mesh.nt, dtype=torch.float64, device=xy.device)
+    q = (_f64 * 3)(*[float(v) for v in quality])
+    n = _i64()
+    _check(lib().trn_quality_stats(ctx.handle, mesh.handle, _vp(xy.data_ptr()),
+                                   _vp(tensor.data_ptr()), _vp(out.data_ptr()), q,
+                                   ctypes.byref(n)))
+    return (q[0], q[1], q[2]), n.value, out
 
 
 def smooth_run(ctx, mesh, xy, tensor, qualit, coloring: Coloring, sweeps: int,

@@ -27,6 +27,7 @@
 //                    triangle array.
 #include <algorithm>
 
+#include "trn_recovery.hpp"
 #include "trn_smooth.hpp"
 
 namespace trn {
@@ -226,6 +227,86 @@ static void quality_cache_logs(trn_ctx* ctx, const trn_mesh* m, const double* xy
          (const int32_t*)m->tri, (const double2*)xy, logm, qualit);
 }
 
+// Mesh::computeQuality(double[3]) (src/mesh.cpp:647-681) reduces the
+// qualities of the active triangles to their minimum, maximum, sum and
+// count.  Block b takes a fixed contiguous range of triangles; sums go
+// through fixed-order trees, so the result does not depend on timing.  The
+// comparisons are the reference's (q_min > q, q_max < q), so a NaN quality
+// changes neither extreme but makes the sum NaN, as there.
+constexpr int kStatB = 256;
+constexpr int kStatBlocks = 1024;
+
+__device__ __forceinline__ void block_min_max(double& lo, double& hi, double* lds) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double l2 = __shfl_xor(lo, o, 64), h2 = __shfl_xor(hi, o, 64);
+    lo = dmin(lo, l2);
+    hi = dmax(hi, h2);
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    lds[2 * wid] = lo;
+    lds[2 * wid + 1] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+      lo = dmin(lo, lds[2 * w]);
+      hi = dmax(hi, lds[2 * w + 1]);
+    }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kStatB) void k_quality_partials(int32_t nt,
+                                                             const int32_t* __restrict__ tri,
+                                                             const double* __restrict__ qualit,
+                                                             double* __restrict__ part) {
+  __shared__ double lds[2 * (kStatB / 64)];
+  const int64_t chunk = ((int64_t)nt + gridDim.x - 1) / gridDim.x;
+  const int64_t t0 = (int64_t)blockIdx.x * chunk;
+  const int64_t t1 = t0 + chunk < nt ? t0 + chunk : nt;
+  double lo = 3., hi = 0., s = 0., c = 0.;  // mesh.cpp:649-652
+  for (int64_t t = t0 + threadIdx.x; t < t1; t += kStatB) {
+    if (tri[3 * t] < 0) continue;  // mesh.cpp:660-661
+    const double q = qualit[t];
+    if (lo > q) lo = q;
+    if (hi < q) hi = q;
+    s += q;
+    c += 1.;
+  }
+  s = block_sum(s, lds);
+  c = block_sum(c, lds);
+  block_min_max(lo, hi, lds);
+  if (threadIdx.x == 0) {
+    part[4 * blockIdx.x] = lo;
+    part[4 * blockIdx.x + 1] = hi;
+    part[4 * blockIdx.x + 2] = s;
+    part[4 * blockIdx.x + 3] = c;
+  }
+}
+
+__global__ __launch_bounds__(kStatB) void k_quality_final(int32_t nparts,
+                                                          const double* __restrict__ part,
+                                                          double* __restrict__ out) {
+  __shared__ double lds[2 * (kStatB / 64)];
+  double lo = 3., hi = 0., s = 0., c = 0.;
+  for (int b = threadIdx.x; b < nparts; b += kStatB) {
+    lo = dmin(lo, part[4 * b]);
+    hi = dmax(hi, part[4 * b + 1]);
+    s += part[4 * b + 2];
+    c += part[4 * b + 3];
+  }
+  s = block_sum(s, lds);
+  c = block_sum(c, lds);
+  block_min_max(lo, hi, lds);
+  if (threadIdx.x == 0) {
+    out[0] = lo;
+    out[1] = hi;
+    out[2] = s;
+    out[3] = c;
+  }
+}
+
 }  // namespace trn
 
 using namespace trn;
@@ -241,6 +322,36 @@ int trn_quality_cache(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d,
     double4* logm = sc.carve<double4>(mesh->nv);
     tensor_logs(ctx, mesh, tensor_d, logm);
     quality_cache_logs(ctx, mesh, xy_d, logm, qualit_d);
+  });
+}
+
+int trn_quality_stats(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d,
+                      const double* tensor_d, double* qualit_d, double quality[3],
+                      int64_t* active) {
+  return guard_impl("trn_quality_stats", [&] {
+    DeviceScope ds(ctx);
+    TRN_REQUIRE(mesh && xy_d && tensor_d && qualit_d && quality, TRN_ERR_ARG, "null argument");
+    Scratch sc(ctx, scratch_bytes(sizeof(double4) * (size_t)mesh->nv) +
+                        scratch_bytes(sizeof(double) * 4 * (size_t)kStatBlocks) +
+                        scratch_bytes(sizeof(double) * 4));
+    double4* logm = sc.carve<double4>(mesh->nv);
+    double* part = sc.carve<double>(4 * (size_t)kStatBlocks);
+    double* res = sc.carve<double>(4);
+    tensor_logs(ctx, mesh, tensor_d, logm);
+    quality_cache_logs(ctx, mesh, xy_d, logm, qualit_d);
+    launch(ctx, "quality_partials", 8.0 * mesh->nt, k_quality_partials, dim3(kStatBlocks),
+           dim3(kStatB), 0, mesh->nt, (const int32_t*)mesh->tri, (const double*)qualit_d, part);
+    launch(ctx, "quality_final", 0.0, k_quality_final, dim3(1), dim3(kStatB), 0,
+           (int32_t)kStatBlocks, (const double*)part, res);
+    copy_async(ctx, ctx->pinned_f64, res, 4 * sizeof(double));
+    sync(ctx);
+    const double* h = ctx->pinned_f64;
+    // mesh.cpp:671-679: fold into the caller's values, then the mean
+    quality[0] = (h[0] < quality[0]) ? h[0] : quality[0];  // std::min(quality[0], q_min)
+    quality[1] = (quality[1] < h[1]) ? h[1] : quality[1];  // std::max(quality[1], q_max)
+    quality[2] += h[2];
+    quality[2] /= h[3];
+    if (active) *active = (int64_t)h[3];
   });
 }
 
