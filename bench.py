@@ -365,6 +365,37 @@ def run_host_boundary(T, mg, ctx, workload, p, sweeps, reps=3, seed=1234, color_
                         "the metric into existing host arrays"}
 
 
+def run_refine(T, mg, ctx, workload, p, reps=5, seed=1234):
+    """Refine::filterElems + Steiner points (trn_refine_filter) on the metric
+    of `workload`: median wall time of `reps` calls (each synchronises once,
+    for the list sizes)."""
+    import numpy as np
+    import torch
+    m = mg.workload(workload, seed=seed)
+    xy = torch.from_numpy(m.xy).cuda()
+    dm = T.DeviceMesh(ctx, m.nv, m.tri, m.tags)
+    tensor, _ = T.metric_run(ctx, dm, xy, torch.from_numpy(m.solut).cuda(), 0.8, p, 1e-9, 0.4,
+                             want_complexity=False)
+    r = T.refine_filter(ctx, dm, xy, tensor)
+    times = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = T.refine_filter(ctx, dm, xy, tensor)
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+    ms = float(np.median(times))
+    nt = m.nt
+    dm.close()
+    del m, xy, tensor
+    free_gpu()
+    log(f"refine filter {workload}: {ms:.3f} ms, {r['split']:,} edges to split, "
+        f"{r['tasks']:,} triangles to cut")
+    return {"workload": workload, "triangles": nt, "ms": round(ms, 4),
+            "Mtriangles_per_s": round(nt / (ms * 1e-3) / 1e6, 1), "edges_to_split": r["split"],
+            "triangles_to_cut": r["tasks"]}
+
+
 def run_adjoint(T, mg, ctx, workload, steps):
     """Forward (metric + qualities + energy) and backward (dE/dxy, dE/df)
     of the energy sum_t (1 - q_M(t))^2."""
@@ -542,6 +573,7 @@ def main():
         extras["host_boundary"] = run_host_boundary(T, mg, ctx, args.workload, args.p,
                                                     args.sweeps, seed=args.seed,
                                                     color_seed=args.color_seed)
+        extras["refine_filter"] = run_refine(T, mg, ctx, args.workload, args.p, seed=args.seed)
         extras["adjoint"] = run_adjoint(T, mg, ctx, "jitter1024:fronts+quad", 5)
         extras["smoothing_modes"] = compare_smoothing(T, mg, ctx, "jitter256:fronts+quad", 5)
     extras["batched"] = {"status": "switched off",

@@ -157,6 +157,28 @@ int trn_color_classes(trn_ctx* ctx, const trn_mesh* mesh, int32_t nv,
 int trn_color_check(trn_ctx* ctx, const trn_mesh* mesh, const int32_t* color_d,
                     int32_t ncolors, int64_t* bad_edges, int64_t* bad_vertices);
 
+/* ---- refinement filter: Refine::filterElems + Steiner points -----------
+ * Replaces Refine::filterElems (src/refinement.cpp:216-265) and the point
+ * computation of Refine::computeSteinerPoints (:267-296;
+ * Mesh::computeSteinerPoint src/mesh.cpp:684-697, numeric::computeSteinerPoint
+ * src/numeric.cpp:387-445).  For every active triangle, pattern_d[t]
+ * (device, [nt]) = number of its edges longer than l_max (the reference's
+ * trinity::l_max = sqrt 2, header.h:67) under Mesh::computeLength; an edge
+ * (v1, v2) = (n[j], n[j+1]) that is long is listed once, from the triangle
+ * where v1 < v2 or from its only triangle, as (v1, v2, opposite triangle or
+ * -1) in edges_d (device, [3 * edge_cap]); elems_d (device, [nt]) lists the
+ * triangles with pattern > 0.  Lists are in (triangle, edge) order, the
+ * order of a one-thread run of the reference.  steiner_xy_d ([2 * edge_cap])
+ * and steiner_tensor_d ([3 * edge_cap]), both or neither, receive each
+ * listed edge's Steiner point and tensor.  counts (host) = {evaluated
+ * triangles, sum of patterns, listed edges, listed triangles} (nb.eval,
+ * nb.adds, nb.split, nb.tasks); they are written before TRN_ERR_CAPACITY is
+ * returned for more than edge_cap edges.  Synchronises. */
+int trn_refine_filter(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d,
+                      const double* tensor_d, double l_max, int32_t* pattern_d,
+                      int32_t* edges_d, int32_t* elems_d, double* steiner_xy_d,
+                      double* steiner_tensor_d, int32_t edge_cap, int32_t counts[4]);
+
 /* ---- smoothing: Smooth::run ---------------------------------------------
  * Smooth::cacheQuality (src/smoothing.cpp:145-155): qualit_d[t] =
  * Mesh::computeQuality(t) (src/mesh.cpp:626-644, numeric.cpp:251-306). */

@@ -75,6 +75,9 @@ def lib():
                                       ctypes.c_double, _d, _d]
         L.orc_energy_grad.restype = ctypes.c_double
         L.orc_libm.argtypes = [ctypes.c_int, ctypes.c_int64, _d, _d, _d]
+        L.orc_steiner_point.argtypes = [_d, _d, _d, _d, _d, _d]
+        L.orc_refine_filter.argtypes = [ctypes.c_int, _d, _i, _d, _i, _i, ctypes.c_double, _i, _i,
+                                        _i, ctypes.c_int, _i]
         _lib = L
     return _lib
 
@@ -231,6 +234,41 @@ def quality_cache(xy, tri, tensor) -> np.ndarray:
     q = np.zeros(tri.reshape(-1, 3).shape[0])
     lib().orc_quality_cache(q.size, _p(xy, _d), _p(tri, _i), _p(tensor, _d), _p(q, _d))
     return q
+
+
+def steiner_point(pa, pb, ta, tb) -> tuple[np.ndarray, np.ndarray]:
+    """numeric::computeSteinerPoint (src/numeric.cpp:387-445)."""
+    pa, pb, ta, tb = f64(pa), f64(pb), f64(ta), f64(tb)
+    pt, tn = np.zeros(2), np.zeros(3)
+    lib().orc_steiner_point(_p(pa, _d), _p(pb, _d), _p(ta, _d), _p(tb, _d), _p(pt, _d),
+                            _p(tn, _d))
+    return pt, tn
+
+
+def refine_filter(xy, tri, tensor, topo=None, l_max=1.414213562373095) -> dict:
+    """Refine::filterElems (src/refinement.cpp:216-265) run by one thread:
+    per-triangle long-edge counts, the listed edges (v1, v2, opposite) in
+    (triangle, edge) order, the triangles to cut and the four counters."""
+    xy = f64(xy)
+    tri = np.ascontiguousarray(tri, dtype=np.int32).reshape(-1, 3)
+    tensor = f64(tensor)
+    nt = tri.shape[0]
+    topo = topo or Topology(xy.reshape(-1, 2).shape[0], tri)
+    pattern = np.zeros(nt, dtype=np.int32)
+    cap = 3 * nt
+    edges = np.zeros(3 * max(cap, 1), dtype=np.int32)
+    elems = np.zeros(max(nt, 1), dtype=np.int32)
+    counts = np.zeros(4, dtype=np.int32)
+    rc = lib().orc_refine_filter(nt, _p(xy, _d), _p(tri, _i), _p(tensor, _d),
+                                 _p(topo.v2t_off, _i), _p(topo.v2t, _i), float(l_max),
+                                 _p(pattern, _i), _p(edges, _i), _p(elems, _i), cap,
+                                 _p(counts, _i))
+    if rc < 0:
+        raise RuntimeError("edge capacity exceeded")
+    ev, adds, split, tasks = (int(c) for c in counts)
+    return {"pattern": pattern, "edges": edges[:3 * split].reshape(-1, 3).copy(),
+            "elems": elems[:tasks].copy(), "eval": ev, "adds": adds, "split": split,
+            "tasks": tasks}
 
 
 def smooth_run(xy, tri, tags, tensor, color, ncolors, sweeps, topo=None, qualit=None):

@@ -627,6 +627,86 @@ void orc_smooth_run(int nv, int nt, double* xy, double* tensor,
 }
 
 /* ------------------------------------------------------------------------ */
+/* numeric::approxRiemannDist, 3-argument form (src/numeric.cpp:449-459). */
+static double riemann_dist1(const double* pa, const double* pb, const double* t) {
+  const double u0 = pb[0] - pa[0], u1 = pb[1] - pa[1];
+  const double v0 = t[0] * u0 + t[1] * u1;
+  const double v1 = t[1] * u0 + t[2] * u1;
+  return sqrt(u0 * v0 + u1 * v1);
+}
+
+/* numeric::computeSteinerPoint, src/numeric.cpp:387-445 (coef = 1). */
+void orc_steiner_point(const double* pa, const double* pb, const double* ta,
+                       const double* tb, double* point, double* tensor) {
+  const double l0 = riemann_dist1(pa, pb, ta);         /* :403-404 */
+  const double l1 = riemann_dist1(pa, pb, tb);
+  const double coef = 1;
+  const double w = fabs(l0 - l1) < ORC_EPSILON            /* :408-412 */
+                       ? 0.5
+                       : (sqrt(coef) * sqrt(l0 * l1) - l1) / (coef * l0 - l1);
+  point[0] = (1 - w) * pa[0] + (w * pb[0]);            /* :415-416 */
+  point[1] = (1 - w) * pa[1] + (w * pb[1]);
+  double m[6];
+  memcpy(m, ta, sizeof(double) * 3);                    /* :428-430 */
+  memcpy(m + 3, tb, sizeof(double) * 3);
+  orc_interpolate_tensor(m, tensor, 2);
+}
+
+/* Mesh::getElemNeigh, src/mesh.cpp:437-453: the triangle other than id that
+ * holds the edge j -> i, searched among i's incident triangles in order. */
+static int elem_neigh(int id, int i, int j, const int32_t* tri,
+                      const int32_t* v2t_off, const int32_t* v2t) {
+  for (int k = v2t_off[i]; k < v2t_off[i + 1]; ++k) {
+    const int t = v2t[k];
+    if (t == id) continue;
+    const int32_t* n = tri + 3 * (size_t)t;
+    if ((n[0] == j && n[1] == i) || (n[1] == j && n[2] == i) || (n[2] == j && n[0] == i))
+      return t;
+  }
+  return -1;
+}
+
+int orc_refine_filter(int nt, const double* xy, const int32_t* tri,
+                      const double* tensor, const int32_t* v2t_off,
+                      const int32_t* v2t, double l_max, int32_t* pattern,
+                      int32_t* edges, int32_t* elems, int cap,
+                      int32_t counts[4]) {
+  int eval = 0, adds = 0, split = 0, tasks = 0;
+  for (int i = 0; i < nt; ++i) {                        /* refinement.cpp:228-255 */
+    pattern[i] = 0;
+    const int32_t* n = tri + 3 * (size_t)i;
+    if (n[0] < 0) continue;
+    double len[3];
+    for (int j = 0; j < 3; ++j) {
+      const int k = (j + 1) % 3;
+      len[j] = orc_riemann_dist(xy + 2 * (size_t)n[j], xy + 2 * (size_t)n[k],
+                                tensor + 3 * (size_t)n[j], tensor + 3 * (size_t)n[k]);
+    }
+    for (int j = 0; j < 3; ++j) {
+      const int k = (j + 1) % 3;
+      if (len[j] > l_max) {
+        ++pattern[i];
+        const int opp = elem_neigh(i, n[j], n[k], tri, v2t_off, v2t);
+        if (n[j] < n[k] || opp < 0) {
+          if (split >= cap) return -1;
+          edges[3 * split] = n[j];
+          edges[3 * split + 1] = n[k];
+          edges[3 * split + 2] = opp;
+          ++split;
+        }
+      }
+    }
+    if (pattern[i]) {
+      adds += pattern[i];
+      elems[tasks++] = i;
+    }
+    ++eval;
+  }
+  counts[0] = eval; counts[1] = adds; counts[2] = split; counts[3] = tasks;
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Host libm values (glibc) of the transcendentals on the path, for
  * comparing the device math library against what the reference links:
  * op 0 log, 1 exp, 2 pow(x, y), 3 pow(x, 3), 4 sqrt. */

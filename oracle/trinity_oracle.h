@@ -120,6 +120,24 @@ void   orc_smooth_run(int nv, int nt, double* xy, double* tensor,
                       const int32_t* color, int ncolors, int sweeps,
                       int recompute_qualit, int32_t* commits, int32_t* tasks);
 
+/* ---- refinement filter (src/refinement.cpp:216-296) --------------------- */
+/* numeric::computeSteinerPoint (src/numeric.cpp:387-445): point of edge
+ * (pa, pb) splitting it evenly in the metric, and its log-Euclidean tensor. */
+void   orc_steiner_point(const double* pa, const double* pb, const double* ta,
+                         const double* tb, double* point, double* tensor);
+/* Refine::filterElems (src/refinement.cpp:216-265) run by one thread:
+ * pattern[t] = number of edges of active triangle t longer than l_max
+ * (Mesh::computeLength, 4-argument Riemann distance); edges (3 ints each:
+ * v1, v2, opposite triangle or -1) lists an edge from triangle t when it is
+ * long and v1 < v2 or it has no opposite triangle, in (t, edge) order;
+ * elems lists the triangles with pattern > 0.  counts = {evaluated, sum of
+ * patterns, edges, elems}.  Returns -1 if more than cap edges. */
+int    orc_refine_filter(int nt, const double* xy, const int32_t* tri,
+                         const double* tensor, const int32_t* v2t_off,
+                         const int32_t* v2t, double l_max, int32_t* pattern,
+                         int32_t* edges, int32_t* elems, int cap,
+                         int32_t counts[4]);
+
 /* ---- host libm probe ---------------------------------------------------- */
 /* op 0 log, 1 exp, 2 pow(x, y), 3 pow(x, 3), 4 sqrt, as linked by the
  * reference (glibc). */

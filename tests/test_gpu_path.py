@@ -328,6 +328,47 @@ def test_quality_stats_match_mesh_compute_quality(ctx):
     assert abs(c - (10.0 + qmean * n) / n) <= 1e-12 * c
 
 
+@pytest.mark.parametrize("wl,lmax", [("jitter64:fronts", 1.414213562373095),
+                                     ("jitter40:layer", 1.0)])
+def test_refine_filter_matches_oracle(ctx, wl, lmax):
+    """Refine::filterElems + Steiner points against the oracle (pinned by
+    closed forms in test_oracle_kat.py), one triangle deleted.  Lengths,
+    lists and Steiner coordinates use only IEEE arithmetic and the Riemann
+    distance (exact here); the Steiner tensor goes through device log/exp
+    (1 ulp from glibc): 1e-13 relative."""
+    m = mg.workload(wl)
+    tri = m.tri.copy()
+    tri[11] = -1
+    dm = T().DeviceMesh(ctx, m.nv, tri, m.tags)
+    metric = oracle.metric_stages(m.xy, m.tri, m.solut)["metric"]
+    r = T().refine_filter(ctx, dm, dev(m.xy), dev(metric), l_max=lmax)
+    ref = oracle.refine_filter(m.xy, tri, metric, l_max=lmax)
+    for k in ("eval", "adds", "split", "tasks"):
+        assert r[k] == ref[k], k
+    assert ref["split"] > 0 and ref["eval"] == len(tri) - 1
+    assert np.array_equal(r["pattern"].cpu().numpy(), ref["pattern"])
+    assert np.array_equal(r["edges"].cpu().numpy(), ref["edges"])
+    assert np.array_equal(r["elems"].cpu().numpy(), ref["elems"])
+    sxy, sten = r["steiner_xy"].cpu().numpy(), r["steiner_tensor"].cpu().numpy()
+    for e, (a, b, _) in enumerate(ref["edges"]):
+        pt, tn = oracle.steiner_point(m.xy[a], m.xy[b], metric[a], metric[b])
+        assert np.array_equal(sxy[e], pt)
+        assert np.abs(sten[e] - tn).max() <= 1e-13 * (abs(tn[0]) + abs(tn[2]))
+
+
+def test_refine_filter_reports_capacity(ctx):
+    import torch
+    m = mg.grid_mesh(10)
+    c = (1.5 * 10) ** 2  # every edge long: 3 n^2 + 2 n = 320 edges
+    ten = torch.tensor(np.tile([c, 0.0, c], (m.nv, 1)), device="cuda")
+    dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags)
+    with pytest.raises(T().TrinityError) as e:
+        T().refine_filter(ctx, dm, dev(m.xy), ten, edge_cap=100)
+    assert e.value.code == T().TRN_ERR_CAPACITY
+    r = T().refine_filter(ctx, dm, dev(m.xy), ten)
+    assert r["split"] == 320 and r["tasks"] == 200 and r["adds"] == 600
+
+
 def test_colour_checker_flags_conflicts(ctx):
     m = mg.workload("uniform8:tanh")
     dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags)

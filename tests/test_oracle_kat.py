@@ -195,3 +195,59 @@ def test_smoothing_keeps_boundary_and_orientation():
     assert com.min() > 0 and (com <= tas).all()
     # accepted moves only raise the worst quality around each moved vertex
     assert q.min() >= q0.min()
+
+
+# ---- refinement filter and Steiner points (src/refinement.cpp:216-296,
+# numeric.cpp:387-445), pinned by closed forms -------------------------------
+
+def test_steiner_point_in_one_isotropic_metric_is_the_midpoint():
+    pa, pb = np.array([0.1, 0.2]), np.array([0.4, 0.65])
+    t = np.array([5.0, 0.0, 5.0])
+    pt, tn = oracle.steiner_point(pa, pb, t, t)
+    assert np.array_equal(pt, 0.5 * pa + 0.5 * pb)
+    assert np.allclose(tn, t, rtol=1e-14, atol=0)
+
+
+def test_steiner_point_between_two_isotropic_metrics():
+    # lengths 2L and 3L: weight (sqrt(6) L - 3L) / (2L - 3L) = 3 - sqrt(6);
+    # log-Euclidean mean of 4 I and 9 I is 6 I
+    pa, pb = np.array([0.0, 0.0]), np.array([0.3, 0.4])
+    pt, tn = oracle.steiner_point(pa, pb, [4.0, 0.0, 4.0], [9.0, 0.0, 9.0])
+    w = 3.0 - np.sqrt(6.0)
+    assert np.allclose(pt, pa + w * (pb - pa), rtol=1e-14, atol=1e-16)
+    assert np.allclose(tn, [6.0, 0.0, 6.0], rtol=1e-14, atol=1e-15)
+
+
+def _grid_with_metric(n, edge_len):
+    m = mg.grid_mesh(n)
+    c = (edge_len * n) ** 2  # axis edges of length 1/n measure edge_len in c I
+    ten = np.tile([c, 0.0, c], (m.nv, 1))
+    return m, ten
+
+
+def test_refine_filter_lists_each_long_diagonal_once():
+    n = 12
+    m, ten = _grid_with_metric(n, 1.2)  # axis 1.2 < sqrt 2 < diagonal 1.2 sqrt 2
+    r = oracle.refine_filter(m.xy, m.tri, ten)
+    nt = 2 * n * n
+    assert r["eval"] == nt and r["tasks"] == nt and r["adds"] == nt
+    assert (r["pattern"] == 1).all()
+    assert r["split"] == n * n
+    e = r["edges"]
+    assert (e[:, 0] < e[:, 1]).all() and (e[:, 2] >= 0).all()
+    assert len({(a, b) for a, b, _ in e}) == n * n
+    # each listed edge is a diagonal of its cell, held by the opposite triangle
+    for a, b, opp in e:
+        assert b - a == n + 2
+        assert {int(a), int(b)} <= set(m.tri[opp].tolist())
+
+
+def test_refine_filter_lists_boundary_edges_from_their_only_triangle():
+    n = 9
+    m, ten = _grid_with_metric(n, 1.5)  # every edge is long
+    r = oracle.refine_filter(m.xy, m.tri, ten)
+    assert (r["pattern"] == 3).all()
+    assert r["split"] == 3 * n * n + 2 * n  # every edge of the grid once
+    e = r["edges"]
+    assert len({(min(a, b), max(a, b)) for a, b, _ in e}) == r["split"]
+    assert int((e[:, 2] < 0).sum()) == 4 * n

@@ -38,7 +38,8 @@ EXPORTS = [
     "trn_mesh_create", "trn_mesh_destroy", "trn_mesh_info", "trn_mesh_csr",
     "trn_metric_run", "trn_gradient", "trn_hessian", "trn_normalize_local",
     "trn_complexity", "trn_normalize_global", "trn_color", "trn_color_classes",
-    "trn_color_check", "trn_quality_cache", "trn_quality_stats", "trn_smooth_run",
+    "trn_color_check", "trn_quality_cache", "trn_quality_stats", "trn_refine_filter",
+    "trn_smooth_run",
     "trn_smooth_prepare",
     "trn_smooth_run_prepared",
     "trn_batch_create", "trn_batch_destroy", "trn_batch_info", "trn_batch_color",
@@ -103,6 +104,8 @@ def _load():
         "trn_color_check": ([_vp, _vp, _vp, _i32, _P(_i64), _P(_i64)], ctypes.c_int),
         "trn_quality_cache": ([_vp, _vp, _vp, _vp, _vp], ctypes.c_int),
         "trn_quality_stats": ([_vp, _vp, _vp, _vp, _vp, _P(_f64), _P(_i64)], ctypes.c_int),
+        "trn_refine_filter": ([_vp, _vp, _vp, _vp, _f64, _vp, _vp, _vp, _vp, _vp, _i32,
+                               _P(_i32)], ctypes.c_int),
         "trn_smooth_run": ([_vp, _vp, _vp, _vp, _vp, _vp, _P(_i32), _i32, _i32, _i32, _P(_i32),
                             _P(_i32)], ctypes.c_int),
         "trn_smooth_prepare": ([_vp, _vp, _vp, _vp, _vp, _i32], ctypes.c_int),
@@ -416,6 +419,35 @@ def quality_stats(ctx, mesh, xy, tensor, out=None, quality=(3.0, 0.0, 0.0)):
                                    _vp(tensor.data_ptr()), _vp(out.data_ptr()), q,
                                    ctypes.byref(n)))
     return (q[0], q[1], q[2]), n.value, out
+
+
+def refine_filter(ctx, mesh, xy, tensor, l_max=1.414213562373095, steiner=True,
+                  edge_cap=None) -> dict:
+    """Refine::filterElems + the Steiner points of computeSteinerPoints on the
+    device.  Returns pattern, edges (split x 3: v1, v2, opposite), elems,
+    Steiner points and tensors (when steiner) and the four counters."""
+    torch = _torch()
+    dev = xy.device
+    cap = int(edge_cap) if edge_cap is not None else 3 * mesh.nt
+    pattern = torch.empty(max(mesh.nt, 1), dtype=torch.int32, device=dev)
+    edges = torch.empty(3 * max(cap, 1), dtype=torch.int32, device=dev)
+    elems = torch.empty(max(mesh.nt, 1), dtype=torch.int32, device=dev)
+    sxy = torch.empty(2 * max(cap, 1), dtype=torch.float64, device=dev) if steiner else None
+    sten = torch.empty(3 * max(cap, 1), dtype=torch.float64, device=dev) if steiner else None
+    counts = (_i32 * 4)()
+    _check(lib().trn_refine_filter(ctx.handle, mesh.handle, _vp(xy.data_ptr()),
+                                   _vp(tensor.data_ptr()), float(l_max),
+                                   _vp(pattern.data_ptr()), _vp(edges.data_ptr()),
+                                   _vp(elems.data_ptr()),
+                                   _vp(sxy.data_ptr()) if steiner else None,
+                                   _vp(sten.data_ptr()) if steiner else None, cap, counts))
+    ev, adds, split, tasks = (int(c) for c in counts)
+    out = {"pattern": pattern[:mesh.nt], "edges": edges[:3 * split].view(-1, 3),
+           "elems": elems[:tasks], "eval": ev, "adds": adds, "split": split, "tasks": tasks}
+    if steiner:
+        out["steiner_xy"] = sxy[:2 * split].view(-1, 2)
+        out["steiner_tensor"] = sten[:3 * split].view(-1, 3)
+    return out
 
 
 def smooth_run(ctx, mesh, xy, tensor, qualit, coloring: Coloring, sweeps: int,
