@@ -396,6 +396,35 @@ def run_refine(T, mg, ctx, workload, p, reps=5, seed=1234):
             "triangles_to_cut": r["tasks"]}
 
 
+def run_indep_set(T, mg, ctx, workload, frac=0.5, reps=5, seed=1234):
+    """Partit::extractIndepSet (trn_indep_set) on the subgraph of a seeded
+    `frac` of the vertices of `workload` (ascending ids, as the coarsening's
+    filter lists them): median wall time of `reps` calls."""
+    import numpy as np
+    import torch
+    m = mg.workload(workload, seed=seed)
+    dm = T.DeviceMesh(ctx, m.nv, m.tri, m.tags)
+    rng = np.random.default_rng(seed)
+    nodes = np.flatnonzero(rng.random(m.nv) < frac).astype(np.int32)
+    nodes_t = torch.from_numpy(nodes).cuda()
+    s = T.indep_set(ctx, dm, nodes_t, seed=1)
+    times = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s = T.indep_set(ctx, dm, nodes_t, seed=1)
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+    ms = float(np.median(times))
+    n, k = int(nodes.size), int(s.numel())
+    dm.close()
+    del m, nodes_t, s
+    free_gpu()
+    log(f"independent set {workload}: {n:,} nodes -> {k:,} in {ms:.3f} ms")
+    return {"workload": workload, "nodes": n, "set_size": k, "ms": round(ms, 4),
+            "Mnodes_per_s": round(n / (ms * 1e-3) / 1e6, 1)}
+
+
 def run_adjoint(T, mg, ctx, workload, steps):
     """Forward (metric + qualities + energy) and backward (dE/dxy, dE/df)
     of the energy sum_t (1 - q_M(t))^2."""
@@ -574,6 +603,7 @@ def main():
                                                     args.sweeps, seed=args.seed,
                                                     color_seed=args.color_seed)
         extras["refine_filter"] = run_refine(T, mg, ctx, args.workload, args.p, seed=args.seed)
+        extras["indep_set"] = run_indep_set(T, mg, ctx, args.workload, seed=args.seed)
         extras["adjoint"] = run_adjoint(T, mg, ctx, "jitter1024:fronts+quad", 5)
         extras["smoothing_modes"] = compare_smoothing(T, mg, ctx, "jitter256:fronts+quad", 5)
     extras["batched"] = {"status": "switched off",

@@ -154,6 +154,16 @@ int trn_color_classes(trn_ctx* ctx, const trn_mesh* mesh, int32_t nv,
                       int32_t* class_off);
 /* Counts edges whose end points share a colour and non-isolated vertices
  * left uncoloured or outside 1..ncolors (0 and 0 = valid).  Synchronises. */
+/* Replaces Partit::extractIndepSet (src/partition.cpp:200-306) on the
+ * subgraph Coarse::extractSubGraph builds (src/coarsening.cpp:307-319): the
+ * n_nodes distinct vertices nodes_d (device) with their 1-rings, neighbours
+ * outside the list never conflicting.  set_d (device, [n_nodes]) receives a
+ * maximal independent set of that subgraph, in input order, *set_size its
+ * size.  The set is the greedy one in decreasing (seeded priority, id)
+ * order (the priority of trn_color), a pure function of (mesh, nodes,
+ * seed); the reference's depends on its thread interleaving.  Synchronises. */
+int trn_indep_set(trn_ctx* ctx, const trn_mesh* mesh, const int32_t* nodes_d, int32_t n_nodes,
+                  uint64_t seed, int32_t* set_d, int32_t* set_size);
 int trn_color_check(trn_ctx* ctx, const trn_mesh* mesh, const int32_t* color_d,
                     int32_t ncolors, int64_t* bad_edges, int64_t* bad_vertices);
 

@@ -76,6 +76,9 @@ def lib():
         L.orc_energy_grad.restype = ctypes.c_double
         L.orc_libm.argtypes = [ctypes.c_int, ctypes.c_int64, _d, _d, _d]
         L.orc_steiner_point.argtypes = [_d, _d, _d, _d, _d, _d]
+        L.orc_color_priority.argtypes = [ctypes.c_uint64, ctypes.c_uint32]
+        L.orc_color_priority.restype = ctypes.c_uint32
+        L.orc_indep_set.argtypes = [ctypes.c_int, _i, _i, _i, ctypes.c_int, ctypes.c_uint64, _i]
         L.orc_refine_filter.argtypes = [ctypes.c_int, _d, _i, _d, _i, _i, ctypes.c_double, _i, _i,
                                         _i, ctypes.c_int, _i]
         _lib = L
@@ -234,6 +237,20 @@ def quality_cache(xy, tri, tensor) -> np.ndarray:
     q = np.zeros(tri.reshape(-1, 3).shape[0])
     lib().orc_quality_cache(q.size, _p(xy, _d), _p(tri, _i), _p(tensor, _d), _p(q, _d))
     return q
+
+
+def indep_set(topo, nodes, seed=1) -> np.ndarray:
+    """Partit::extractIndepSet on the subgraph of `nodes` (greedy in
+    decreasing seeded-priority order); returns in_set per node (0/1)."""
+    nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+    out = np.zeros(max(nodes.size, 1), dtype=np.int32)
+    lib().orc_indep_set(topo.nv, _p(topo.v2v_off, _i), _p(topo.v2v, _i), _p(nodes, _i),
+                        nodes.size, int(seed) & ((1 << 64) - 1), _p(out, _i))
+    return out[:nodes.size].copy()
+
+
+def color_priority(seed, v) -> int:
+    return int(lib().orc_color_priority(int(seed) & ((1 << 64) - 1), int(v)))
 
 
 def steiner_point(pa, pb, ta, tb) -> tuple[np.ndarray, np.ndarray]:

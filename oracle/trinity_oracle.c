@@ -627,6 +627,55 @@ void orc_smooth_run(int nv, int nt, double* xy, double* tensor,
 }
 
 /* ------------------------------------------------------------------------ */
+/* splitmix64 finaliser (Steele, Lea, Flood 2014), the colouring's seeded
+ * vertex priority. */
+uint32_t orc_color_priority(uint64_t seed, uint32_t v) {
+  uint64_t z = seed + 0x9E3779B97F4A7C15ull * ((uint64_t)v + 1);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (uint32_t)(z >> 32);
+}
+
+static uint64_t mis_key(uint64_t seed, int32_t v) {
+  return ((uint64_t)orc_color_priority(seed, (uint32_t)v) << 32) | (uint32_t)v;
+}
+
+static int cmp_key_desc(const void* a, const void* b) {
+  const uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+  return x < y ? 1 : (x > y ? -1 : 0);
+}
+
+/* Partit::extractIndepSet (src/partition.cpp:200-306) semantics on the
+ * subgraph Coarse::extractSubGraph builds (src/coarsening.cpp:307-319),
+ * visited sequentially in decreasing (priority, id) order. */
+int orc_indep_set(int nv, const int32_t* v2v_off, const int32_t* v2v,
+                  const int32_t* nodes, int n, uint64_t seed, int32_t* in_set) {
+  int8_t* state = (int8_t*)calloc((size_t)(nv > 0 ? nv : 1), 1); /* 1 sub, 2 in */
+  uint64_t* keys = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)(n > 0 ? n : 1));
+  for (int k = 0; k < n; ++k) {
+    state[nodes[k]] = 1;
+    keys[k] = mis_key(seed, nodes[k]);
+  }
+  qsort(keys, (size_t)n, sizeof(uint64_t), cmp_key_desc);
+  for (int k = 0; k < n; ++k) {
+    const int32_t v = (int32_t)(keys[k] & 0xFFFFFFFFu);
+    int free_ = 1;
+    for (int a = v2v_off[v]; a < v2v_off[v + 1]; ++a)
+      if (state[v2v[a]] == 2) { free_ = 0; break; }
+    if (free_) state[v] = 2;
+  }
+  int size = 0;
+  for (int k = 0; k < n; ++k) {
+    in_set[k] = state[nodes[k]] == 2;
+    size += in_set[k];
+  }
+  free(state);
+  free(keys);
+  return size;
+}
+
+/* ------------------------------------------------------------------------ */
 /* numeric::approxRiemannDist, 3-argument form (src/numeric.cpp:449-459). */
 static double riemann_dist1(const double* pa, const double* pb, const double* t) {
   const double u0 = pb[0] - pa[0], u1 = pb[1] - pa[1];

@@ -120,6 +120,17 @@ void   orc_smooth_run(int nv, int nt, double* xy, double* tensor,
                       const int32_t* color, int ncolors, int sweeps,
                       int recompute_qualit, int32_t* commits, int32_t* tasks);
 
+/* ---- independent set for coarsening (src/partition.cpp:200-306) ------- */
+/* The colouring's seeded priority (splitmix64 finaliser, high word). */
+uint32_t orc_color_priority(uint64_t seed, uint32_t v);
+/* Maximal independent set of the subgraph induced by the n distinct
+ * vertices `nodes` in the 1-ring graph: greedy in decreasing (priority, id)
+ * order, i.e. a vertex is in the set iff no neighbour in the subgraph that
+ * precedes it in that order is.  in_set[k] = 1 if nodes[k] is in the set.
+ * Returns the set size. */
+int    orc_indep_set(int nv, const int32_t* v2v_off, const int32_t* v2v,
+                     const int32_t* nodes, int n, uint64_t seed, int32_t* in_set);
+
 /* ---- refinement filter (src/refinement.cpp:216-296) --------------------- */
 /* numeric::computeSteinerPoint (src/numeric.cpp:387-445): point of edge
  * (pa, pb) splitting it evenly in the metric, and its log-Euclidean tensor. */

@@ -369,6 +369,31 @@ def test_refine_filter_reports_capacity(ctx):
     assert r["split"] == 320 and r["tasks"] == 200 and r["adds"] == 600
 
 
+@pytest.mark.parametrize("wl,frac,seed", [("jitter64:fronts", 0.5, 3), ("uniform40:tanh", 1.0, 8)])
+def test_indep_set_matches_oracle(ctx, wl, frac, seed):
+    """Partit::extractIndepSet on the subgraph of a random vertex subset
+    (shuffled): the device set equals the oracle's greedy set (pinned by
+    its characterisation in test_oracle_kat.py), in input order."""
+    import torch
+    m = mg.workload(wl)
+    dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags)
+    rng = np.random.default_rng(seed)
+    nodes = rng.permutation(m.nv)[:int(frac * m.nv)].astype(np.int32)
+    got = T().indep_set(ctx, dm, dev(nodes, torch.int32), seed=seed).cpu().numpy()
+    ins = oracle.indep_set(oracle.Topology(m.nv, m.tri), nodes, seed=seed)
+    assert np.array_equal(got, nodes[ins == 1])
+
+
+def test_indep_set_rejects_bad_ids_and_takes_empty_input(ctx):
+    import torch
+    m = mg.workload("uniform8:tanh")
+    dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags)
+    with pytest.raises(T().TrinityError) as e:
+        T().indep_set(ctx, dm, dev(np.array([0, 3, m.nv + 2], np.int32), torch.int32))
+    assert e.value.code == T().TRN_ERR_ARG
+    assert T().indep_set(ctx, dm, torch.empty(0, dtype=torch.int32, device="cuda")).numel() == 0
+
+
 def test_colour_checker_flags_conflicts(ctx):
     m = mg.workload("uniform8:tanh")
     dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags)

@@ -251,3 +251,32 @@ def test_refine_filter_lists_boundary_edges_from_their_only_triangle():
     e = r["edges"]
     assert len({(min(a, b), max(a, b)) for a, b, _ in e}) == r["split"]
     assert int((e[:, 2] < 0).sum()) == 4 * n
+
+
+# ---- independent set for coarsening (src/partition.cpp:200-306) ----------
+
+@pytest.mark.parametrize("wl,frac,seed", [("jitter40:fronts", 0.6, 1), ("uniform24:tanh", 1.0, 7),
+                                          ("jitter30:layer", 0.2, 3)])
+def test_indep_set_is_the_greedy_maximal_independent_set(wl, frac, seed):
+    """A vertex is in the set iff no neighbour in the subgraph with a larger
+    (priority, id) key is: this characterises the greedy set uniquely, and
+    makes it independent and maximal."""
+    m = mg.workload(wl)
+    topo = oracle.Topology(m.nv, m.tri)
+    rng = np.random.default_rng(seed)
+    nodes = np.sort(rng.choice(m.nv, size=int(frac * m.nv), replace=False)).astype(np.int32)
+    ins = oracle.indep_set(topo, nodes, seed=seed)
+    sub = np.zeros(m.nv, bool)
+    sub[nodes] = True
+    inset = np.zeros(m.nv, bool)
+    inset[nodes[ins == 1]] = True
+    key = {int(v): (oracle.color_priority(seed, int(v)), int(v)) for v in nodes}
+    for v in nodes:
+        ring = topo.v2v[topo.v2v_off[v]:topo.v2v_off[v + 1]]
+        ring = [int(u) for u in ring if sub[u]]
+        blocked = any(inset[u] and key[u] > key[int(v)] for u in ring)
+        assert inset[v] == (not blocked)
+        if inset[v]:
+            assert not any(inset[u] for u in ring)        # independent
+        else:
+            assert any(inset[u] for u in ring)            # maximal

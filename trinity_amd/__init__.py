@@ -38,7 +38,7 @@ EXPORTS = [
     "trn_mesh_create", "trn_mesh_destroy", "trn_mesh_info", "trn_mesh_csr",
     "trn_metric_run", "trn_gradient", "trn_hessian", "trn_normalize_local",
     "trn_complexity", "trn_normalize_global", "trn_color", "trn_color_classes",
-    "trn_color_check", "trn_quality_cache", "trn_quality_stats", "trn_refine_filter",
+    "trn_color_check", "trn_indep_set", "trn_quality_cache", "trn_quality_stats", "trn_refine_filter",
     "trn_smooth_run",
     "trn_smooth_prepare",
     "trn_smooth_run_prepared",
@@ -103,6 +103,7 @@ def _load():
         "trn_color_classes": ([_vp, _vp, _i32, _vp, _i32, _vp, _P(_i32)], ctypes.c_int),
         "trn_color_check": ([_vp, _vp, _vp, _i32, _P(_i64), _P(_i64)], ctypes.c_int),
         "trn_quality_cache": ([_vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+        "trn_indep_set": ([_vp, _vp, _vp, _i32, ctypes.c_uint64, _vp, _P(_i32)], ctypes.c_int),
         "trn_quality_stats": ([_vp, _vp, _vp, _vp, _vp, _P(_f64), _P(_i64)], ctypes.c_int),
         "trn_refine_filter": ([_vp, _vp, _vp, _vp, _f64, _vp, _vp, _vp, _vp, _vp, _i32,
                                _P(_i32)], ctypes.c_int),
@@ -390,6 +391,18 @@ def color_classes(ctx, color_t, ncolors, mesh=None) -> Coloring:
                                    color_t.numel(), _vp(color_t.data_ptr()), int(ncolors),
                                    _vp(order.data_ptr()), off))
     return Coloring(color_t, order, np.array(off[:ncolors + 1], dtype=np.int32), int(ncolors))
+
+
+def indep_set(ctx, mesh, nodes_t, seed=1):
+    """Partit::extractIndepSet on the subgraph of the vertices nodes_t
+    (device int32): returns the independent vertices (device, input order)."""
+    torch = _torch()
+    out = torch.empty(max(nodes_t.numel(), 1), dtype=torch.int32, device=nodes_t.device)
+    n = _i32()
+    _check(lib().trn_indep_set(ctx.handle, mesh.handle, _vp(nodes_t.data_ptr()),
+                               int(nodes_t.numel()), int(seed) & ((1 << 64) - 1),
+                               _vp(out.data_ptr()), ctypes.byref(n)))
+    return out[:n.value]
 
 
 def color_check(ctx, mesh, color_t, ncolors) -> tuple[int, int]:

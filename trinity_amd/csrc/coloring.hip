@@ -280,6 +280,87 @@ __global__ __launch_bounds__(kScanB) void k_class_scan(const int32_t* __restrict
 
 static unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// ---- independent set for coarsening ---------------------------------------
+// Partit::extractIndepSet (src/partition.cpp:200-306) on the subgraph that
+// Coarse::extractSubGraph builds (src/coarsening.cpp:307-319): the filtered
+// vertices with their 1-rings, unfiltered neighbours never conflicting.  The
+// reference keeps the colour-1 class of a speculative first fit, a maximal
+// independent set whose members depend on the thread interleaving.  Here
+// the set is the greedy one in decreasing (seeded priority, id) order,
+// reached in rounds: an undecided vertex leaves when a neighbour has joined
+// and joins when no undecided neighbour outranks it.  A vertex that reads a
+// neighbour's earlier state in a round only waits a round, so the result
+// does not depend on timing; the top undecided vertex always decides, so
+// every round makes progress.  state: 0 outside the subgraph, 1 undecided,
+// 2 in the set, 3 out.
+constexpr int kMisB = 256;
+
+__global__ __launch_bounds__(kMisB) void k_mis_mark(int32_t n, const int32_t* __restrict__ nodes,
+                                                    int32_t nv, int32_t* __restrict__ state,
+                                                    int32_t* bad) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t v = nodes[i];
+  if (v < 0 || v >= nv) {
+    atomicOr(bad, 1);
+    return;
+  }
+  state[v] = 1;
+}
+
+__global__ __launch_bounds__(kMisB) void k_mis_round(int32_t n, const int32_t* __restrict__ nodes,
+                                                     int32_t nv,
+                                                     const int32_t* __restrict__ v2v_off,
+                                                     const int32_t* __restrict__ v2v,
+                                                     int32_t* state, uint64_t seed,
+                                                     int32_t* remaining,
+                                                     const int32_t* __restrict__ prev_slots) {
+  if (prev_losers(prev_slots) == 0) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int undecided = 0;
+  if (i < n) {
+    const int32_t v = nodes[i];
+    if (v >= 0 && v < nv && state[v] == 1) {  // ids outside [0, nv) are reported by the host
+      const uint32_t pv = color_priority(seed, (uint32_t)v);
+      int join = 1, out = 0;
+      for (int32_t a = v2v_off[v]; a < v2v_off[v + 1]; ++a) {
+        const int32_t u = v2v[a];
+        const int32_t su = state[u];
+        if (su == 2) {
+          out = 1;
+          break;
+        }
+        if (su == 1) {
+          const uint32_t pu = color_priority(seed, (uint32_t)u);
+          if (pu > pv || (pu == pv && u > v)) join = 0;
+        }
+      }
+      if (out) state[v] = 3;
+      else if (join) state[v] = 2;
+      else undecided = 1;
+    }
+  }
+  const int left = __syncthreads_count(undecided);
+  if (threadIdx.x == 0 && left) atomicAdd(&remaining[blockIdx.x & (kRoundSlots - 1)], left);
+}
+
+__global__ __launch_bounds__(kMisB) void k_mis_flags(int32_t n, const int32_t* __restrict__ nodes,
+                                                     int32_t nv, const int32_t* __restrict__ state,
+                                                     int32_t* __restrict__ flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t v = nodes[i];
+  flag[i] = v >= 0 && v < nv && state[v] == 2;
+}
+
+__global__ __launch_bounds__(kMisB) void k_mis_scatter(int32_t n, const int32_t* __restrict__ nodes,
+                                                       const int32_t* __restrict__ flag,
+                                                       const int32_t* __restrict__ pos,
+                                                       int32_t* __restrict__ set) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && flag[i]) set[pos[i]] = nodes[i];
+}
+
 constexpr int kStatusWords = 3 + kMaxRows + 1;
 
 struct ClassBufs {
@@ -395,6 +476,61 @@ int trn_color_classes(trn_ctx* ctx, const trn_mesh* mesh, int32_t nv, const int3
                   order_d, cb);
     sync(ctx);
     for (int k = 0; k <= ncolors; ++k) class_off[k] = ctx->pinned_i32[3 + k];
+  });
+}
+
+int trn_indep_set(trn_ctx* ctx, const trn_mesh* mesh, const int32_t* nodes_d, int32_t n_nodes,
+                  uint64_t seed, int32_t* set_d, int32_t* set_size) {
+  return guard_impl("trn_indep_set", [&] {
+    DeviceScope ds(ctx);
+    TRN_REQUIRE(mesh && set_size && n_nodes >= 0 && (n_nodes == 0 || (nodes_d && set_d)),
+                TRN_ERR_ARG, "null argument");
+    *set_size = 0;
+    if (n_nodes == 0) return;
+    const int32_t nv = mesh->nv;
+    const size_t nslots = (size_t)kRoundsPerSync * kRoundSlots;
+    Scratch sc(ctx, scratch_bytes(sizeof(int32_t) * (size_t)nv) +
+                        scratch_bytes(sizeof(int32_t) * (16 + nslots)) +
+                        2 * scratch_bytes(sizeof(int32_t) * ((size_t)n_nodes + 1)) +
+                        exclusive_scan_scratch(n_nodes));
+    int32_t* state = sc.carve<int32_t>(nv);
+    int32_t* flags = sc.carve<int32_t>(16 + nslots);  // [0] bad vertex id, then the slots
+    int32_t* slots = flags + 16;
+    int32_t* flag = sc.carve<int32_t>((size_t)n_nodes + 1);
+    int32_t* pos = sc.carve<int32_t>((size_t)n_nodes + 1);
+    int32_t* tmp = sc.carve<int32_t>(exclusive_scan_scratch(n_nodes) / 4);
+    memset_async(ctx, state, 0, sizeof(int32_t) * (size_t)nv);
+    memset_async(ctx, flags, 0, sizeof(int32_t) * (16 + nslots));
+    launch(ctx, "mis_mark", 4.0 * n_nodes, k_mis_mark, dim3(nblk(n_nodes, kMisB)), dim3(kMisB),
+           0, n_nodes, nodes_d, nv, state, flags);
+    int32_t* h = ctx->pinned_i32;
+    int rounds = 0;
+    for (;;) {
+      if (rounds) memset_async(ctx, slots, 0, sizeof(int32_t) * nslots);
+      for (int r = 0; r < kRoundsPerSync; ++r)
+        launch(ctx, "mis_round", 0.0, k_mis_round, dim3(nblk(n_nodes, kMisB)), dim3(kMisB), 0,
+               n_nodes, nodes_d, nv, (const int32_t*)mesh->v2v_off, (const int32_t*)mesh->v2v,
+               state, seed, slots + (size_t)r * kRoundSlots,
+               r ? (const int32_t*)(slots + (size_t)(r - 1) * kRoundSlots) : nullptr);
+      // the set is compacted with the batch; one synchronisation returns it
+      launch(ctx, "mis_flags", 0.0, k_mis_flags, dim3(nblk(n_nodes, kMisB)), dim3(kMisB), 0,
+             n_nodes, nodes_d, nv, (const int32_t*)state, flag);
+      exclusive_scan_i32(ctx, flag, pos, n_nodes, true, tmp);
+      launch(ctx, "mis_scatter", 0.0, k_mis_scatter, dim3(nblk(n_nodes, kMisB)), dim3(kMisB), 0,
+             n_nodes, nodes_d, (const int32_t*)flag, (const int32_t*)pos, set_d);
+      copy_async(ctx, h, flags, sizeof(int32_t));
+      copy_async(ctx, h + 1, pos + n_nodes, sizeof(int32_t));
+      copy_async(ctx, h + 2, slots + (size_t)(kRoundsPerSync - 1) * kRoundSlots,
+                 sizeof(int32_t) * kRoundSlots);
+      sync(ctx);
+      TRN_REQUIRE(h[0] == 0, TRN_ERR_ARG, "a node id is outside [0, nv)");
+      rounds += kRoundsPerSync;
+      int left = 0;
+      for (int k = 0; k < kRoundSlots; ++k) left += h[2 + k];
+      if (left == 0) break;
+      TRN_REQUIRE(rounds < 100000, TRN_ERR_STATE, "independent set did not converge");
+    }
+    *set_size = h[1];
   });
 }
 
