@@ -17,10 +17,12 @@ them.
 Default workload: 2048^2 jittered grid (4,198,401 vertices, 8,388,608
 triangles), sum of 3 seeded tanh fronts, L^2 metric with target factor 0.8,
 h_min=1e-9, h_max=0.4, GPU colouring, 3 sweeps.  Unless --no-extras, the
-JSON line also carries (under "extras") the same pass with L^1 and L^4, the
-16.8M-vertex boundary-layer pass (4096^2, L^4, 5 sweeps), the drop-in's
-per-call sequence from host arrays (copies and mesh creation included), the adjoint
-forward/backward on the 1024^2 grid, and gradient-based against Laplacian
+JSON line also carries (under "extras") the same pass with L^1 and L^4; the
+16.8M-vertex boundary-layer pass (4096^2, L^4, 5 sweeps) with its adjoint
+forward/backward; the drop-in's per-call sequence from host arrays (copies
+and mesh creation included); the refinement filter with Steiner points and
+the coarsening's independent set on the default grid; the adjoint
+forward/backward on the 1024^2 grid; and gradient-based against Laplacian
 smoothing scored by the same energy.
 
 Inside the timed region only the dominant kernel's launches (smooth_phase)
@@ -425,7 +427,7 @@ def run_indep_set(T, mg, ctx, workload, frac=0.5, reps=5, seed=1234):
             "Mnodes_per_s": round(n / (ms * 1e-3) / 1e6, 1)}
 
 
-def run_adjoint(T, mg, ctx, workload, steps):
+def run_adjoint(T, mg, ctx, workload, steps, p=2):
     """Forward (metric + qualities + energy) and backward (dE/dxy, dE/df)
     of the energy sum_t (1 - q_M(t))^2."""
     import torch
@@ -436,12 +438,12 @@ def run_adjoint(T, mg, ctx, workload, steps):
     f = torch.from_numpy(m.solut).cuda()
     gx = torch.empty_like(xy)
     gf = torch.empty_like(f)
-    adj.forward(xy, f, 0.8, 2)
+    adj.forward(xy, f, 0.8, p)
     adj.backward(gx, gf)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        E = adj.forward(xy, f, 0.8, 2)
+        E = adj.forward(xy, f, 0.8, p)
     t1 = time.perf_counter()
     for _ in range(steps):
         adj.backward(gx, gf)
@@ -449,10 +451,11 @@ def run_adjoint(T, mg, ctx, workload, steps):
     t2 = time.perf_counter()
     fwd, bwd = (t1 - t0) * 1e3 / steps, (t2 - t1) * 1e3 / steps
     log(f"adjoint {workload}: forward {fwd:.3f} ms, backward {bwd:.3f} ms, E={E:.6g}")
-    out = {"workload": workload, "vertices": m.nv, "triangles": m.nt, "lp_norm": 2,
+    finite = bool(torch.isfinite(gx).all().item() and torch.isfinite(gf).all().item())
+    out = {"workload": workload, "vertices": m.nv, "triangles": m.nt, "lp_norm": p,
            "forward_ms": round(fwd, 4), "backward_ms": round(bwd, 4),
            "Mvertices_per_s_forward_backward": round(m.nv / ((fwd + bwd) * 1e-3) / 1e6, 1),
-           "energy": E}
+           "energy": E, "gradient_finite": finite}
     adj.close()
     dm.close()
     del adj, dm, xy, f, gx, gf
@@ -605,6 +608,7 @@ def main():
         extras["refine_filter"] = run_refine(T, mg, ctx, args.workload, args.p, seed=args.seed)
         extras["indep_set"] = run_indep_set(T, mg, ctx, args.workload, seed=args.seed)
         extras["adjoint"] = run_adjoint(T, mg, ctx, "jitter1024:fronts+quad", 5)
+        extras["stress_16M"]["adjoint"] = run_adjoint(T, mg, ctx, "jitter4096:layer", 2, p=4)
         extras["smoothing_modes"] = compare_smoothing(T, mg, ctx, "jitter256:fronts+quad", 5)
     extras["batched"] = {"status": "switched off",
                          "reason": "the batched pass stalled on MI355X and its cause has not "
