@@ -294,6 +294,7 @@ static unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 // every round makes progress.  state: 0 outside the subgraph, 1 undecided,
 // 2 in the set, 3 out.
 constexpr int kMisB = 256;
+constexpr int kMisRoundsPerSync = 16;  // the set needs 9 to 16 rounds on grids of 10^5 to 10^7
 
 __global__ __launch_bounds__(kMisB) void k_mis_mark(int32_t n, const int32_t* __restrict__ nodes,
                                                     int32_t nv, int32_t* __restrict__ state,
@@ -488,7 +489,7 @@ int trn_indep_set(trn_ctx* ctx, const trn_mesh* mesh, const int32_t* nodes_d, in
     *set_size = 0;
     if (n_nodes == 0) return;
     const int32_t nv = mesh->nv;
-    const size_t nslots = (size_t)kRoundsPerSync * kRoundSlots;
+    const size_t nslots = (size_t)kMisRoundsPerSync * kRoundSlots;
     Scratch sc(ctx, scratch_bytes(sizeof(int32_t) * (size_t)nv) +
                         scratch_bytes(sizeof(int32_t) * (16 + nslots)) +
                         2 * scratch_bytes(sizeof(int32_t) * ((size_t)n_nodes + 1)) +
@@ -507,7 +508,7 @@ int trn_indep_set(trn_ctx* ctx, const trn_mesh* mesh, const int32_t* nodes_d, in
     int rounds = 0;
     for (;;) {
       if (rounds) memset_async(ctx, slots, 0, sizeof(int32_t) * nslots);
-      for (int r = 0; r < kRoundsPerSync; ++r)
+      for (int r = 0; r < kMisRoundsPerSync; ++r)
         launch(ctx, "mis_round", 0.0, k_mis_round, dim3(nblk(n_nodes, kMisB)), dim3(kMisB), 0,
                n_nodes, nodes_d, nv, (const int32_t*)mesh->v2v_off, (const int32_t*)mesh->v2v,
                state, seed, slots + (size_t)r * kRoundSlots,
@@ -520,11 +521,11 @@ int trn_indep_set(trn_ctx* ctx, const trn_mesh* mesh, const int32_t* nodes_d, in
              n_nodes, nodes_d, (const int32_t*)flag, (const int32_t*)pos, set_d);
       copy_async(ctx, h, flags, sizeof(int32_t));
       copy_async(ctx, h + 1, pos + n_nodes, sizeof(int32_t));
-      copy_async(ctx, h + 2, slots + (size_t)(kRoundsPerSync - 1) * kRoundSlots,
+      copy_async(ctx, h + 2, slots + (size_t)(kMisRoundsPerSync - 1) * kRoundSlots,
                  sizeof(int32_t) * kRoundSlots);
       sync(ctx);
       TRN_REQUIRE(h[0] == 0, TRN_ERR_ARG, "a node id is outside [0, nv)");
-      rounds += kRoundsPerSync;
+      rounds += kMisRoundsPerSync;
       int left = 0;
       for (int k = 0; k < kRoundSlots; ++k) left += h[2 + k];
       if (left == 0) break;
