@@ -329,6 +329,7 @@ __global__ void k_union_offsets(int32_t nslab, const int32_t* __restrict__ row,
 // the slab's ELL offset, and, for every patch entry, its position in the
 // union.
 constexpr int kUnionB = 256;
+constexpr int kUnionWindow = kUnionKeys * 16;  // ids a bitmap in the key array covers
 
 __global__ __launch_bounds__(kUnionB) void k_patch_union(
     int32_t nv, const int32_t* __restrict__ row, const int32_t* __restrict__ ell,
@@ -344,10 +345,91 @@ __global__ __launch_bounds__(kUnionB) void k_patch_union(
     if (tid == 0) union_n[b] = -1;
     return;
   }
-  int P = 1;
-  while (P < n) P <<= 1;
   const int32_t* src = ell + r0 * kPatchSlab;
   const int64_t v0 = (int64_t)b * kPatchSlab;  // first vertex of the slab
+  // Window of the slab's triangle ids.  When it spans fewer than
+  // kUnionWindow ids (a few grid rows on a numbered mesh), a bitmap over it
+  // gives the ascending union and each entry's rank (word prefix plus the
+  // bits below it) without sorting; otherwise the entries are sorted.
+  __shared__ int32_t wlo, whi;
+  if (tid == 0) {
+    wlo = INT32_MAX;
+    whi = -1;
+  }
+  __syncthreads();
+  int32_t lo = INT32_MAX, hi = -1;
+  for (int k = tid; k < n; k += kUnionB) {
+    const int32_t t = v0 + (k % kPatchSlab) < nv ? src[k] : -1;
+    if (t >= 0) {
+      lo = t < lo ? t : lo;
+      hi = t > hi ? t : hi;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const int32_t l2 = __shfl_xor(lo, o, 64), h2 = __shfl_xor(hi, o, 64);
+    lo = l2 < lo ? l2 : lo;
+    hi = h2 > hi ? h2 : hi;
+  }
+  if ((tid & 63) == 0) {
+    atomicMin(&wlo, lo);
+    atomicMax(&whi, hi);
+  }
+  __syncthreads();
+  const int32_t L = wlo, H = whi;
+  if (H >= L && H - L < kUnionWindow) {
+    uint32_t* bits = reinterpret_cast<uint32_t*>(keys);  // words [0, kUnionWindow / 32)
+    int32_t* wpre = keys + kUnionWindow / 32;             // their exclusive prefixes
+    const int nw = ((H - L) >> 5) + 1;
+    for (int w = tid; w < nw; w += kUnionB) bits[w] = 0;
+    __syncthreads();
+    for (int k = tid; k < n; k += kUnionB) {
+      const int32_t t = v0 + (k % kPatchSlab) < nv ? src[k] : -1;
+      if (t >= 0) atomicOr(&bits[(t - L) >> 5], 1u << ((t - L) & 31));
+    }
+    __syncthreads();
+    const int Cw = (nw + kUnionB - 1) / kUnionB, w0 = tid * Cw;
+    int cnt = 0;
+    for (int w = w0; w < w0 + Cw && w < nw; ++w) cnt += __popc(bits[w]);
+    part[tid] = cnt;
+    __syncthreads();
+    if (tid == 0) {
+      int s = 0;
+      for (int w = 0; w < kUnionB; ++w) {
+        const int c = part[w];
+        part[w] = s;
+        s += c;
+      }
+      total = s;
+      union_n[b] = s <= kUnionCap ? s : -1;
+    }
+    __syncthreads();
+    if (total > kUnionCap) return;
+    int32_t* U = union_ids + union_off[b];
+    int pos = part[tid];
+    for (int w = w0; w < w0 + Cw && w < nw; ++w) {
+      wpre[w] = pos;
+      uint32_t word = bits[w];
+      while (word) {
+        U[pos++] = L + (w << 5) + (__ffs(word) - 1);
+        word &= word - 1;
+      }
+    }
+    __syncthreads();
+    uint16_t* dst = local + r0 * kPatchSlab;
+    for (int k = tid; k < n; k += kUnionB) {
+      const int32_t t = v0 + (k % kPatchSlab) < nv ? src[k] : -1;
+      uint16_t v = 0xFFFF;
+      if (t >= 0) {
+        const int32_t d = t - L, w = d >> 5;
+        v = (uint16_t)(wpre[w] + __popc(bits[w] & ((1u << (d & 31)) - 1u)));
+      }
+      dst[k] = v;
+    }
+    return;
+  }
+  int P = 1;
+  while (P < n) P <<= 1;
   for (int k = tid; k < P; k += kUnionB) {
     const int32_t t = (k < n && v0 + (k % kPatchSlab) < nv) ? src[k] : -1;
     keys[k] = t >= 0 ? t : INT32_MAX;
