@@ -98,16 +98,23 @@ static inline void mat_to_sym_add(const double m[4], double a[3]) {
   a[0] += m[0]; a[1] += m[1] + m[2]; a[2] += m[3];
 }
 
-/* divided differences of log and exp, stable for close arguments */
+/* divided differences of log and exp, stable for close arguments.  These
+ * are the adjoint's own derivatives, not reference code: they call the
+ * functions the device calls (shared_libm.cpp), so that both sides round
+ * them identically. */
+double orc_shared_log(double x);
+double orc_shared_exp(double x);
+double orc_shared_log1p_small(double r);
+double orc_shared_expm1(double d);
 static double dd_log(double x, double y) {
   if (x == y) return 1. / x;
   const double r = (x - y) / y;
-  if (fabs(r) < 1e-4) return log1p(r) / (x - y);
-  return (log(x) - log(y)) / (x - y);
+  if (fabs(r) < 1e-4) return orc_shared_log1p_small(r) / (x - y);
+  return (orc_shared_log(x) - orc_shared_log(y)) / (x - y);
 }
 static double dd_exp(double x, double y) {
-  if (x == y) return exp(x);
-  return exp(y) * expm1(x - y) / (x - y);
+  if (x == y) return orc_shared_exp(x);
+  return orc_shared_exp(y) * orc_shared_expm1(x - y) / (x - y);
 }
 
 /* ---- forward pieces as the reference computes them ---------------------- */

@@ -2,17 +2,20 @@
 and of the gradient-based smoothing mode, through the C ABI.
 
   forward metric        equals trn_metric_run's bit for bit (same pass)
-  energy                1e-12 relative vs the FP64 CPU reference (device
-                        log/exp/pow differ from glibc by at most one ulp)
-  dE/dxy, dE/df         1e-10 of the largest entry vs the FP64 CPU
-                        reference adjoint (oracle/trinity_adjoint.c); 1e-9 on
-                        the pure tanh-fronts field, whose saturated regions
-                        make the gradient sensitive to the last bit of
-                        log/exp/pow: perturbing those by one ulp in the CPU
-                        reference itself moves its dE/dxy by 3.2e-10 of the
-                        largest entry on the 128^2 case (2e-12 or less on the
-                        other fields), and device and glibc libm differ by
-                        one ulp
+  energy                1e-12 relative vs the FP64 CPU reference
+  dE/dxy, dE/df         vs the FP64 CPU reference adjoint
+                        (oracle/trinity_adjoint.c): 1e-10 of the largest
+                        entry on smooth fields; 1e-9 on the tanh-fronts
+                        fields, whose gradient is ill-conditioned against
+                        rounding in the backward itself: multiplying the
+                        CPU reference's divided differences of log and exp
+                        by (1 + 2^-52) moves its own dE/dxy by 2.9e-10 of
+                        the largest entry (and single vertices by 8e-4 of
+                        their own gradient) on the 128^2 fronts case, and the
+                        two sides add the backward's terms in different
+                        orders.  The forward is identical on both sides
+                        (exp/pow bit-identical to the host libm, log
+                        correctly rounded, the same divided-difference code)
   central differences   steps 1e-6 and 5e-7 of the GPU energy, Richardson
                         extrapolated, on 1,000 seeded random vertices (x, y
                         and f), checked as a distribution: at 64^2 the
@@ -123,10 +126,8 @@ def _report(tag, res):
               f"{np.mean(rich <= 1e-3):.3f}, within 1e-2 {np.mean(rich <= 1e-2):.3f}")
 
 
-def test_adjoint_central_differences_64():
+def test_adjoint_central_differences_64(ctx):
     """On a 64^2 mesh E is smooth at the step scale almost everywhere."""
-    import trinity_amd
-    ctx = trinity_amd.Context(0)
     m = mg.workload("jitter64:fronts+quad")
     dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags)
     adj = T().Adjoint(ctx, dm)
@@ -139,13 +140,14 @@ def test_adjoint_central_differences_64():
         assert np.median(rich) <= 1e-6
         assert np.mean(rich <= 1e-5) >= 0.97
         assert np.mean(rich <= 1e-3) >= 0.99
-    adj.close(), dm.close(), ctx.close()
+    adj.close(), dm.close()
 
 
-def test_adjoint_1024_matches_cpu_reference_and_differences():
+def test_adjoint_1024_matches_cpu_reference_and_differences(ctx):
     """The 1024^2 jittered grid (1,050,625 vertices): the GPU adjoint equals
-    the FP64 CPU reference adjoint to 5e-10 of the largest entry (the
-    reference's own sensitivity to one-ulp libm differences is 1e-10), and
+    the FP64 CPU reference adjoint to 1e-9 of the largest entry (3.0e-10
+    measured; the backward's own rounding sensitivity, see the module
+    docstring), and
     central differences (step 1e-6, 1,000 seeded random vertices, x, y and
     f) converge to it.  At this resolution E is not smooth at the step
     scale near the curves where a recovered-Hessian eigenvalue crosses zero
@@ -153,8 +155,6 @@ def test_adjoint_1024_matches_cpu_reference_and_differences():
     metric clamps bind, so the differences are checked as a distribution,
     with the bounds measured on MI355X: extrapolated median <= 1e-4 and at
     least 80% of the samples within 1e-3."""
-    import trinity_amd
-    ctx = trinity_amd.Context(0)
     m = mg.workload("jitter1024:fronts+quad")
     dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags)
     adj = T().Adjoint(ctx, dm)
@@ -164,19 +164,16 @@ def test_adjoint_1024_matches_cpu_reference_and_differences():
     gx, gf = gx.cpu().numpy(), gf.cpu().numpy()
     Eo, gxo, gfo = oracle.energy_grad(m.xy, m.tri, m.solut, 0.8, 2)
     assert abs(E - Eo) <= 1e-12 * Eo
-    # one-ulp changes of log/exp/pow move the CPU reference's own dE/dxy by
-    # 1.0e-10 of its largest entry here (3.5e-11 for dE/df): the bound is
-    # five times that
-    assert np.abs(gx - gxo).max() <= 5e-10 * np.abs(gxo).max()
-    assert np.abs(gf - gfo).max() <= 5e-10 * np.abs(gfo).max()
     print(f"1024^2 vs CPU reference: dxy {np.abs(gx - gxo).max() / np.abs(gxo).max():.2e} "
-          f"df {np.abs(gf - gfo).max() / np.abs(gfo).max():.2e}")
+          f"df {np.abs(gf - gfo).max() / np.abs(gfo).max():.2e}", flush=True)
+    assert np.abs(gx - gxo).max() <= 1e-9 * np.abs(gxo).max()
+    assert np.abs(gf - gfo).max() <= 1e-9 * np.abs(gfo).max()
     res = _fd_check(adj, m, xy, f, gx, gf, 1000, 2024)
     _report("1024^2", res)
     for kind, (plain, rich) in res.items():
         assert np.median(rich) <= 1e-4
         assert np.mean(rich <= 1e-3) >= 0.8
-    adj.close(), dm.close(), ctx.close()
+    adj.close(), dm.close()
 
 
 def test_gradient_smoothing_decreases_energy(ctx):

@@ -1,41 +1,30 @@
 """GPU parity at the benchmark's full sizes, against the oracle run on the
 host (OpenMP C, up to 16 threads):
 
-  jitter2048:fronts  4,198,401 vertices, L^2, 3 sweeps (the headline pass)
+  jitter2048:fronts  4,198,401 vertices, L^1, L^2 and L^4, 3 sweeps
   jitter4096:layer  16,785,409 vertices, L^4, 5 sweeps (the stress pass)
   flip1024:fronts    1,050,625 vertices with randomly flipped diagonals
                      (1 to 8 incident triangles; 31 % of the vertices have
                      more than 6 and take the generic move), L^2, 3 sweeps
 
-Checked: gradient and raw Hessian bit-exact; the locally normalised metric
-1e-14 per vertex (norm-relative; device pow within 1 ulp of glibc); the
-final metric bit-identical to the oracle's global normalisation applied to
-the device's local metric with the device complexity (IEEE operations only).
-Against the oracle's own final metric the boundary-layer grid shows a tail
-(3 of 16.8 M vertices above 1e-10, the worst 5.5e-10): where the local
-metric is nearly isotropic and one eigenvalue is clamped, the reference's
-eigen-clamp-rebuild turns a 1-ulp difference of pow into a rotation of the
-clamped eigenvector, so that comparison is printed, not bounded.  Then the
-device colouring valid; the
-sweeps, fed the device colouring and the oracle's metric, with exact
-per-sweep commit and task counts and coordinates 1e-12 per vertex.
-Size-independent properties beside them: every colour class independent
-(the checker), boundary vertices unmoved, every cached quality positive.
+Checked, at the tolerances of the small sizes:
+  gradient, raw Hessian        bit-exact
+  locally normalised metric    bit-exact (the device pow is glibc's)
+  final metric                 1e-10 per vertex (norm-relative) against the
+                               oracle's own; also bit-identical to the
+                               oracle's global normalisation of the device's
+                               local metric with the device complexity
+  sweeps (device colouring,    per-sweep commit and task counts exact,
+  oracle's metric fed to both) coordinates 1e-12, smoothed metric 1e-10
+Size-independent properties beside them: the colouring valid, boundary
+vertices unmoved, every cached quality positive.
 
-Two bounds are wider than at the small sizes, for reasons of size:
-  complexity   a sum of nt positive terms; the oracle adds them in `threads`
-               sequential chunks, whose rounding error is bounded by
-               (nt / threads + threads) * 2^-52 relative (the device tree
-               is far tighter).  1.9e-12 was measured on the 33.5 M-triangle
-               grid, against 1e-12 used at the small sizes.
-  smoothed     exp(mean log) of the neighbours' tensors, through device
-  metric       log/exp within 1 ulp of glibc.  Where the mean log matrix is
-               nearly isotropic the reference's closed-form eigen solve
-               (term^2 - det) cancels about 9 digits, so a 1-ulp difference
-               in an input moves the result by about 1e-10: on the 2048^2
-               grid the worst vertex (eigenvalues 1.0059e8 and 1.0071e8)
-               differs by 1.4e-10 and 1 vertex in 4.2 M exceeds 1e-10.  The
-               bound is 1e-9, and at most 1 vertex in 10^5 above 1e-10.
+One bound is wider than at the small sizes, for reasons of size: the
+complexity is a sum of nt positive terms that the oracle adds in `threads`
+sequential chunks, whose rounding error is bounded by
+(nt / threads + threads) * 2^-52 relative (the device's tree is far
+tighter); 1.9e-12 was measured on the 33.5 M-triangle grid, against 1e-12
+used at the small sizes.
 """
 import os
 
@@ -68,6 +57,8 @@ def bits_equal(a, b):
 
 
 @pytest.mark.parametrize("wl,p,sweeps", [("jitter2048:fronts", 2, 3),
+                                         ("jitter2048:fronts", 1, 3),
+                                         ("jitter2048:fronts", 4, 3),
                                          ("jitter4096:layer", 4, 5),
                                          ("flip1024:fronts", 2, 3)])
 def test_full_size_pass_matches_oracle(ctx, wl, p, sweeps):
@@ -86,7 +77,7 @@ def test_full_size_pass_matches_oracle(ctx, wl, p, sweeps):
     threads = min(16, os.cpu_count() or 1)
     tol = (m.nt / threads + threads) * 2.0 ** -52
     assert abs(c - st["complexity"]) <= tol * st["complexity"]
-    # local normalisation: the only non-IEEE step (pow, within 1 ulp)
+    # local normalisation: pow is the only step that is not an IEEE operation
     mloc = T().normalize_local(ctx, hess.clone(), p).cpu().numpy()
     rl = normrel(mloc, st["metric_local"])
     # global normalisation of the device's own local metric with the device
@@ -97,8 +88,9 @@ def test_full_size_pass_matches_oracle(ctx, wl, p, sweeps):
     rm = normrel(tn, st["metric"])
     print(f"local metric: max {rl.max():.3e}; final metric vs the oracle's own: max {rm.max():.3e}, "
           f"above 1e-10: {(rm > 1e-10).sum()}", flush=True)
-    assert rl.max() <= 1e-14
+    assert bits_equal(mloc, st["metric_local"])
     assert bits_equal(tn, ref)
+    assert rm.max() <= 1e-10
     del grad, hess, ten
 
     col = T().color(ctx, dm, seed=1, max_colors=64)
@@ -121,7 +113,7 @@ def test_full_size_pass_matches_oracle(ctx, wl, p, sweeps):
         ev = np.linalg.eigvalsh(np.array([[to[v, 0], to[v, 1]], [to[v, 1], to[v, 2]]]))
         print(f"  vertex {v}: eigenvalues {ev.tolist()}, moved {bool((xgn[v] != m.xy[v]).any())}",
               flush=True)
-    assert r.max() <= 1e-9 and (r > 1e-10).sum() <= m.nv // 100000
+    assert r.max() <= 1e-10
     boundary = (m.tags & 4) != 0
     assert np.array_equal(xgn[boundary], m.xy[boundary])
     assert (qn > 0).all()

@@ -6,6 +6,7 @@
 #include <string>
 
 #include "trn_api.hpp"
+#include "trn_numeric.hpp"
 
 namespace trn {
 
@@ -139,10 +140,10 @@ __global__ void k_debug_math(int op, int64_t n, const double* __restrict__ x,
   if (i >= n) return;
   double r;
   switch (op) {
-    case 0: r = log(x[i]); break;
-    case 1: r = exp(x[i]); break;
-    case 2: r = pow(x[i], y[i]); break;
-    case 3: r = pow(x[i], 3.); break;
+    case 0: r = trn::libm::log(x[i]); break;
+    case 1: r = trn::libm::exp(x[i]); break;
+    case 2: r = trn::libm::pow(x[i], y[i]); break;
+    case 3: r = trn::cube_pow(x[i]); break;
     default: r = __builtin_sqrt(x[i]); break;
   }
   out[i] = r;

@@ -86,16 +86,17 @@ __device__ __forceinline__ void sym_to_mat(const double a0, const double a1, con
   m[3] = a2;
 }
 
-// divided differences of log and exp, stable for close arguments
+// divided differences of log and exp, stable for close arguments; the CPU
+// reference adjoint evaluates the same expressions with the same functions
 __device__ __forceinline__ double dd_log(const double x, const double y) {
   if (x == y) return 1. / x;
   const double r = (x - y) / y;
-  if (fabs(r) < 1e-4) return log1p(r) / (x - y);
-  return (log(x) - log(y)) / (x - y);
+  if (fabs(r) < 1e-4) return libm::log1p_small(r) / (x - y);
+  return (libm::log(x) - libm::log(y)) / (x - y);
 }
 __device__ __forceinline__ double dd_exp(const double x, const double y) {
-  if (x == y) return exp(x);
-  return exp(y) * expm1(x - y) / (x - y);
+  if (x == y) return libm::exp(x);
+  return libm::exp(y) * libm::expm1(x - y) / (x - y);
 }
 
 // Metrics::normalizeLocally (metric.cpp:113-148) as a spectral map.
@@ -108,7 +109,7 @@ __device__ __forceinline__ void local_jac(const Eig2& e, const double ex, double
     c[k] = dmax(a, kEps);
     dc[k] = (a < kEps) ? 0. : (e.val[k] < 0 ? -1. : 1.);
   }
-  const double s = pow(c[0] * c[1], ex);
+  const double s = libm::pow(c[0] * c[1], ex);
   const double ds0 = ex * s / c[0] * dc[0], ds1 = ex * s / c[1] * dc[1];
   jac[0] = dc[0] * s + c[0] * ds0;
   jac[1] = c[0] * ds1;
@@ -194,7 +195,7 @@ __device__ __forceinline__ void quality_adjoint(const double2 pa, const double2 
 #pragma unroll
   for (int k = 0; k < 3; ++k) {  // numeric.cpp:113-172
     eig2(tk[k][0], tk[k][1], tk[k][1], tk[k][2], ek[k]);
-    const double v[2] = {log(ek[k].val[0]), log(ek[k].val[1])};
+    const double v[2] = {libm::log(ek[k].val[0]), libm::log(ek[k].val[1])};
     double c[4];
     rebuild(v, ek[k].P, c);
     lsum[0] += c[0]; lsum[1] += c[1]; lsum[2] += c[2]; lsum[3] += c[3];
@@ -202,7 +203,7 @@ __device__ __forceinline__ void quality_adjoint(const double2 pa, const double2 
   const double lm[4] = {lsum[0] / 3, lsum[1] / 3, lsum[2] / 3, lsum[3] / 3};
   Eig2 el;
   eig2(lm[0], lm[1], lm[2], lm[3], el);
-  const double ev[2] = {exp(el.val[0]), exp(el.val[1])};
+  const double ev[2] = {libm::exp(el.val[0]), libm::exp(el.val[1])};
   double jm[4];
   rebuild(ev, el.P, jm);
   const double J0 = jm[0], J1 = jm[1], J2 = jm[3];

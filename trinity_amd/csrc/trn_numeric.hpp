@@ -2,9 +2,11 @@
 // as __device__ functions for gfx950.
 //
 // Every routine reproduces the operation order of the reference routine it
-// cites, so that, with -ffp-contract=off, results are bit-identical wherever
-// the same libm values are used (sqrt and division are IEEE-exact on both
-// sides; log/exp/pow come from the device math library).
+// cites, so that, with -ffp-contract=off, results are bit-identical: sqrt
+// and division are IEEE-exact on both sides, exp and pow are restatements of
+// the host C library's (trn_libm.hpp), and log is correctly rounded, which
+// the host's is on every metric eigenvalue and all but about 2e-3 of the
+// edge-length ratios near 1 that approxRiemannDist takes it of.
 //
 // std::min(a,b) == (b < a ? b : a) and std::max(a,b) == (a < b ? b : a) are
 // restated literally so that NaN and signed-zero behaviour match.
@@ -12,6 +14,8 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "trn_libm.hpp"
 
 namespace trn {
 
@@ -82,8 +86,8 @@ __device__ __forceinline__ void tensor_log(const double t0, const double t1,
                                            const double t2, double out[4]) {
   double val[2], vec[4];
   eigen(t0, t1, t1, t2, val, vec);
-  val[0] = log(val[0]);
-  val[1] = log(val[1]);
+  val[0] = libm::log(val[0]);
+  val[1] = libm::log(val[1]);
   rebuild(val, vec, out);
 }
 
@@ -94,8 +98,8 @@ __device__ __forceinline__ void tensor_exp_mean(double lm[4], const int nb,
   lm[0] /= nb; lm[1] /= nb; lm[2] /= nb; lm[3] /= nb;
   double val[2], vec[4], c[4];
   eigen(lm[0], lm[1], lm[2], lm[3], val, vec);
-  val[0] = exp(val[0]);
-  val[1] = exp(val[1]);
+  val[0] = libm::exp(val[0]);
+  val[1] = libm::exp(val[1]);
   rebuild(val, vec, c);
   r[0] = c[0];
   r[1] = c[1];
@@ -115,20 +119,9 @@ __device__ __forceinline__ void interpolate3(const double* ta, const double* tb,
   tensor_exp_mean(lm, 3, r);
 }
 
-// x^3 rounded once: x^2 = p + e exactly (FMA error term), p*x = q + f
-// exactly, x^3 = q + f + e*x with e*x below 2^-52 of q, so the final sum
-// rounds correctly except in cases within ~2^-100 of a tie.  The reference's
-// pow(x, 3) is glibc's pow, which is correctly rounded in all but ~1e-3 of
-// cases; the device library's pow(x, 3.) is expanded to x*x*x, which is not
-// in about a quarter of cases.
-__device__ __forceinline__ double cube_rn(const double x) {
-  const double p = x * x;
-  const double e = __builtin_fma(x, x, -p);
-  const double q = p * x;
-  if (!__builtin_isfinite(q)) return q;
-  const double f = __builtin_fma(p, x, -q);
-  return q + (f + e * x);
-}
+// pow(x, 3) as numeric.cpp:304 evaluates it: the C library's pow (which is
+// not x*x*x, nor always the correctly rounded cube).
+__device__ __forceinline__ double cube_pow(const double x) { return libm::pow(x, 3.); }
 
 // min(perim / 3, 3 / perim) as numeric.cpp:302 takes it, with one division:
 // rounding is monotonic, so for perim < 3 the rounded perim / 3 is at most
@@ -163,7 +156,7 @@ __device__ __forceinline__ double quality_tail(const double* pa, const double* p
   const double a3 = pc[1] - pa[1];
   const double area = 0.5 * fabs(a0 * a3 - a1 * a2);
   const double mean = perim_mean(perim);
-  const double norm = (12 * 1.7320508075688772) * cube_rn(mean * (2. - mean));
+  const double norm = (12 * 1.7320508075688772) * cube_pow(mean * (2. - mean));
   return area * norm * sdet / (perim * perim);
 }
 
@@ -255,7 +248,7 @@ __device__ __forceinline__ double riemann_dist(const double* pa, const double* p
   const double lmin = dmin(l0, l1);
   const double lmax = dmax(l0, l1);
   const double ratio = lmax / lmin;
-  return lmax * (ratio - 1) / (ratio * log(ratio));
+  return lmax * (ratio - 1) / (ratio * libm::log(ratio));
 }
 
 // Mesh::isCounterclockwise, src/mesh.cpp:700-713 (absolute threshold).

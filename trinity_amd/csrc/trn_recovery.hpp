@@ -110,7 +110,7 @@ __device__ __forceinline__ void normalize_local(double& t0, double& t1, double& 
   val[0] = dmax(fabs(val[0]), kEps);
   val[1] = dmax(fabs(val[1]), kEps);
   const double det = val[0] * val[1];
-  const double scale_loc = pow(det, ex);
+  const double scale_loc = libm::pow(det, ex);  // metric.cpp:129
   val[0] *= scale_loc;
   val[1] *= scale_loc;
   rebuild(val, vec, h);
