@@ -152,8 +152,6 @@ int trn_color(trn_ctx* ctx, const trn_mesh* mesh, uint64_t seed,
 int trn_color_classes(trn_ctx* ctx, const trn_mesh* mesh, int32_t nv,
                       const int32_t* color_d, int32_t ncolors, int32_t* order_d,
                       int32_t* class_off);
-/* Counts edges whose end points share a colour and non-isolated vertices
- * left uncoloured or outside 1..ncolors (0 and 0 = valid).  Synchronises. */
 /* Replaces Partit::extractIndepSet (src/partition.cpp:200-306) on the
  * subgraph Coarse::extractSubGraph builds (src/coarsening.cpp:307-319): the
  * n_nodes distinct vertices nodes_d (device) with their 1-rings, neighbours
@@ -164,6 +162,8 @@ int trn_color_classes(trn_ctx* ctx, const trn_mesh* mesh, int32_t nv,
  * seed); the reference's depends on its thread interleaving.  Synchronises. */
 int trn_indep_set(trn_ctx* ctx, const trn_mesh* mesh, const int32_t* nodes_d, int32_t n_nodes,
                   uint64_t seed, int32_t* set_d, int32_t* set_size);
+/* Counts edges whose end points share a colour and non-isolated vertices
+ * left uncoloured or outside 1..ncolors (0 and 0 = valid).  Synchronises. */
 int trn_color_check(trn_ctx* ctx, const trn_mesh* mesh, const int32_t* color_d,
                     int32_t ncolors, int64_t* bad_edges, int64_t* bad_vertices);
 
@@ -240,16 +240,13 @@ int trn_smooth_run_prepared(trn_ctx* ctx, const trn_mesh* mesh, double* xy_d,
 /* ---- batched form: many meshes of uneven sizes -----------------------------
  * No counterpart in the reference, which processes one mesh per Metrics /
  * Smooth object.  A batch is the disjoint union of nmesh meshes; one pass
- * performs, for every mesh, what Metrics::run (src/metric.cpp:59-70) and the
- * sweeps of Smooth::run (src/smoothing.cpp:293-297) do, with per-mesh
- * complexity and target (floor(nv_m * target_factor), metric.cpp:39).  Each
- * stage is ONE persistent-kernel launch over all meshes, fed by a work queue
- * of (stage, mesh, tile) items; the launches of a pass can be replayed from
- * a hipGraph.
- * SWITCHED OFF in this build: the pass stalled on MI355X and the cause has
- * not been shown.  trn_batch_pass, trn_batch_pass_graph, trn_batch_autotune
- * and trn_batch_pass_pipelined return TRN_ERR_STATE without launching;
- * creation, colouring and configuration work. */
+ * performs, for every mesh, what Metrics::run (src/metric.cpp:59-70) and
+ * Smooth::run after its colouring (src/smoothing.cpp:287-297) do, with the
+ * per-mesh target floor(nv_m * target_factor) (metric.cpp:39) and per-mesh
+ * complexity.  Every launch of the pass covers all meshes.  Each mesh's
+ * results are bit-identical to a plain pass over that mesh (trn_metric_run,
+ * then trn_smooth_run fed the batch colouring restricted to the mesh): the
+ * per-mesh complexity is summed in the plain path's order. */
 typedef struct trn_batch trn_batch;
 /* nv[m], nt[m] (host): sizes; tri (host): the meshes' triangle arrays
  * concatenated, each with its own 0-based vertex indices; tags (host or
@@ -265,46 +262,31 @@ int trn_batch_info(const trn_batch* batch, int32_t* nv_total, int32_t* nt_total,
  * for the sweeps.  Synchronises. */
 int trn_batch_color(trn_ctx* ctx, trn_batch* batch, uint64_t seed,
                     int32_t max_colors, int32_t* ncolors);
-/* Launch shape of the persistent kernel: threads per block (64/128/256),
- * vertices and triangles per queue item, blocks per CU. */
-int trn_batch_set_config(trn_batch* batch, int32_t block, int32_t tile_v,
-                         int32_t tile_t, int32_t blocks_per_cu);
-int trn_batch_get_config(const trn_batch* batch, int32_t* config /* [4] */);
+/* The kept colouring: device colour [nv] and class list [nv] (valid until
+ * the next trn_batch_color or destroy), host class offsets (ncolors + 1
+ * entries, may be NULL) and the colour count; any output may be NULL. */
+int trn_batch_coloring(const trn_batch* batch, const int32_t** color_d,
+                       const int32_t** order_d, int32_t* class_off, int32_t* ncolors);
 /* One pass over every mesh.  Arrays are the concatenations over meshes in
  * batch order (device).  xy_in_d (may be NULL: xy_d holds the input) are the
- * input coordinates read by the metric stages; the sweeps move xy_d, which
- * the pass first fills from xy_in_d.  complexity_d (device, [nmesh], may be
- * NULL) receives the per-mesh complexities; commits/tasks (host, [sweeps],
- * may be NULL) the per-sweep counters (then the call synchronises). */
+ * input coordinates; the pass copies them into xy_d, which the metric stages
+ * read and the sweeps move.  complexity_d (device, [nmesh], may be NULL)
+ * receives the per-mesh complexities; commits/tasks (host, [sweeps], may be
+ * NULL) the per-sweep counters summed over the meshes (then the call
+ * synchronises).  sweeps > 0 needs trn_batch_color first. */
 int trn_batch_pass(trn_ctx* ctx, trn_batch* batch, const double* xy_in_d,
                    double* xy_d, const double* solut_d, double target_factor,
                    int32_t lp_norm, double h_min, double h_max, double* tensor_d,
                    double* qualit_d, int32_t sweeps, double* complexity_d,
                    int32_t* commits, int32_t* tasks);
-/* Same pass replayed from a hipGraph captured on the first call with these
- * arguments (re-captured when they change).  Needs a non-null stream. */
+/* The same pass replayed from a hipGraph captured on the first call with
+ * these arguments (re-captured when they, the stream or the colouring
+ * change).  Needs a created stream (not the null stream) and timing off. */
 int trn_batch_pass_graph(trn_ctx* ctx, trn_batch* batch, const double* xy_in_d,
                          double* xy_d, const double* solut_d,
                          double target_factor, int32_t lp_norm, double h_min,
                          double h_max, double* tensor_d, double* qualit_d,
                          int32_t sweeps);
-/* The same pass as ONE launch that pipelines stages across meshes (an item
- * waits in the kernel for its mesh's previous stage).  Switched off. */
-int trn_batch_pass_pipelined(trn_ctx* ctx, trn_batch* batch,
-                             const double* xy_in_d, double* xy_d,
-                             const double* solut_d, double target_factor,
-                             int32_t lp_norm, double h_min, double h_max,
-                             double* tensor_d, double* qualit_d, int32_t sweeps);
-/* Raises the error recorded by the last pass, if any (synchronises). */
-int trn_batch_check(trn_ctx* ctx, trn_batch* batch);
-/* Times the pass over block sizes x tile sizes x occupancies on this batch,
- * keeps the fastest shape (returned in best[4] = block, tile_v, tile_t,
- * blocks_per_cu) and its time in best_ms.  xy_d is overwritten. */
-int trn_batch_autotune(trn_ctx* ctx, trn_batch* batch, const double* xy_in_d,
-                       double* xy_d, const double* solut_d,
-                       double target_factor, int32_t lp_norm, double h_min,
-                       double h_max, double* tensor_d, double* qualit_d,
-                       int32_t sweeps, int32_t* best, double* best_ms);
 
 /* ---- adjoint: gradient of a metric-space quality energy -------------------
  * No counterpart in the reference.  The energy is

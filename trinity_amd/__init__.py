@@ -43,8 +43,7 @@ EXPORTS = [
     "trn_smooth_prepare",
     "trn_smooth_run_prepared",
     "trn_batch_create", "trn_batch_destroy", "trn_batch_info", "trn_batch_color",
-    "trn_batch_set_config", "trn_batch_get_config", "trn_batch_pass", "trn_batch_pass_graph",
-    "trn_batch_pass_pipelined", "trn_batch_check", "trn_batch_autotune",
+    "trn_batch_coloring", "trn_batch_pass", "trn_batch_pass_graph",
     "trn_adjoint_create", "trn_adjoint_destroy", "trn_adjoint_forward", "trn_adjoint_backward",
     "trn_adjoint_smooth", "trn_debug_math",
 ]
@@ -116,17 +115,11 @@ def _load():
         "trn_batch_destroy": ([_vp], ctypes.c_int),
         "trn_batch_info": ([_vp, _P(_i32), _P(_i32), _P(_vp)], ctypes.c_int),
         "trn_batch_color": ([_vp, _vp, ctypes.c_uint64, _i32, _P(_i32)], ctypes.c_int),
-        "trn_batch_set_config": ([_vp, _i32, _i32, _i32, _i32], ctypes.c_int),
-        "trn_batch_get_config": ([_vp, _P(_i32)], ctypes.c_int),
+        "trn_batch_coloring": ([_vp, _P(_vp), _P(_vp), _P(_i32), _P(_i32)], ctypes.c_int),
         "trn_batch_pass": ([_vp, _vp, _vp, _vp, _vp, _f64, _i32, _f64, _f64, _vp, _vp, _i32, _vp,
                             _P(_i32), _P(_i32)], ctypes.c_int),
         "trn_batch_pass_graph": ([_vp, _vp, _vp, _vp, _vp, _f64, _i32, _f64, _f64, _vp, _vp,
                                   _i32], ctypes.c_int),
-        "trn_batch_pass_pipelined": ([_vp, _vp, _vp, _vp, _vp, _f64, _i32, _f64, _f64, _vp, _vp,
-                                      _i32], ctypes.c_int),
-        "trn_batch_check": ([_vp, _vp], ctypes.c_int),
-        "trn_batch_autotune": ([_vp, _vp, _vp, _vp, _vp, _f64, _i32, _f64, _f64, _vp, _vp, _i32,
-                                _P(_i32), _P(_f64)], ctypes.c_int),
         "trn_adjoint_create": ([_vp, _vp, _P(_vp)], ctypes.c_int),
         "trn_adjoint_destroy": ([_vp], ctypes.c_int),
         "trn_adjoint_forward": ([_vp, _vp, _vp, _vp, _f64, _i32, _f64, _f64, _P(_f64), _vp, _vp],
@@ -507,9 +500,11 @@ def smooth_run_prepared(ctx, mesh, xy, tensor, qualit, coloring: Coloring, sweep
 
 
 class Batch:
-    """Many meshes of uneven sizes processed in one persistent launch per
-    pass (trn_batch).  No counterpart in the reference, which runs
-    Metrics::run and Smooth::run on one mesh at a time.
+    """Many meshes of uneven sizes processed together (trn_batch): the
+    disjoint union of the meshes runs through the plain kernels, every launch
+    covering every mesh, with per-mesh complexity and target.  No
+    counterpart in the reference, which runs Metrics::run and Smooth::run on
+    one mesh at a time.
 
     meshes: list of (nv, tri[nt,3] 0-based per mesh, tags[nv]).  Field
     arrays passed to `run` are the per-mesh arrays concatenated in list
@@ -542,19 +537,17 @@ class Batch:
         self.ncolors = nc.value
         return nc.value
 
-    def set_config(self, block=64, tile_v=256, tile_t=1024, blocks_per_cu=4) -> None:
-        _check(lib().trn_batch_set_config(self.handle, int(block), int(tile_v), int(tile_t),
-                                          int(blocks_per_cu)))
-
-    def config(self) -> tuple:
-        c = (_i32 * 4)()
-        _check(lib().trn_batch_get_config(self.handle, c))
-        return tuple(c)
+    def coloring(self) -> np.ndarray:
+        """The union's colouring (host copy, [nv], colours 1..ncolors)."""
+        cptr = _vp()
+        _check(lib().trn_batch_coloring(self.handle, ctypes.byref(cptr), None, None, None))
+        return _d2h_i32(cptr.value, self.nv)
 
     def run(self, xy_in, xy, solut, tensor, qualit, target_factor=0.8, lp_norm=2, h_min=1e-9,
             h_max=0.4, sweeps=3, complexity=None, counters=False):
         """One pass (metric + quality cache + sweeps) over every mesh.
-        Returns (commits, tasks) per sweep when counters=True."""
+        Returns (commits, tasks) per sweep, summed over the meshes, when
+        counters=True."""
         commits = (_i32 * max(sweeps, 1))()
         tasks = (_i32 * max(sweeps, 1))()
         _check(lib().trn_batch_pass(
@@ -569,34 +562,14 @@ class Batch:
 
     def run_graph(self, xy_in, xy, solut, tensor, qualit, target_factor=0.8, lp_norm=2,
                   h_min=1e-9, h_max=0.4, sweeps=3):
-        """The same pass replayed from a captured hipGraph (needs the
-        context's own stream: Context.use_own_stream)."""
+        """The same pass replayed from a hipGraph captured on the first call
+        with these arguments (needs a created stream, e.g. the context's own:
+        Context.use_own_stream)."""
         _check(lib().trn_batch_pass_graph(
-            self.ctx.handle, self.handle, _vp(xy_in.data_ptr()), _vp(xy.data_ptr()),
-            _vp(solut.data_ptr()), float(target_factor), int(lp_norm), float(h_min),
-            float(h_max), _vp(tensor.data_ptr()), _vp(qualit.data_ptr()), int(sweeps)))
-
-    def run_pipelined(self, xy_in, xy, solut, tensor, qualit, target_factor=0.8, lp_norm=2,
-                      h_min=1e-9, h_max=0.4, sweeps=3):
-        """The single-launch pipelined pass (switched off: raises)."""
-        _check(lib().trn_batch_pass_pipelined(
-            self.ctx.handle, self.handle, _vp(xy_in.data_ptr()), _vp(xy.data_ptr()),
-            _vp(solut.data_ptr()), float(target_factor), int(lp_norm), float(h_min),
-            float(h_max), _vp(tensor.data_ptr()), _vp(qualit.data_ptr()), int(sweeps)))
-
-    def check(self) -> None:
-        _check(lib().trn_batch_check(self.ctx.handle, self.handle))
-
-    def autotune(self, xy_in, xy, solut, tensor, qualit, target_factor=0.8, lp_norm=2,
-                 h_min=1e-9, h_max=0.4, sweeps=3):
-        best = (_i32 * 4)()
-        ms = _f64()
-        _check(lib().trn_batch_autotune(
-            self.ctx.handle, self.handle, _vp(xy_in.data_ptr()), _vp(xy.data_ptr()),
-            _vp(solut.data_ptr()), float(target_factor), int(lp_norm), float(h_min),
-            float(h_max), _vp(tensor.data_ptr()), _vp(qualit.data_ptr()), int(sweeps), best,
-            ctypes.byref(ms)))
-        return tuple(best), ms.value
+            self.ctx.handle, self.handle, _vp(_ptr(xy_in)) if xy_in is not None else None,
+            _vp(xy.data_ptr()), _vp(solut.data_ptr()), float(target_factor), int(lp_norm),
+            float(h_min), float(h_max), _vp(tensor.data_ptr()), _vp(qualit.data_ptr()),
+            int(sweeps)))
 
     def close(self) -> None:
         if getattr(self, "handle", None):

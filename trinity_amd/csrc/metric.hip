@@ -188,14 +188,12 @@ struct GradBuffers {
   double* tarea;
   double* partial;
   double* cplx;
-  int32_t* err;
 };
 
 static size_t grad_scratch_bytes(const trn_mesh* m) {
   return scratch_bytes(sizeof(double) * m->nt) + scratch_bytes(sizeof(double2) * m->nt) +
          scratch_bytes(sizeof(double4) * m->nt) + scratch_bytes(sizeof(double) * m->nv) +
-         scratch_bytes(sizeof(double) * kCplxBlocks) + scratch_bytes(sizeof(double) * 4) +
-         scratch_bytes(sizeof(int32_t) * 4);
+         scratch_bytes(sizeof(double) * kCplxBlocks) + scratch_bytes(sizeof(double) * 4);
 }
 
 static GradBuffers carve_grad(Scratch& sc, const trn_mesh* m) {
@@ -206,7 +204,6 @@ static GradBuffers carve_grad(Scratch& sc, const trn_mesh* m) {
   b.tarea = sc.carve<double>(m->nv);
   b.partial = sc.carve<double>(kCplxBlocks);
   b.cplx = sc.carve<double>(4);
-  b.err = sc.carve<int32_t>(4);
   return b;
 }
 
@@ -270,10 +267,30 @@ static void run_complexity(trn_ctx* ctx, const trn_mesh* m, const double* xy,
          (const double*)b.partial, blocks, b.cplx);
 }
 
-static void check_err(trn_ctx* ctx, GradBuffers& b) {
-  copy_async(ctx, ctx->pinned_i32, b.err, sizeof(int32_t));
-  sync(ctx);
-  TRN_REQUIRE(ctx->pinned_i32[0] == 0, TRN_ERR_STATE, "2-ring patch larger than measured");
+// ---- internal launch sequence (also used by the batched pass) ------------
+
+size_t recover_scratch(const trn_mesh* m) {
+  return scratch_bytes(sizeof(double) * m->nt) + scratch_bytes(sizeof(double2) * m->nt) +
+         scratch_bytes(sizeof(double4) * m->nt) + scratch_bytes(sizeof(double) * m->nv) +
+         scratch_bytes(sizeof(double2) * m->nv);
+}
+
+// Gradient and Hessian recovery with the local normalisation fused
+// (Metrics::computeGradient, computeHessian, normalizeLocally) into tensor.
+void recover_launch(trn_ctx* ctx, const trn_mesh* m, const double* xy, const double* f,
+                    double* grad_out, double* hess_out, double* tensor, int lp_norm,
+                    void* scratch, size_t bytes) {
+  Carve c(scratch, bytes);
+  GradBuffers b;
+  b.area_t = c.carve<double>(m->nt);
+  b.ag_t = c.carve<double2>(m->nt);
+  b.ah_t = c.carve<double4>(m->nt);
+  b.tarea = c.carve<double>(m->nv);
+  b.partial = nullptr;
+  b.cplx = nullptr;
+  double* grad = grad_out ? grad_out : (double*)c.carve<double2>(m->nv);
+  run_gradient(ctx, m, xy, f, grad, b);
+  run_hessian(ctx, m, xy, grad, hess_out, tensor, true, metric_exponent(lp_norm), b);
 }
 
 }  // namespace trn
@@ -295,7 +312,6 @@ int trn_metric_run(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d,
     Scratch sc(ctx, grad_scratch_bytes(m) + scratch_bytes(sizeof(double2) * m->nv));
     GradBuffers b = carve_grad(sc, m);
     double* grad = gradient_d ? gradient_d : (double*)sc.carve<double2>(m->nv);
-    memset_async(ctx, b.err, 0, sizeof(int32_t));
     run_gradient(ctx, m, xy_d, solut_d, grad, b);
     run_hessian(ctx, m, xy_d, grad, hessian_d, tensor_d, true, metric_exponent(lp_norm), b);
     run_complexity(ctx, m, xy_d, tensor_d, b);
@@ -305,7 +321,7 @@ int trn_metric_run(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d,
            1 / (h_min * h_min), 1 / (h_max * h_max));
     if (complexity) {
       copy_async(ctx, ctx->pinned_f64, b.cplx, sizeof(double));
-      check_err(ctx, b);
+      sync(ctx);
       *complexity = ctx->pinned_f64[0];
     }
   });
@@ -319,9 +335,7 @@ int trn_gradient(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d, const d
     TRN_REQUIRE(xy_d && solut_d && gradient_d, TRN_ERR_ARG, "null array");
     Scratch sc(ctx, grad_scratch_bytes(mesh));
     GradBuffers b = carve_grad(sc, mesh);
-    memset_async(ctx, b.err, 0, sizeof(int32_t));
     run_gradient(ctx, mesh, xy_d, solut_d, gradient_d, b);
-    check_err(ctx, b);
   });
 }
 
@@ -333,7 +347,6 @@ int trn_hessian(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d, const do
     TRN_REQUIRE(xy_d && gradient_d && tensor_d, TRN_ERR_ARG, "null array");
     Scratch sc(ctx, grad_scratch_bytes(mesh));
     GradBuffers b = carve_grad(sc, mesh);
-    memset_async(ctx, b.err, 0, sizeof(int32_t));
     // total stencil areas: the same ordered sums as the gradient pass
     launch(ctx, "tri_area", 0.0, k_tri_grad, dim3(nblk(mesh->nt, 256)), dim3(256), 0, mesh->nt,
            (const int32_t*)mesh->tri, (const double2*)xy_d, (const double*)nullptr, b.area_t,
@@ -345,7 +358,6 @@ int trn_hessian(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d, const do
            (const int32_t*)mesh->union_ids, (const uint16_t*)mesh->patch_local, mesh->max_union,
            (const double*)b.area_t, (const double2*)b.ag_t, (double2*)nullptr, b.tarea);
     run_hessian(ctx, mesh, xy_d, gradient_d, nullptr, tensor_d, false, 0.0, b);
-    check_err(ctx, b);
   });
 }
 

@@ -125,8 +125,34 @@ Scratch::Scratch(trn_ctx* ctx, size_t total) : ctx_(ctx) {
 
 void sync(trn_ctx* ctx) { TRN_HIP(hipStreamSynchronize(ctx->stream)); }
 
+// Byte fill as a kernel rather than hipMemsetAsync, so that every sequence
+// of launches a caller may capture into a hipGraph holds kernel nodes only
+// (memset nodes have given wrong results on graph replays).  16-byte
+// stores over the aligned middle, single bytes at the ends.
+__global__ __launch_bounds__(256) void k_fill_bytes(uint8_t* __restrict__ p, uint32_t v,
+                                                    uint64_t head, uint64_t nvec,
+                                                    uint64_t bytes) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t w = v * 0x01010101u;
+  uint4* q = (uint4*)(p + head);
+  for (uint64_t k = tid; k < nvec; k += stride) q[k] = make_uint4(w, w, w, w);
+  const uint64_t tail0 = head + 16 * nvec;
+  for (uint64_t k = tid; k < head; k += stride) p[k] = (uint8_t)v;
+  for (uint64_t k = tail0 + tid; k < bytes; k += stride) p[k] = (uint8_t)v;
+}
+
 void memset_async(trn_ctx* ctx, void* p, int v, size_t bytes) {
-  if (bytes) TRN_HIP(hipMemsetAsync(p, v, bytes, ctx->stream));
+  if (!bytes) return;
+  uint64_t head = (16 - ((uintptr_t)p & 15)) & 15;
+  if (head > bytes) head = bytes;
+  const uint64_t nvec = (bytes - head) / 16;
+  uint64_t blocks = (nvec + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_fill_bytes, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
+                     (uint8_t*)p, (uint32_t)(v & 0xff), head, nvec, (uint64_t)bytes);
+  TRN_HIP(hipGetLastError());
 }
 
 void copy_async(trn_ctx* ctx, void* dst, const void* src, size_t bytes) {

@@ -307,6 +307,73 @@ __global__ __launch_bounds__(kStatB) void k_quality_final(int32_t nparts,
   }
 }
 
+// ---- internal launch sequences (also used by the batched pass) -----------
+
+void smooth_prepare_launch(trn_ctx* ctx, const trn_mesh* m, const double* xy,
+                           const double* tensor, double* qualit, bool recompute_qualit,
+                           double4* logm) {
+  launch(ctx, "pack_records", 256.0 * m->nv, k_pack_records, dim3(nblk(m->nv, 256)), dim3(256),
+         0, m->nv, (const double2*)xy, tensor, m->vrec, logm);
+  if (recompute_qualit) quality_cache_logs(ctx, m, xy, logm, qualit);
+}
+
+size_t smooth_sweep_scratch(const trn_mesh* m, int ncolors, int sweeps, int max_class) {
+  const size_t ncount = 2 * (size_t)kCounterSlots * (size_t)(sweeps + 1);
+  const size_t nlive = (size_t)(sweeps + 1) * (size_t)(ncolors + 1);
+  return scratch_bytes(sizeof(int32_t) * (ncount + nlive)) +
+         scratch_bytes(sizeof(int32_t) * 2 * (size_t)(sweeps + 1)) +
+         scratch_bytes((size_t)m->nv) + scratch_bytes(sizeof(int32_t) * (size_t)max_class);
+}
+
+int32_t* smooth_sweeps_launch(trn_ctx* ctx, const trn_mesh* m, double* xy, double* tensor,
+                              double* qualit, const int32_t* order_d, const int32_t* class_off,
+                              int ncolors, int sweeps, int max_class, void* scratch,
+                              size_t scratch_size, bool want_totals) {
+  const size_t ncount = 2 * (size_t)kCounterSlots * (size_t)(sweeps + 1);
+  const size_t nlive = (size_t)(sweeps + 1) * (size_t)(ncolors + 1);
+  Carve sc(scratch, scratch_size);
+  int32_t* counters = sc.carve<int32_t>(ncount + nlive);
+  int32_t* live_count = counters + ncount;
+  int32_t* totals = sc.carve<int32_t>(2 * (size_t)(sweeps + 1));
+  uint8_t* stale = sc.carve<uint8_t>(m->nv);
+  int32_t* live = sc.carve<int32_t>((size_t)(max_class > 0 ? max_class : 1));
+  VRecord* rec = m->vrec;
+  memset_async(ctx, counters, 0, sizeof(int32_t) * (ncount + nlive));
+  int block;
+  size_t shmem;
+  phase_shape(m, &block, &shmem);
+  for (int s = 0; s < sweeps; ++s) {
+    int32_t* cs = counters + 2 * kCounterSlots * s;
+    for (int c = 0; c < ncolors; ++c) {
+      const int32_t count = class_off[c + 1] - class_off[c];
+      if (count == 0) continue;
+      const int32_t* list = order_d + class_off[c];
+      const int32_t* count_d = nullptr;
+      if (s > 0) {  // nothing is stale before the first sweep
+        int32_t* lc = live_count + (size_t)s * (ncolors + 1) + c;
+        launch(ctx, "class_live", 5.0 * count, k_class_live, dim3(nblk(count, kLiveB * kLivePer)),
+               dim3(kLiveB), 0, count, list, (const uint8_t*)stale, live, lc, cs);
+        list = live;
+        count_d = lc;
+      }
+      // algorithmic bytes: per moved vertex; a live list's length is known
+      // on the device only, so its launch reports it as work units
+      unsigned long long* units =
+          count_d ? timing_units(ctx, "smooth_phase", kBytesSweepPerV) : nullptr;
+      launch(ctx, "smooth_phase", count_d ? 0.0 : kBytesSweepPerV * count, k_smooth_phase,
+             dim3(nblk(count, block)), dim3(block), shmem, count, count_d, list, rec, qualit,
+             stale, (const int32_t*)m->v2t_off, (const int32_t*)m->v2t, (const int32_t*)m->star,
+             (const int32_t*)m->v2v_off, (const int32_t*)m->v2v, cs, units);
+    }
+  }
+  launch(ctx, "unpack_records", 80.0 * m->nv, k_unpack_records, dim3(nblk(m->nv, 256)),
+         dim3(256), 0, m->nv, (const VRecord*)rec, (double2*)xy, tensor);
+  if (!want_totals || sweeps == 0) return nullptr;
+  launch(ctx, "counter_reduce", 0.0, k_counter_reduce, dim3(1), dim3(1024), 0, sweeps,
+         (const int32_t*)counters, totals);
+  return totals;
+}
+
 }  // namespace trn
 
 using namespace trn;
@@ -360,12 +427,9 @@ int trn_smooth_prepare(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d,
   return guard_impl("trn_smooth_prepare", [&] {
     DeviceScope ds(ctx);
     TRN_REQUIRE(mesh && xy_d && tensor_d && qualit_d, TRN_ERR_ARG, "null argument");
-    const trn_mesh* m = mesh;
-    Scratch sc(ctx, scratch_bytes(sizeof(double4) * (size_t)m->nv));
-    double4* logm = sc.carve<double4>(m->nv);
-    launch(ctx, "pack_records", 256.0 * m->nv, k_pack_records, dim3(nblk(m->nv, 256)),
-           dim3(256), 0, m->nv, (const double2*)xy_d, (const double*)tensor_d, m->vrec, logm);
-    if (recompute_qualit) quality_cache_logs(ctx, m, xy_d, logm, qualit_d);
+    Scratch sc(ctx, scratch_bytes(sizeof(double4) * (size_t)mesh->nv));
+    smooth_prepare_launch(ctx, mesh, xy_d, tensor_d, qualit_d, recompute_qualit != 0,
+                          sc.carve<double4>(mesh->nv));
   });
 }
 
@@ -383,57 +447,17 @@ int trn_smooth_run_prepared(trn_ctx* ctx, const trn_mesh* mesh, double* xy_d, do
       TRN_REQUIRE(class_off[c] >= 0 && class_off[c] <= class_off[c + 1] &&
                       class_off[c + 1] <= mesh->nv,
                   TRN_ERR_ARG, "class offsets out of range");
-    const trn_mesh* m = mesh;
-    const size_t ncount = 2 * (size_t)kCounterSlots * (size_t)(sweeps + 1);
-    const size_t nlive = (size_t)(sweeps + 1) * (size_t)(ncolors + 1);
     int32_t max_class = 1;
     for (int c = 0; c < ncolors; ++c)
       max_class = std::max(max_class, class_off[c + 1] - class_off[c]);
-    Scratch sc(ctx, scratch_bytes(sizeof(int32_t) * (ncount + nlive)) +
-                        scratch_bytes(sizeof(int32_t) * 2 * (size_t)(sweeps + 1)) +
-                        scratch_bytes((size_t)m->nv) +
-                        scratch_bytes(sizeof(int32_t) * (size_t)max_class));
-    int32_t* counters = sc.carve<int32_t>(ncount + nlive);
-    int32_t* live_count = counters + ncount;
-    int32_t* totals = sc.carve<int32_t>(2 * (size_t)(sweeps + 1));
-    uint8_t* stale = sc.carve<uint8_t>(m->nv);
-    int32_t* live = sc.carve<int32_t>((size_t)max_class);
-    VRecord* rec = m->vrec;
-    memset_async(ctx, counters, 0, sizeof(int32_t) * (ncount + nlive));
-    int block;
-    size_t shmem;
-    phase_shape(m, &block, &shmem);
-    for (int s = 0; s < sweeps; ++s) {
-      int32_t* cs = counters + 2 * kCounterSlots * s;
-      for (int c = 0; c < ncolors; ++c) {
-        const int32_t count = class_off[c + 1] - class_off[c];
-        if (count == 0) continue;
-        const int32_t* list = order_d + class_off[c];
-        const int32_t* count_d = nullptr;
-        if (s > 0) {  // nothing is stale before the first sweep
-          int32_t* lc = live_count + (size_t)s * (ncolors + 1) + c;
-          launch(ctx, "class_live", 5.0 * count, k_class_live,
-                 dim3(nblk(count, kLiveB * kLivePer)), dim3(kLiveB), 0, count, list,
-                 (const uint8_t*)stale, live, lc, cs);
-          list = live;
-          count_d = lc;
-        }
-        // algorithmic bytes: per moved vertex; a live list's length is
-        // known on the device only, so its launch reports it as work units
-        unsigned long long* units =
-            count_d ? timing_units(ctx, "smooth_phase", kBytesSweepPerV) : nullptr;
-        launch(ctx, "smooth_phase", count_d ? 0.0 : kBytesSweepPerV * count, k_smooth_phase,
-               dim3(nblk(count, block)), dim3(block), shmem, count, count_d, list, rec, qualit_d,
-               stale, (const int32_t*)m->v2t_off, (const int32_t*)m->v2t,
-               (const int32_t*)m->star, (const int32_t*)m->v2v_off, (const int32_t*)m->v2v, cs,
-               units);
-      }
-    }
-    launch(ctx, "unpack_records", 80.0 * m->nv, k_unpack_records, dim3(nblk(m->nv, 256)),
-           dim3(256), 0, m->nv, (const VRecord*)rec, (double2*)xy_d, tensor_d);
-    if ((commits || tasks) && sweeps > 0) {
-      launch(ctx, "counter_reduce", 0.0, k_counter_reduce, dim3(1), dim3(1024), 0, sweeps,
-             (const int32_t*)counters, totals);
+    const size_t bytes = smooth_sweep_scratch(mesh, ncolors, sweeps, max_class);
+    Scratch sc(ctx, bytes);
+    void* base = sc.carve<char>(bytes);
+    const bool want = commits || tasks;
+    const int32_t* totals = smooth_sweeps_launch(ctx, mesh, xy_d, tensor_d, qualit_d, order_d,
+                                                 class_off, ncolors, sweeps, max_class, base,
+                                                 bytes, want);
+    if (totals) {
       copy_async(ctx, ctx->pinned_i32, totals, sizeof(int32_t) * 2 * (size_t)sweeps);
       sync(ctx);
       for (int s = 0; s < sweeps; ++s) {

@@ -41,6 +41,28 @@ void spatial_order(trn_ctx* ctx, int32_t nv, const double* xy, int32_t* perm);
 // src/hessian.cpp:53), from metric.hip's gather.
 void patch_area_sums(trn_ctx* ctx, const trn_mesh* m, const double* area_t, double* tarea);
 
+// Internal launch sequences shared by the entry points and the batched pass.
+// Each takes its device scratch from the caller (sized by the *_scratch
+// function) and only queues work on ctx->stream: no allocation, no host
+// synchronisation, no memset node, so a caller may capture it in a hipGraph.
+// metric.hip: gradient + Hessian recovery + local normalisation into tensor
+// (grad_out / hess_out optional stage outputs).
+size_t recover_scratch(const trn_mesh* m);
+void recover_launch(trn_ctx* ctx, const trn_mesh* m, const double* xy, const double* f,
+                    double* grad_out, double* hess_out, double* tensor, int lp_norm,
+                    void* scratch, size_t bytes);
+// smoothing.hip: Smooth::run's preparation (records, quality cache) and its
+// sweeps; logm is [nv] device scratch of the preparation.  The sweeps return
+// the device array of per-sweep (commits, tasks) when want_totals is set.
+void smooth_prepare_launch(trn_ctx* ctx, const trn_mesh* m, const double* xy,
+                           const double* tensor, double* qualit, bool recompute_qualit,
+                           double4* logm);
+size_t smooth_sweep_scratch(const trn_mesh* m, int ncolors, int sweeps, int max_class);
+int32_t* smooth_sweeps_launch(trn_ctx* ctx, const trn_mesh* m, double* xy, double* tensor,
+                              double* qualit, const int32_t* order_d, const int32_t* class_off,
+                              int ncolors, int sweeps, int max_class, void* scratch,
+                              size_t scratch_size, bool want_totals);
+
 // Reductions used by several passes.
 void reduce_max_i32(trn_ctx* ctx, const int32_t* in, int64_t n, int32_t* out_d);
 

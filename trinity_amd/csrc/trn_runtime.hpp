@@ -218,6 +218,27 @@ class Scratch {
   size_t off_ = 0;
 };
 
+// Slices of a caller-provided device buffer, carved in order (the same
+// alignment as Scratch), for internal launch sequences whose buffers live
+// in an object of their own (a batch) rather than the context workspace.
+class Carve {
+ public:
+  Carve(void* base, size_t bytes) : base_((char*)base), cap_(bytes) {}
+  template <typename T>
+  T* carve(size_t count) {
+    size_t bytes = scratch_bytes(count * sizeof(T));
+    TRN_REQUIRE(off_ + bytes <= cap_, TRN_ERR_STATE, "carve overflow");
+    T* p = reinterpret_cast<T*>(base_ + off_);
+    off_ += bytes;
+    return p;
+  }
+
+ private:
+  char* base_;
+  size_t cap_;
+  size_t off_ = 0;
+};
+
 void sync(trn_ctx* ctx);
 void memset_async(trn_ctx* ctx, void* p, int v, size_t bytes);
 
