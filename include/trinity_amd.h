@@ -237,6 +237,22 @@ int trn_smooth_run_prepared(trn_ctx* ctx, const trn_mesh* mesh, double* xy_d,
                             const int32_t* class_off, int32_t ncolors, int32_t sweeps,
                             int32_t* commits, int32_t* tasks);
 
+/* trn_smooth_run_prepared without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and counters as
+ * trn_smooth_run_prepared (the same kernels); its scratch (per-sweep
+ * counters, stale flags, live lists: about nv + 4 * largest class bytes) is
+ * a fixed 64 MiB device array per GPU, held until the process ends, and a
+ * mesh needing more returns TRN_ERR_CAPACITY.  Every call on one GPU shares
+ * that array, so calls that overlap on one GPU (other streams, other
+ * contexts, other threads) race; calls in stream order do not.  Per-kernel
+ * timing must be off. */
+int trn_smooth_run_prepared_noalloc(trn_ctx* ctx, const trn_mesh* mesh, double* xy_d,
+                                    double* tensor_d, double* qualit_d, const int32_t* order_d,
+                                    const int32_t* class_off, int32_t ncolors, int32_t sweeps,
+                                    int32_t* commits, int32_t* tasks);
+
 /* ---- batched form: many meshes of uneven sizes -----------------------------
  * No counterpart in the reference, which processes one mesh per Metrics /
  * Smooth object.  A batch is the disjoint union of nmesh meshes; one pass

@@ -42,6 +42,7 @@ EXPORTS = [
     "trn_smooth_run",
     "trn_smooth_prepare",
     "trn_smooth_run_prepared",
+    "trn_smooth_run_prepared_noalloc",
     "trn_batch_create", "trn_batch_destroy", "trn_batch_info", "trn_batch_color",
     "trn_batch_coloring", "trn_batch_pass", "trn_batch_pass_graph",
     "trn_adjoint_create", "trn_adjoint_destroy", "trn_adjoint_forward", "trn_adjoint_backward",
@@ -111,6 +112,8 @@ def _load():
         "trn_smooth_prepare": ([_vp, _vp, _vp, _vp, _vp, _i32], ctypes.c_int),
         "trn_smooth_run_prepared": ([_vp, _vp, _vp, _vp, _vp, _vp, _P(_i32), _i32, _i32,
                                      _P(_i32), _P(_i32)], ctypes.c_int),
+        "trn_smooth_run_prepared_noalloc": ([_vp, _vp, _vp, _vp, _vp, _vp, _P(_i32), _i32, _i32,
+                                             _P(_i32), _P(_i32)], ctypes.c_int),
         "trn_batch_create": ([_vp, _i32, _P(_i32), _P(_i32), _vp, _vp, _P(_vp)], ctypes.c_int),
         "trn_batch_destroy": ([_vp], ctypes.c_int),
         "trn_batch_info": ([_vp, _P(_i32), _P(_i32), _P(_vp)], ctypes.c_int),

@@ -26,6 +26,7 @@
 //                    the two other corners), so the kernel reads no
 //                    triangle array.
 #include <algorithm>
+#include <mutex>
 
 #include "trn_recovery.hpp"
 #include "trn_smooth.hpp"
@@ -374,6 +375,30 @@ int32_t* smooth_sweeps_launch(trn_ctx* ctx, const trn_mesh* m, double* xy, doubl
   return totals;
 }
 
+// ---- allocation-free sweeps ------------------------------------------------
+// Device memory of trn_smooth_run_prepared_noalloc: a file-scope array, one
+// copy per GPU, held until the process ends.  Host code never uses the
+// array's name as an address (it is a host-side stand-in there): the device
+// address comes from hipGetSymbolAddress, once per device, with that device
+// current, on the entry point's first call on it.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+constexpr int kNoAllocDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kNoAllocDevices] = {};
+
+static void* noalloc_scratch(int device) {
+  TRN_REQUIRE(device >= 0 && device < kNoAllocDevices, TRN_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    TRN_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    TRN_REQUIRE(ptr != nullptr, TRN_ERR_HIP, "hipGetSymbolAddress returned a null address");
+    g_scratch_ptr[device] = ptr;
+  }
+  return g_scratch_ptr[device];
+}
+
 }  // namespace trn
 
 using namespace trn;
@@ -457,6 +482,47 @@ int trn_smooth_run_prepared(trn_ctx* ctx, const trn_mesh* mesh, double* xy_d, do
     const int32_t* totals = smooth_sweeps_launch(ctx, mesh, xy_d, tensor_d, qualit_d, order_d,
                                                  class_off, ncolors, sweeps, max_class, base,
                                                  bytes, want);
+    if (totals) {
+      copy_async(ctx, ctx->pinned_i32, totals, sizeof(int32_t) * 2 * (size_t)sweeps);
+      sync(ctx);
+      for (int s = 0; s < sweeps; ++s) {
+        if (commits) commits[s] = ctx->pinned_i32[2 * s];
+        if (tasks) tasks[s] = ctx->pinned_i32[2 * s + 1];
+      }
+    }
+  });
+}
+
+int trn_smooth_run_prepared_noalloc(trn_ctx* ctx, const trn_mesh* mesh, double* xy_d,
+                                    double* tensor_d, double* qualit_d, const int32_t* order_d,
+                                    const int32_t* class_off, int32_t ncolors, int32_t sweeps,
+                                    int32_t* commits, int32_t* tasks) {
+  return guard_impl("trn_smooth_run_prepared_noalloc", [&] {
+    DeviceScope ds(ctx);
+    TRN_REQUIRE(mesh && xy_d && tensor_d && qualit_d && order_d && class_off, TRN_ERR_ARG,
+                "null argument");
+    TRN_REQUIRE(ncolors >= 0 && ncolors <= 64 && sweeps >= 0, TRN_ERR_ARG,
+                "bad ncolors or sweeps");
+    TRN_REQUIRE(sweeps <= 1024, TRN_ERR_ARG, "at most 1024 sweeps per call");
+    TRN_REQUIRE(!ctx->timing, TRN_ERR_STATE,
+                "per-kernel timing creates events: switch it off for the allocation-free call");
+    for (int c = 0; c < ncolors; ++c)
+      TRN_REQUIRE(class_off[c] >= 0 && class_off[c] <= class_off[c + 1] &&
+                      class_off[c + 1] <= mesh->nv,
+                  TRN_ERR_ARG, "class offsets out of range");
+    int32_t max_class = 1;
+    for (int c = 0; c < ncolors; ++c)
+      max_class = std::max(max_class, class_off[c + 1] - class_off[c]);
+    const size_t bytes = smooth_sweep_scratch(mesh, ncolors, sweeps, max_class);
+    TRN_REQUIRE(bytes <= kNoAllocScratchBytes, TRN_ERR_CAPACITY,
+                "the allocation-free sweeps need " + std::to_string(bytes) +
+                    " bytes of device scratch; its fixed array holds " +
+                    std::to_string(kNoAllocScratchBytes) + " (mesh too large for this call)");
+    void* base = noalloc_scratch(ctx->device);
+    const bool want = commits || tasks;
+    const int32_t* totals = smooth_sweeps_launch(ctx, mesh, xy_d, tensor_d, qualit_d, order_d,
+                                                 class_off, ncolors, sweeps, max_class, base,
+                                                 kNoAllocScratchBytes, want);
     if (totals) {
       copy_async(ctx, ctx->pinned_i32, totals, sizeof(int32_t) * 2 * (size_t)sweeps);
       sync(ctx);
