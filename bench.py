@@ -2,28 +2,31 @@
 """Benchmark of the MI355X metric-and-smoothing path.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
-                    [--sweeps S] [--p P] [--no-cpu-baseline]
+                    [--sweeps S] [--p P] [--no-cpu-baseline] [--no-extras]
 
 One step is one pass of the path over one synthetic mesh already resident in
 HBM: L2-projection gradient/Hessian recovery over the 2-ring patches, metric
 construction (eigen clamp, L^p local normalisation, complexity, target
 scaling), vertex colouring, and S coloured smoothing sweeps -- what
 Metrics::run + Smooth::run do in the reference (src/metric.cpp:59-70,
-src/smoothing.cpp:274-302).  The mesh adjacency (v2t/v2v) is built once
-before timing, as Mesh::load does before Metrics::run; each step restores
-the input coordinates (a 16 B/vertex device copy) because smoothing moves
-them.
+src/smoothing.cpp:274-302).  The mesh adjacency (v2t/v2v, stored 2-ring
+patches, vertex records) is built once before timing, as Mesh::load does
+before Metrics::run; the line also reports the per-call figure with that
+construction included.  Each step restores the input coordinates (a
+16 B/vertex device copy) because smoothing moves them.
 
-Default workload: 2048^2 jittered grid (4,198,401 vertices, 8,388,608
-triangles), sum of 3 seeded tanh fronts, L^2 metric with target factor 0.8,
-h_min=1e-9, h_max=0.4, GPU colouring, 3 sweeps.  Unless --no-extras, the
-JSON line also carries (under "extras") the same pass with L^1 and L^4; the
-16.8M-vertex boundary-layer pass (4096^2, L^4, 5 sweeps) with its adjoint
-forward/backward; the drop-in's per-call sequence from host arrays (copies
-and mesh creation included); the refinement filter with Steiner points and
-the coarsening's independent set on the default grid; the adjoint
-forward/backward on the 1024^2 grid; and gradient-based against Laplacian
-smoothing scored by the same energy.
+Default workload (the stress configuration the target is quoted on):
+4096^2 jittered grid (16,785,409 vertices, 33,554,432 triangles),
+boundary-layer field exp(-(y-0.3)/1e-3) + 0.1 sin(8 pi x), L^4 metric with
+target factor 0.8, h_min=1e-9, h_max=0.4, GPU colouring, 5 sweeps.  The CPU
+baseline is the reference's OpenMP Metrics::run + Smooth::run on the same
+mesh on this host's cores.  Unless --no-extras, the JSON line also carries
+(under "extras") the 2048^2 three-fronts pass at L^1, L^2 and L^4; the
+batched pass over 512 uneven meshes against looping the plain pass over
+them; the adjoint forward/backward on the 1024^2 grid and on the default
+mesh; the drop-in's per-call sequence from host arrays; the refinement
+filter with Steiner points and the coarsening's independent set; and
+gradient-based against Laplacian smoothing scored by the same energy.
 
 Inside the timed region only the dominant kernel's launches (smooth_phase)
 carry HIP events, for the roofline figures; the per-kernel table and the
@@ -104,11 +107,13 @@ def cpu_baseline(args) -> dict | None:
     import oracle
     from trinity_amd import meshgen as mg
 
-    n = args.cpu_sample_n
-    m = mg.workload(f"jitter{n}:fronts", seed=args.seed)
+    wl = args.cpu_workload or args.workload
+    m = mg.workload(wl, seed=args.seed)
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or cpu_cores()
-    sample = (f"jitter{n} fronts (V={m.nv:,}), p={args.p}, {args.sweeps} sweeps, "
-              f"Metrics::run + Smooth::run, median of {args.cpu_reps}")
+    sample = (f"{wl} (V={m.nv:,}, the {'same mesh as the headline' if wl == args.workload else 'a smaller mesh of the family'}), "
+              f"p={args.p}, {args.sweeps} sweeps, Metrics::run + Smooth::run wall time, "
+              f"{'median of ' + str(args.cpu_reps) if args.cpu_reps > 1 else 'one run'}; "
+              f"mesh loading untimed")
     if os.path.exists(oracle.REF_BENCH):
         with tempfile.TemporaryDirectory() as td:
             mp = os.path.join(td, "m.trnm")
@@ -124,7 +129,7 @@ def cpu_baseline(args) -> dict | None:
             log(f"cpu baseline (reference, {threads} threads): {r} in {time.time() - t0:.1f}s")
         t = (r["metric_ms"] + r["smooth_ms"]) * 1e-3
         out = {"value": round(m.nv / t / 1e6, 4), "unit": "Mvertices/s", "cores": threads,
-               "kind": "reference", "sample": sample,
+               "kind": "reference", "sample": sample, "workload": wl, "vertices": m.nv,
                "metric_ms": r["metric_ms"], "smooth_ms": r["smooth_ms"]}
         try:
             out["arithmetic_only"] = cpu_arithmetic_only(args, m, threads)
@@ -154,7 +159,7 @@ def free_gpu():
 
 def run_plain(T, mg, ctx, workload, p, sweeps, steps, warmup, seed, color_seed,
               locality=False, cctx=None, cstream=None, stat_steps=3,
-              timed_kernel="smooth_phase"):
+              timed_kernel="smooth_phase", create_reps=0):
     """Times `steps` passes (coordinates restored, metric, colouring, sweeps)
     and returns the wall time per pass and the per-kernel statistics.
 
@@ -182,6 +187,26 @@ def run_plain(T, mg, ctx, workload, p, sweeps, steps, warmup, seed, color_seed,
     log(f"{workload}: V={m.nv:,} T={m.nt:,} max 2-ring patch {dm.max_patch}, p={p}, "
         f"{sweeps} sweeps")
     ncolors = []
+    create_ms = None
+    if create_reps:
+        # the topology the reference builds inside each call (2-ring patches
+        # in Metrics::run, mesh.cpp:390-434; the primal graph in Smooth::run,
+        # mesh.cpp:297-330): trn_mesh_create from device-resident triangles
+        # and tags (CSR, stored patches and unions, star codes, records)
+        tri_d = torch.from_numpy(m.tri.reshape(-1)).cuda()
+        tags_d = torch.from_numpy(m.tags).cuda()
+        T.DeviceMesh(ctx, m.nv, tri_d, tags_d).close()
+        times = []
+        for _ in range(create_reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            d2 = T.DeviceMesh(ctx, m.nv, tri_d, tags_d)
+            torch.cuda.synchronize()
+            times.append((time.perf_counter() - t0) * 1e3)
+            d2.close()
+        create_ms = float(np.median(times))
+        del tri_d, tags_d
+        log(f"{workload}: device mesh creation {create_ms:.3f} ms (median of {create_reps})")
 
     def step():
         xy.copy_(xy0)
@@ -232,7 +257,7 @@ def run_plain(T, mg, ctx, workload, p, sweeps, steps, warmup, seed, color_seed,
     free_gpu()
     return {"nv": nv, "nt": nt, "ms": ms, "stats": stats, "timed_stats": timed_stats,
             "colours": int(np.median(ncolors)), "steps": steps, "stat_steps": stat_steps,
-            "colouring_overlapped": cctx is not None}
+            "colouring_overlapped": cctx is not None, "create_ms": create_ms}
 
 
 def summarize(r, sweeps, label=None):
@@ -291,9 +316,10 @@ def read_traffic(kernel, workload):
 
 
 def read_valu(kernel, workload):
-    """FP64 flops and VALU wave instructions per launch of `kernel` on
-    `workload` from the committed rocprofv3 summary (profiles/valu.txt:
-    'kernel workload fp64_flops valu_wave_instr waves'), or None."""
+    """(FP64 flops issued, VALU wave instructions, active-lane fraction or
+    None) per launch of `kernel` on `workload` from the committed rocprofv3
+    summary (profiles/valu.txt: 'kernel workload fp64_flops valu_wave_instr
+    waves [active_lane_fraction]'), or None."""
     path = os.path.join(ROOT, "profiles", "valu.txt")
     if not os.path.exists(path):
         return None
@@ -301,7 +327,8 @@ def read_valu(kernel, workload):
         parts = line.split()
         if len(parts) >= 4 and not line.startswith("#") and parts[0] == kernel \
                 and parts[1] == workload:
-            return float(parts[2]), float(parts[3])
+            active = float(parts[5]) if len(parts) >= 6 else None
+            return float(parts[2]), float(parts[3]), active
     return None
 
 
@@ -501,26 +528,178 @@ def compare_smoothing(T, mg, ctx, workload, iters):
     return out
 
 
+def run_batched(T, mg, count=512, p=2, sweeps=2, steps=5, warmup=2, loop_steps=2, seed=99,
+                color_seed=1):
+    """The uneven batch: `count` jittered grids with per-mesh vertex counts
+    log-uniform in [2e3, 2e5], each with its own seeded fronts field.  One
+    batched step = the batch colouring (trn_batch_color) + the pass replayed
+    from its hipGraph (coordinates restored, metric with per-mesh complexity
+    and target, quality cache, sweeps) -- every launch covering every mesh.
+    One plain step = the plain pass (coordinates restored, trn_metric_run,
+    trn_color, trn_smooth_run) looped over the meshes, each already resident
+    as its own device mesh.  Both are whole-batch wall times."""
+    import numpy as np
+    import torch
+    t0 = time.perf_counter()
+    ms = mg.batch_meshes(count, seed=seed)
+    nv = sum(m.nv for m in ms)
+    nt = sum(m.nt for m in ms)
+    log(f"batch: {count} meshes, {nv:,} vertices, {nt:,} triangles "
+        f"(generated in {time.perf_counter() - t0:.1f} s)")
+    # batched, on a context with its own stream (graph capture needs one)
+    gctx = T.Context(0, use_torch_stream=False)
+    gctx.use_own_stream()
+    b = T.Batch(gctx, [(m.nv, m.tri, m.tags) for m in ms])
+    xy_in = torch.from_numpy(np.concatenate([m.xy for m in ms])).cuda()
+    solut = torch.from_numpy(np.concatenate([m.solut for m in ms])).cuda()
+    xy = torch.empty_like(xy_in)
+    ten = torch.empty((nv, 3), dtype=torch.float64, device="cuda")
+    q = torch.empty(nt, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    ncol = b.color(seed=color_seed)
+    b.run(xy_in, xy, solut, ten, q, 0.8, p, 1e-9, 0.4, sweeps=sweeps)
+    gctx.synchronize()
+    # per-kernel view of one eager pass
+    gctx.set_timing(True)
+    gctx.reset_timing()
+    b.run(xy_in, xy, solut, ten, q, 0.8, p, 1e-9, 0.4, sweeps=sweeps)
+    kstats = gctx.timings()
+    gctx.set_timing(False)
+    for _ in range(warmup):
+        b.color(seed=color_seed)
+        b.run_graph(xy_in, xy, solut, ten, q, 0.8, p, 1e-9, 0.4, sweeps=sweeps)
+    gctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.color(seed=color_seed)
+        b.run_graph(xy_in, xy, solut, ten, q, 0.8, p, 1e-9, 0.4, sweeps=sweeps)
+    gctx.synchronize()
+    bms = (time.perf_counter() - t0) * 1e3 / steps
+    b.close()
+    del xy_in, solut, xy, ten, q
+    gctx.close()
+    free_gpu()
+    log(f"batched: {bms:.3f} ms per pass, {nv / (bms * 1e-3) / 1e6:.1f} Mvertices/s, "
+        f"{ncol} colours")
+    # plain pass looped over the meshes
+    ctx = T.Context(0)
+    dms, arrs = [], []
+    for m in ms:
+        dms.append(T.DeviceMesh(ctx, m.nv, m.tri, m.tags))
+        arrs.append((torch.from_numpy(m.xy).cuda(), torch.from_numpy(m.solut).cuda(),
+                     torch.empty((m.nv, 2), dtype=torch.float64, device="cuda"),
+                     torch.empty((m.nv, 3), dtype=torch.float64, device="cuda"),
+                     torch.empty(m.nt, dtype=torch.float64, device="cuda")))
+
+    def loop():
+        for dm, (x0, f, x, t, qq) in zip(dms, arrs):
+            x.copy_(x0)
+            T.metric_run(ctx, dm, x, f, 0.8, p, 1e-9, 0.4, tensor=t, want_complexity=False)
+            col = T.color(ctx, dm, seed=color_seed, max_colors=64)
+            T.smooth_run(ctx, dm, x, t, qq, col, sweeps, recompute_qualit=True, counters=False)
+
+    loop()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(loop_steps):
+        loop()
+    torch.cuda.synchronize()
+    pms = (time.perf_counter() - t0) * 1e3 / loop_steps
+    for dm in dms:
+        dm.close()
+    del dms, arrs
+    ctx.close()
+    free_gpu()
+    log(f"plain loop over {count} meshes: {pms:.3f} ms per pass, "
+        f"{nv / (pms * 1e-3) / 1e6:.1f} Mvertices/s")
+    bpv = BYTES_METRIC_PER_V + BYTES_COLOR_PER_V + BYTES_SWEEP_PER_V * sweeps
+    kern = {n: {"ms": round(v["ms"], 4), "launches": v["launches"]}
+            for n, v in sorted(kstats.items(), key=lambda kv: -kv[1]["ms"])[:10]}
+    return {"meshes": count, "vertices": nv, "triangles": nt, "lp_norm": p, "sweeps": sweeps,
+            "colours": ncol,
+            "batched": {"ms_per_pass": round(bms, 3),
+                        "Mvertices_per_s": round(nv / (bms * 1e-3) / 1e6, 2),
+                        "alg_GBps": round(bpv * nv / (bms * 1e-3) / 1e9, 1),
+                        "hbm_frac": round(bpv * nv / (bms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "how": "trn_batch_color + trn_batch_pass_graph (hipGraph replay)"},
+            "plain_loop": {"ms_per_pass": round(pms, 3),
+                           "Mvertices_per_s": round(nv / (pms * 1e-3) / 1e6, 2),
+                           "how": "per mesh: coordinates restored, trn_metric_run, trn_color, "
+                                  "trn_smooth_run, each mesh already a device mesh"},
+            "speedup_batched_vs_plain": round(pms / bms, 2),
+            "eager_pass_kernels": kern,
+            "bytes_per_vertex": bpv}
+
+
+def roofline_for(kname, dstat, workload, stats_all):
+    """Roofline of the dominant kernel from its HIP-event launch times:
+    compute-bound (FP64) when the committed rocprofv3 summary holds its FP64
+    work on this workload, else HBM-bound on its algorithmic bytes."""
+    bytes_per_launch = dstat["bytes"] / max(dstat["launches"], 1)
+    avg_s = dstat["ms"] * 1e-3 / max(dstat["launches"], 1)
+    alg_gbps = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    traffic = read_traffic(kname, workload)
+    valu = read_valu(kname, workload)
+    hbm = {"achieved": round(alg_gbps, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(alg_gbps / HBM_PEAK_GBS, 5),
+           "bytes_per_launch": round(bytes_per_launch, 1)}
+    if valu and avg_s > 0:
+        flops, winstr, active = valu
+        useful = flops * active if active else flops
+        tf = useful / avg_s / 1e12
+        out = {"kernel": kname, "bound": "mfma", "compute": "fp64",
+               "achieved": round(tf, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+               "frac": round(tf / FP64_PEAK_TFLOPS, 4), "traffic": traffic,
+               "avg_launch_us": round(avg_s * 1e6, 2),
+               "fp64_flops_per_launch": useful,
+               "lane_utilisation": active,
+               "issued_fp64_flops_per_launch": flops,
+               "valu_issue_frac": round(winstr * 4 / (1024 * avg_s * 2.4e9), 3),
+               "hbm": hbm,
+               "note": ("compute roof: FP64 vector arithmetic (the kernel issues no MFMA "
+                        "instruction; MI355X's FP64 vector and FP64 matrix peaks are both "
+                        "78.6 TFLOP/s).  achieved = FP64 flops per launch with masked lanes "
+                        "excluded (rocprofv3 F64 VALU instruction counters x 64 lanes x the "
+                        "active-lane fraction, profiles/valu.txt) / HIP-event launch time; "
+                        "hbm = algorithmic bytes per launch / the same time; traffic = HBM "
+                        "bytes per launch from FETCH_SIZE / WRITE_SIZE (profiles/traffic.txt)")}
+        return out
+    return {"kernel": kname, "bound": "hbm", "achieved": hbm["achieved"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": hbm["frac"], "traffic": traffic,
+            "avg_launch_us": round(avg_s * 1e6, 2),
+            "note": ("achieved = algorithmic bytes per launch / HIP-event launch time (events on "
+                     "this kernel's launches inside the timed region); traffic = HBM bytes per "
+                     "launch from rocprofv3 FETCH_SIZE and WRITE_SIZE (profiles/traffic.txt)")}
+
+
+def pass_stage(nv, ms, sweeps):
+    bpv = BYTES_METRIC_PER_V + BYTES_COLOR_PER_V + BYTES_SWEEP_PER_V * sweeps
+    gbps = bpv * nv / (ms * 1e-3) / 1e9
+    return {"ms": round(ms, 4), "bytes_per_vertex": bpv, "alg_GBps": round(gbps, 1),
+            "hbm_frac": round(gbps / HBM_PEAK_GBS, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="jitter2048:fronts")
-    ap.add_argument("--p", type=int, default=2)
-    ap.add_argument("--sweeps", type=int, default=3)
+    ap.add_argument("--workload", default="jitter4096:layer")
+    ap.add_argument("--p", type=int, default=4)
+    ap.add_argument("--sweeps", type=int, default=5)
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--color-seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
-                    help="only the headline pass (no p sweep, adjoint, 16.8M stress run)")
+                    help="only the headline pass (no 2048^2 passes, batch, adjoint, ...)")
     ap.add_argument("--serial-colouring", action="store_true",
                     help="colour on the pass's stream after the metric instead of on a "
                          "second stream beside it")
     ap.add_argument("--locality-order", action="store_true",
                     help="visit colour classes along a Z-curve of the coordinates")
-    ap.add_argument("--cpu-sample-n", type=int, default=1024)
-    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--cpu-workload", default=None,
+                    help="mesh of the CPU baseline (default: the headline's)")
+    ap.add_argument("--cpu-reps", type=int, default=1)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 512 MiB device-to-device 3 times (scales the rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE counters against known traffic)")
@@ -552,67 +731,54 @@ def main():
         cctx = T.Context(0, use_torch_stream=False)
         cctx.set_stream(cstream.cuda_stream)
     r = run_plain(T, mg, ctx, args.workload, args.p, args.sweeps, args.steps, args.warmup,
-                  args.seed, args.color_seed, args.locality_order, cctx, cstream)
+                  args.seed, args.color_seed, args.locality_order, cctx, cstream, create_reps=3)
     ms = r["ms"]
     value = r["nv"] / (ms * 1e-3) / 1e6
     kern, stages = summarize(r, args.sweeps, f"{args.workload} p={args.p}")
-    # whole pass against its compulsory bytes: recovery + metric 368, colouring
-    # 32 and 196 per sweep, per vertex
-    pass_bpv = BYTES_METRIC_PER_V + 32 + BYTES_SWEEP_PER_V * args.sweeps
-    pass_gbps = pass_bpv * r["nv"] / (ms * 1e-3) / 1e9
-    stages["pass"] = {"ms": round(ms, 4), "bytes_per_vertex": pass_bpv,
-                      "alg_GBps": round(pass_gbps, 1),
-                      "hbm_frac": round(pass_gbps / HBM_PEAK_GBS, 4)}
+    stages["pass"] = pass_stage(r["nv"], ms, args.sweeps)
+    if r["create_ms"] is not None:
+        stages["per_call_with_topology"] = {
+            "ms": round(ms + r["create_ms"], 4), "mesh_create_ms": round(r["create_ms"], 4),
+            "Mvertices_per_s": round(r["nv"] / ((ms + r["create_ms"]) * 1e-3) / 1e6, 2),
+            "note": "the pass plus trn_mesh_create from device-resident triangles and tags "
+                    "(v2t/v2v CSR, stored 2-ring patches and their unions, star codes, vertex "
+                    "records), which the reference rebuilds inside Metrics::run (getVicinity) "
+                    "and Smooth::run (extractPrimalGraph)"}
     stats = r["stats"]
     dname = max(kern.items(), key=lambda kv: kv[1]["ms_per_step"])[0]
     # the dominant kernel's launches carry HIP events inside the timed region
     dstat = r["timed_stats"].get(dname, stats[dname])
-    bytes_per_launch = dstat["bytes"] / max(dstat["launches"], 1)
-    avg_s = dstat["ms"] * 1e-3 / max(dstat["launches"], 1)
-    achieved = bytes_per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
-    traffic = read_traffic(dname, args.workload)
-    valu = read_valu(dname, args.workload)
-    fp64 = None
-    if valu and avg_s > 0:
-        flops, winstr = valu
-        fp64 = {"kernel": dname, "achieved": round(flops / avg_s / 1e12, 3),
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(flops / avg_s / 1e12 / FP64_PEAK_TFLOPS, 4),
-                "valu_issue_frac": round(winstr * 4 / (1024 * avg_s * 2.4e9), 3),
-                "note": ("FP64 flops per launch from rocprofv3 F64 VALU counters x 64 lanes "
-                         "(profiles/valu.txt) / HIP-event launch time; valu_issue_frac = VALU "
-                         "wave instructions x 4 cycles / (1024 SIMDs x time x 2.4 GHz)")}
+    roof = roofline_for(dname, dstat, args.workload, stats)
 
     extras = {}
     if not args.no_extras:
-        for p in (1, 4):
-            if p == args.p:
-                continue
-            rp = run_plain(T, mg, ctx, args.workload, p, args.sweeps, 3, 1, args.seed,
-                           args.color_seed, False, cctx, cstream, stat_steps=2)
-            _, st = summarize(rp, args.sweeps, f"{args.workload} p={p}")
-            extras[f"p{p}"] = {"ms_per_pass": round(rp["ms"], 4),
-                               "Mvertices_per_s": round(rp["nv"] / (rp["ms"] * 1e-3) / 1e6, 2),
-                               "stages": st}
-        rs = run_plain(T, mg, ctx, "jitter4096:layer", 4, 5, 2, 1, args.seed, args.color_seed,
-                       False, cctx, cstream, stat_steps=1)
-        _, st = summarize(rs, 5, "jitter4096:layer p=4")
-        extras["stress_16M"] = {"workload": "jitter4096:layer", "vertices": rs["nv"],
-                                "triangles": rs["nt"], "lp_norm": 4, "sweeps": 5,
-                                "ms_per_pass": round(rs["ms"], 3),
-                                "Mvertices_per_s": round(rs["nv"] / (rs["ms"] * 1e-3) / 1e6, 2),
-                                "stages": st}
-        extras["host_boundary"] = run_host_boundary(T, mg, ctx, args.workload, args.p,
-                                                    args.sweeps, seed=args.seed,
+        wl2 = "jitter2048:fronts"
+        for p in (2, 1, 4):
+            rp = run_plain(T, mg, ctx, wl2, p, 3, 10 if p == 2 else 3, 2 if p == 2 else 1,
+                           args.seed, args.color_seed, False, cctx, cstream,
+                           stat_steps=3 if p == 2 else 2)
+            kp, st = summarize(rp, 3, f"{wl2} p={p}")
+            st["pass"] = pass_stage(rp["nv"], rp["ms"], 3)
+            e = {"workload": wl2, "vertices": rp["nv"], "lp_norm": p, "sweeps": 3,
+                 "ms_per_pass": round(rp["ms"], 4),
+                 "Mvertices_per_s": round(rp["nv"] / (rp["ms"] * 1e-3) / 1e6, 2), "stages": st}
+            if p == 2:
+                dn = max(kp.items(), key=lambda kv: kv[1]["ms_per_step"])[0]
+                e["roofline"] = roofline_for(dn, rp["timed_stats"].get(dn, rp["stats"][dn]), wl2,
+                                             rp["stats"])
+            extras[f"fronts2048_p{p}"] = e
+        try:
+            extras["batched"] = run_batched(T, mg)
+        except Exception as e:  # the headline stands on its own
+            log(f"batched run failed: {e!r}")
+            extras["batched"] = {"error": repr(e)}
+        extras["host_boundary"] = run_host_boundary(T, mg, ctx, wl2, 2, 3, seed=args.seed,
                                                     color_seed=args.color_seed)
-        extras["refine_filter"] = run_refine(T, mg, ctx, args.workload, args.p, seed=args.seed)
-        extras["indep_set"] = run_indep_set(T, mg, ctx, args.workload, seed=args.seed)
+        extras["refine_filter"] = run_refine(T, mg, ctx, wl2, 2, seed=args.seed)
+        extras["indep_set"] = run_indep_set(T, mg, ctx, wl2, seed=args.seed)
         extras["adjoint"] = run_adjoint(T, mg, ctx, "jitter1024:fronts+quad", 5)
-        extras["stress_16M"]["adjoint"] = run_adjoint(T, mg, ctx, "jitter4096:layer", 2, p=4)
+        extras["adjoint_headline_mesh"] = run_adjoint(T, mg, ctx, args.workload, 2, p=args.p)
         extras["smoothing_modes"] = compare_smoothing(T, mg, ctx, "jitter256:fronts+quad", 5)
-    extras["batched"] = {"status": "switched off",
-                         "reason": "the batched pass stalled on MI355X and its cause has not "
-                                   "been shown; its entry points refuse without launching"}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -639,20 +805,16 @@ def main():
                    "sweeps": args.sweeps, "colouring": "gpu seeded speculative greedy" +
                    ("" if args.serial_colouring else ", on a second stream beside the metric"),
                    "parallelism": "single GPU"},
-        "roofline": {"kernel": dname, "bound": "hbm", "achieved": round(achieved, 2),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": traffic,
-                     "note": ("achieved = algorithmic bytes per launch / HIP-event launch time "
-                              "(events on this kernel's launches inside the timed region); "
-                              "traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE and "
-                              "WRITE_SIZE (profiles/traffic.txt)")},
-        "fp64_valu": fp64,
+        "roofline": roof,
         "stages": stages,
         "extras": extras,
         "cpu_baseline": cpu,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+        out["speedup_vs_cpu_note"] = ("GPU pass rate / the CPU's Metrics::run + Smooth::run "
+                                      "rate on " + ("the same mesh" if cpu.get("workload") ==
+                                                    args.workload else cpu.get("workload", "?")))
     print(json.dumps(out), flush=True)
 
 

@@ -41,8 +41,10 @@
 
 #if defined(__HIPCC__)
 #define TRN_LIBM_FN __host__ __device__ __forceinline__
+#define TRN_LIBM_COLD static __host__ __device__ __noinline__
 #else
 #define TRN_LIBM_FN static inline
+#define TRN_LIBM_COLD static __attribute__((noinline))
 #endif
 
 namespace trn {
@@ -156,8 +158,28 @@ TRN_LIBM_FN double exp_core(const double x, const double xtail, const uint32_t s
   return fma_(scale, tmp, scale);
 }
 
-// glibc exp.
-TRN_LIBM_FN double exp(const double x) { return exp_core(x, 0.0, 0, false); }
+// exp for arguments outside the main range, out of line (cold).
+TRN_LIBM_COLD double exp_rare(const double x) { return exp_core(x, 0.0, 0, false); }
+
+// glibc exp: the main range 2^-54 <= |x| < 512 inline and without branches
+// (the tables and arithmetic of exp_core), everything else out of line.
+TRN_LIBM_FN double exp(const double x) {
+  const uint32_t abstop = top12(x) & 0x7ff;
+  double kd = fma_(kInvLn2N, x, kShift);
+  const uint64_t ki = asu(kd);
+  kd -= kShift;
+  const double r = fma_(kd, kNegLn2loN, fma_(kd, kNegLn2hiN, x)) + 0.0;
+  const uint32_t idx = 2 * ((uint32_t)ki % kN);
+  const uint64_t top = ki << (52 - kTableBits);
+  const double tail = asd(kExpTab[idx]);
+  const uint64_t sbits = kExpTab[idx + 1] + top;
+  const double r2 = r * r;
+  const double tmp = fma_(r2 * r2, fma_(r, kC5, kC4), fma_(r2, fma_(r, kC3, kC2), tail + r));
+  const double scale = asd(sbits);
+  const double y = fma_(scale, tmp, scale);
+  if (abstop - top12(0x1p-54) >= top12(512.0) - top12(0x1p-54)) return exp_rare(x);
+  return y;
+}
 
 // log(x) = hi + *tail for a normal positive x given by its bits, relative
 // error about 2^-68 (pow's inner logarithm).
@@ -252,17 +274,28 @@ TRN_LIBM_FN double pow(const double x, const double y) {
 // Accurate log (see the file comment): x = 2^k z, r = z/c - 1 (exact),
 // log x = (k ln2_hi + log_hi c) + r - r^2/2 + r^3/3 + [k ln2_lo + log_lo c
 // + r^4 (-1/4 + r/5 - ... - r^8/12)], with the leading sums error-free.
-TRN_LIBM_FN double log(const double x) {
-  uint64_t ix = asu(x);
-  if (ix == asu(1.0)) return 0.0;
+TRN_LIBM_FN double log_main(uint64_t ix);
+
+// log of zero, negative, subnormal, infinite and NaN arguments, out of line.
+TRN_LIBM_COLD double log_rare(const double x) {
+  const uint64_t ix = asu(x);
   const uint32_t top = (uint32_t)(ix >> 48);
-  if (top - 0x0010 >= 0x7ff0 - 0x0010) {
-    if (2 * ix == 0) return -__builtin_inf();
-    if (ix == 0x7ff0000000000000ULL) return x;
-    if ((top & 0x8000) || (top & 0x7ff0) == 0x7ff0) return (x - x) / (x - x);
-    ix = asu(x * 0x1p52);   // subnormal: normalise
-    ix -= 52ULL << 52;
-  }
+  if (2 * ix == 0) return -__builtin_inf();
+  if (ix == 0x7ff0000000000000ULL) return x;
+  if ((top & 0x8000) || (top & 0x7ff0) == 0x7ff0) return (x - x) / (x - x);
+  return log_main(asu(x * 0x1p52) - (52ULL << 52));   // subnormal: normalise
+}
+
+TRN_LIBM_FN double log(const double x) {
+  const uint64_t ix = asu(x);
+  const uint32_t top = (uint32_t)(ix >> 48);
+  const double y = log_main(ix);
+  if (top - 0x0010 >= 0x7ff0 - 0x0010) return log_rare(x);
+  return y;
+}
+
+// The logarithm of a positive normal number given by its bits.
+TRN_LIBM_FN double log_main(uint64_t ix) {
   const uint64_t tmp = ix - kLogOff;
   const int i = (int)((tmp >> (52 - kTableBits)) % kN);
   const int k = (int)((int64_t)tmp >> 52);

@@ -123,6 +123,21 @@ __device__ __forceinline__ void interpolate3(const double* ta, const double* tb,
 // not x*x*x, nor always the correctly rounded cube).
 __device__ __forceinline__ double cube_pow(const double x) { return libm::pow(x, 3.); }
 
+// x^3 rounded once: x^2 = p + e exactly (FMA error term), p*x = q + f
+// exactly, x^3 = q + f + e*x with e*x below 2^-52 of q, so the final sum
+// rounds correctly except in cases within ~2^-100 of a tie.  It differs
+// from cube_pow by at most one ulp (glibc's pow is within 0.52 ulp), in
+// about 7 of 10^4 arguments; the smoothing tries use it to decide, and
+// fall back to cube_pow where that could change a decision.
+__device__ __forceinline__ double cube_rn(const double x) {
+  const double p = x * x;
+  const double e = __builtin_fma(x, x, -p);
+  const double q = p * x;
+  if (!__builtin_isfinite(q)) return q;
+  const double f = __builtin_fma(p, x, -q);
+  return q + (f + e * x);
+}
+
 // min(perim / 3, 3 / perim) as numeric.cpp:302 takes it, with one division:
 // rounding is monotonic, so for perim < 3 the rounded perim / 3 is at most
 // 1 and the rounded 3 / perim at least 1 (the reverse for perim > 3), and
@@ -146,32 +161,55 @@ __device__ __forceinline__ double edge_len(const double j[3], const double c0,
   return __builtin_sqrt(c0 * v0 + c1 * v1);
 }
 
+// The last steps of numeric::computeQuality (numeric.cpp:303-305) from the
+// area, the normalisation factor, sqrt(det j) and the perimeter.
+__device__ __forceinline__ double quality_finish(const double area, const double norm,
+                                                 const double sdet, const double perim) {
+  return area * norm * sdet / (perim * perim);
+}
+
 // Quality from the area, the perimeter and sqrt(det j) (numeric.cpp:296-305).
+// EXACT: the reference's value (its pow).  Otherwise the cube is the
+// correctly rounded one, which differs from the reference's pow(x, 3) by at
+// most one ulp (in about 7 of 10^4 arguments), and the quality is within 5
+// ulps of the reference's (one ulp from the cube, one more rounding in each
+// of the four products and quotients after it); *near is set when that
+// quality lies within 2^-48 (16 ulps) of q_min, the only case where it and
+// the reference's could fall on different sides of q_min in the comparison
+// of smoothing.cpp:109.
+template <bool EXACT = true>
 __device__ __forceinline__ double quality_tail(const double* pa, const double* pb,
                                                const double* pc, const double perim,
-                                               const double sdet) {
+                                               const double sdet, const double q_min = 0,
+                                               bool* near = nullptr) {
   const double a0 = pb[0] - pa[0];
   const double a1 = pc[0] - pa[0];
   const double a2 = pb[1] - pa[1];
   const double a3 = pc[1] - pa[1];
   const double area = 0.5 * fabs(a0 * a3 - a1 * a2);
   const double mean = perim_mean(perim);
-  const double norm = (12 * 1.7320508075688772) * cube_pow(mean * (2. - mean));
-  return area * norm * sdet / (perim * perim);
+  const double m2 = mean * (2. - mean);
+  const double norm = (12 * 1.7320508075688772) * (EXACT ? cube_pow(m2) : cube_rn(m2));
+  const double q = quality_finish(area, norm, sdet, perim);
+  if (!EXACT && near) *near = fabs(q - q_min) <= 0x1p-48 * q_min;
+  return q;
 }
 
 // Metric-space quality from the averaged tensor j and its sqrt(det j)
 // (numeric.cpp:275-305).
+template <bool EXACT = true>
 __device__ __forceinline__ double quality_from_mean(const double* pa,
                                                     const double* pb,
                                                     const double* pc,
                                                     const double j[3],
-                                                    const double sdet) {
+                                                    const double sdet,
+                                                    const double q_min = 0,
+                                                    bool* near = nullptr) {
   double perim = 0;
   perim += edge_len(j, pb[0] - pa[0], pb[1] - pa[1]);
   perim += edge_len(j, pc[0] - pb[0], pc[1] - pb[1]);
   perim += edge_len(j, pa[0] - pc[0], pa[1] - pc[1]);
-  return quality_tail(pa, pb, pc, perim, sdet);
+  return quality_tail<EXACT>(pa, pb, pc, perim, sdet, q_min, near);
 }
 
 // quality_from_mean of the triangle whose corner r is the point p and whose
@@ -179,25 +217,28 @@ __device__ __forceinline__ double quality_from_mean(const double* pa,
 // (the edge that does not move with p) given.  Whatever r, the reference
 // forms the two other edge vectors as a - p and p - b, and adds the three
 // lengths in corner order (edge k joins corners k and k+1), which this does.
+template <bool EXACT = true>
 __device__ __forceinline__ double quality_with_corner(const double pp[3][2], const int r,
                                                       const double px, const double py,
                                                       const double2 a, const double2 b,
                                                       const double j[3], const double sdet,
-                                                      const double lab) {
+                                                      const double lab, const double q_min = 0,
+                                                      bool* near = nullptr) {
   const double la = edge_len(j, a.x - px, a.y - py);  // edge r
   const double lb = edge_len(j, px - b.x, py - b.y);  // edge r+2
   double perim = 0;
   perim += r == 0 ? la : (r == 1 ? lb : lab);
   perim += r == 0 ? lab : (r == 1 ? la : lb);
   perim += r == 0 ? lb : (r == 1 ? lab : la);
-  return quality_tail(pp[0], pp[1], pp[2], perim, sdet);
+  return quality_tail<EXACT>(pp[0], pp[1], pp[2], perim, sdet, q_min, near);
 }
 
-__device__ __forceinline__ double quality_from_mean(const double* pa,
-                                                    const double* pb,
-                                                    const double* pc,
-                                                    const double j[3]) {
-  return quality_from_mean(pa, pb, pc, j, mean_sqrt_det(j));
+template <bool EXACT = true>
+__device__ __forceinline__ double quality_of_mean(const double* pa, const double* pb,
+                                                  const double* pc, const double j[3],
+                                                  const double q_min = 0,
+                                                  bool* near = nullptr) {
+  return quality_from_mean<EXACT>(pa, pb, pc, j, mean_sqrt_det(j), q_min, near);
 }
 
 // Log-Euclidean mean of three tensors from their cached matrix logarithms
@@ -230,7 +271,7 @@ __device__ __forceinline__ double quality(const double* pa, const double* pb,
                                           const double* tb, const double* tc) {
   double j[3];
   interpolate3(ta, tb, tc, j);
-  return quality_from_mean(pa, pb, pc, j);
+  return quality_of_mean(pa, pb, pc, j);
 }
 
 // numeric::approxRiemannDist (4-argument form), src/numeric.cpp:469-497.

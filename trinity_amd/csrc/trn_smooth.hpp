@@ -1,14 +1,12 @@
 // trn_smooth.hpp -- per-vertex body of the coloured smoothing
-// (Smooth::moveSmartLaplacian, src/smoothing.cpp:40-126), shared by the
-// per-colour kernels of smoothing.hip and the persistent batch kernel.
+// (Smooth::moveSmartLaplacian, src/smoothing.cpp:40-126) of the per-colour
+// kernels of smoothing.hip.
 //
 // The move reads, for the vertex and its 1-ring, coordinates, metric and
-// the cached matrix log of the metric.  Two layouts of that vertex data
-// share one move body through an accessor:
-//   SoaVerts   separate xy / metric / log arrays (the batch kernel);
-//   RecVerts   one 128-byte VRecord per vertex holding all three and the
-//              vertex's compact topology, so a neighbour costs one cache
-//              line instead of three or four (the per-colour sweeps).
+// the cached matrix log of the metric, through an accessor over the vertex
+// records (RecVerts: one 128-byte VRecord per vertex holding all three and
+// the vertex's compact topology, so a neighbour costs one cache line
+// instead of three or four).
 #pragma once
 
 #include <cfloat>
@@ -29,34 +27,12 @@ __device__ __forceinline__ double quality_logs(const int64_t t, const int32_t* _
   mean_from_logs(logm[a], logm[b], logm[c], j);
   const double2 pa = xy[a], pb = xy[b], pc = xy[c];
   const double A[2] = {pa.x, pa.y}, B[2] = {pb.x, pb.y}, C[2] = {pc.x, pc.y};
-  return quality_from_mean(A, B, C, j);
+  return quality_of_mean(A, B, C, j);
 }
 
 static_assert(offsetof(VRecord, xy) == 32 && offsetof(VRecord, t) == 48 &&
                   offsetof(VRecord, topo) == 72,
               "VRecord layout");
-
-struct SoaVerts {
-  double2* __restrict__ xy;
-  double* __restrict__ tensor;
-  double4* __restrict__ logm;
-  __device__ __forceinline__ double2 pos(const int32_t j) const { return xy[j]; }
-  __device__ __forceinline__ void metric(const int32_t j, double T[3]) const {
-    const double* tj = tensor + 3 * (int64_t)j;
-    T[0] = tj[0];
-    T[1] = tj[1];
-    T[2] = tj[2];
-  }
-  __device__ __forceinline__ double4 log(const int32_t j) const { return logm[j]; }
-  __device__ __forceinline__ void commit(const int32_t i, const double px, const double py,
-                                         const double TN[3], const double4 LN) const {
-    xy[i] = make_double2(px, py);
-    tensor[3 * (int64_t)i] = TN[0];
-    tensor[3 * (int64_t)i + 1] = TN[1];
-    tensor[3 * (int64_t)i + 2] = TN[2];
-    logm[i] = LN;
-  }
-};
 
 struct RecVerts {
   VRecord* __restrict__ rec;
@@ -101,34 +77,6 @@ __device__ __forceinline__ int32_t sel6(const int k, const int32_t v0, const int
   return r;
 }
 
-// FastTopo from the CSR arrays and the mesh's star codes
-// (r | a << 2 | b << 10, a and b 1-ring slots).
-__device__ __forceinline__ FastTopo fast_topo_csr(const int32_t i, const int32_t t0,
-                                                  const int deg, const int32_t n0, const int nb,
-                                                  const int32_t* __restrict__ v2t,
-                                                  const int32_t* __restrict__ star,
-                                                  const int32_t* __restrict__ v2v) {
-  FastTopo f;
-  f.deg = deg;
-  f.nb = nb;
-#pragma unroll
-  for (int k = 0; k < kVtopoDeg; ++k) {
-    f.ring[k] = k < nb ? v2v[n0 + k] : i;
-    if (k < deg) {
-      const int32_t ck = star[t0 + k];
-      f.tris[k] = v2t[t0 + k];
-      f.rot[k] = ck & 3;
-      f.ja[k] = v2v[n0 + ((ck >> 2) & 255)];
-      f.jb[k] = v2v[n0 + ((ck >> 10) & 255)];
-    } else {
-      f.tris[k] = 0;
-      f.rot[k] = 0;
-      f.ja[k] = f.jb[k] = i;
-    }
-  }
-  return f;
-}
-
 // FastTopo from a VTopo record (star bytes r | a << 2 | b << 5).
 __device__ __forceinline__ FastTopo fast_topo_rec(const int32_t i, const VTopo& v) {
   FastTopo f;
@@ -163,7 +111,9 @@ __device__ __forceinline__ FastTopo fast_topo_rec(const int32_t i, const VTopo& 
 // tries.  Arithmetic and its order are those of move_vertex_generic.
 // lds holds, per thread with stride `stride`, the averaged tensor of each
 // incident triangle (3 doubles), its sqrt(det) and the metric length of the
-// triangle's edge opposite the vertex: 5 * kVtopoDeg doubles.
+// triangle's edge opposite the vertex: 5 * kVtopoDeg doubles.  Returns 1
+// for a commit, 0 for none, 2 when a quality came within 16 ulps of q_min
+// (nothing written; the caller redoes the move with the exact quality).
 template <class V>
 __device__ __forceinline__ int move_vertex_fast(const int32_t i, const FastTopo& ft,
                                                 const double2 pini, const double T[3],
@@ -227,6 +177,7 @@ __device__ __forceinline__ int move_vertex_fast(const int32_t i, const FastTopo&
   }
 
   int valid = 0;
+  bool undecided = false;
   double w = 1;
   double px = P[0], py = P[1];
   double qs[kVtopoDeg];
@@ -248,15 +199,24 @@ __device__ __forceinline__ int move_vertex_fast(const int32_t i, const FastTopo&
         if (ccw(pp[0], pp[1], pp[2])) {
           const double J[3] = {jm[(3 * k) * stride], jm[(3 * k + 1) * stride],
                                jm[(3 * k + 2) * stride]};
-          qs[k] = quality_with_corner(pp, r, px, py, a, b, J, jm[(3 * kVtopoDeg + k) * stride],
-                                      jm[(4 * kVtopoDeg + k) * stride]);
-          valid = qs[k] > q_min;
+          // correctly rounded cube: decides as the reference does unless
+          // the quality is within 16 ulps of q_min (then the move is left
+          // undecided, for the exact path); a stored value may be one ulp
+          // from the reference's where glibc's pow(x, 3) is not correctly
+          // rounded
+          bool near = false;
+          qs[k] = quality_with_corner<false>(pp, r, px, py, a, b, J,
+                                             jm[(3 * kVtopoDeg + k) * stride],
+                                             jm[(4 * kVtopoDeg + k) * stride], q_min, &near);
+          undecided |= near;
+          valid = !near && qs[k] > q_min;
         }
       }
     }
     w *= 0.5;
-    if (valid) break;
+    if (valid || undecided) break;
   }
+  if (undecided) return 2;
   if (valid) {  // smoothing.cpp:116-123
 #pragma unroll
     for (int k = 0; k < kVtopoDeg; ++k)
@@ -272,7 +232,7 @@ __device__ __forceinline__ int move_vertex_fast(const int32_t i, const FastTopo&
 // each incident triangle.  The corners of incident triangle k are taken in
 // the reference's vertex order (the log-Euclidean mean accumulates in that
 // order).
-template <class V>
+template <bool EXACT, class V>
 __device__ __forceinline__ int move_vertex_generic(
     const int32_t i, const int32_t t0, const int deg, const int32_t n0, const int nb,
     const double2 pini, const double T[3], const V& v, double* __restrict__ qualit,
@@ -345,7 +305,12 @@ __device__ __forceinline__ int move_vertex_generic(
           J[1] = jm[(3 * k + 1) * stride];
           J[2] = jm[(3 * k + 2) * stride];
         }
-        const double qk = quality_from_mean(pp[0], pp[1], pp[2], J);
+        // EXACT: the reference's quality; otherwise the correctly rounded
+        // cube, and the move is left undecided (2) when a quality comes
+        // within 16 ulps of q_min
+        bool near = false;
+        const double qk = quality_of_mean<EXACT>(pp[0], pp[1], pp[2], J, q_min, &near);
+        if (near) return 2;
         qs[k * stride] = qk;
         valid = qk > q_min;
       }
@@ -360,28 +325,24 @@ __device__ __forceinline__ int move_vertex_generic(
   return valid;
 }
 
-// The move on separate xy / metric / log arrays (batch kernel).
-__device__ __forceinline__ int move_vertex(const int i, double2* __restrict__ xy,
-                                           double* __restrict__ tensor,
-                                           double4* __restrict__ logm,
-                                           double* __restrict__ qualit,
+// The move of vertex i with every quality as the reference computes it
+// (glibc's pow), through the generic loop; out of line, so that its
+// registers do not weigh on the fast path.  Called only for a move the fast
+// or generic path left undecided (a quality within 16 ulps of q_min).
+static __device__ __noinline__ int move_exact_rec(const int32_t i, const double2 pini,
+                                           VRecord* __restrict__ rec, double* __restrict__ qualit,
                                            const int32_t* __restrict__ v2t_off,
                                            const int32_t* __restrict__ v2t,
                                            const int32_t* __restrict__ star,
                                            const int32_t* __restrict__ v2v_off,
                                            const int32_t* __restrict__ v2v, double* lds,
                                            const int stride) {
-  const SoaVerts v{xy, tensor, logm};
+  const RecVerts v{rec};
+  const double T[3] = {rec[i].t[0], rec[i].t[1], rec[i].t[2]};
   const int32_t t0 = v2t_off[i], deg = v2t_off[i + 1] - t0;
   const int32_t n0 = v2v_off[i], nb = v2v_off[i + 1] - n0;
-  const double2 pini = xy[i];
-  double T[3];
-  v.metric(i, T);
-  if (deg <= kVtopoDeg && nb <= kVtopoDeg)
-    return move_vertex_fast(i, fast_topo_csr(i, t0, deg, n0, nb, v2t, star, v2v), pini, T, v,
-                            qualit, lds, stride);
-  return move_vertex_generic(i, t0, deg, n0, nb, pini, T, v, qualit, v2t, star, v2v, lds,
-                             stride);
+  return move_vertex_generic<true>(i, t0, deg, n0, nb, pini, T, v, qualit, v2t, star, v2v,
+                                   lds, stride);
 }
 
 // The move on vertex records (per-colour sweeps).  Boundary vertices (tag
@@ -420,7 +381,8 @@ __device__ __forceinline__ int move_vertex_rec(const int32_t i, VRecord* __restr
     if ((topo.meta & 15) == 0) return 0;
     *task = 1;
     const FastTopo ft = fast_topo_rec(i, topo);
-    const int c = move_vertex_fast(i, ft, pini, T, v, qualit, lds, stride);
+    int c = move_vertex_fast(i, ft, pini, T, v, qualit, lds, stride);
+    if (c == 2) c = move_exact_rec(i, pini, rec, qualit, v2t_off, v2t, star, v2v_off, v2v, lds, stride);
     if (c) {
 #pragma unroll
       for (int k = 0; k < kVtopoDeg; ++k)
@@ -434,8 +396,9 @@ __device__ __forceinline__ int move_vertex_rec(const int32_t i, VRecord* __restr
   const int32_t n0 = v2v_off[i], nb = v2v_off[i + 1] - n0;
   if (deg == 0) return 0;
   *task = 1;
-  const int c = move_vertex_generic(i, t0, deg, n0, nb, pini, T, v, qualit, v2t, star, v2v,
-                                    lds, stride);
+  int c = move_vertex_generic<false>(i, t0, deg, n0, nb, pini, T, v, qualit, v2t, star, v2v,
+                                     lds, stride);
+  if (c == 2) c = move_exact_rec(i, pini, rec, qualit, v2t_off, v2t, star, v2v_off, v2v, lds, stride);
   if (c) {
     for (int k = 0; k < nb; ++k) stale[v2v[n0 + k]] = 0;
   } else {
