@@ -131,20 +131,39 @@ int trn_normalize_global(trn_ctx* ctx, int32_t nv, double* tensor_d,
                          double scale_fact, double h_min, double h_max);
 
 /* ---- colouring: Partit::extractColoring ------------------------------------
- * Replaces Partit::extractColoring (src/partition.cpp:79-197): a seeded,
- * deterministic speculative-greedy colouring with conflict repair.  Every
- * vertex with a non-empty 1-ring gets a colour in 1..max_colors such that no
- * edge joins two equal colours (the reference uses 1..max_part-1 with
- * Partit(mesh, 8), i.e. at most 7).  Returns TRN_ERR_PALETTE if a vertex
- * cannot be coloured within max_colors (<= 64).  Also emits the colour
- * classes (Partit::task.subset / cardin / nb.parts): order_d [nv] lists the
- * coloured vertices grouped by colour (within a colour in the mesh's
- * locality order, else ascending);
- * class_off (host, max_colors+2 entries) receives the offsets of colours
- * 1..ncolors in order_d.  Synchronises. */
+ * Replaces Partit::extractColoring (src/partition.cpp:79-197) with a
+ * deterministic colouring of the 1-ring graph: a sequential first fit in
+ * index order inside tiles of consecutive vertex ids (the reference's
+ * one-thread colouring wherever an edge stays inside a tile), then the
+ * conflicts on edges between tiles removed and repaired by seeded
+ * speculative-greedy rounds.  Every vertex with a non-empty 1-ring gets a
+ * colour in 1..max_colors such that no edge joins two equal colours (the
+ * reference uses 1..max_part-1 with Partit(mesh, 8), i.e. at most 7).
+ * Returns TRN_ERR_PALETTE if a vertex cannot be coloured within max_colors
+ * (<= 64); see trn_color_partial for the reference's behaviour instead.
+ * Also emits the colour classes (Partit::task.subset / cardin / nb.parts):
+ * order_d [nv] lists the coloured vertices grouped by colour (within a
+ * colour in the mesh's locality order, else ascending); class_off (host,
+ * max_colors+2 entries) receives the offsets of colours 1..ncolors in
+ * order_d.  The result is a pure function of (mesh, seed, tile size).
+ * Synchronises. */
 int trn_color(trn_ctx* ctx, const trn_mesh* mesh, uint64_t seed,
               int32_t max_colors, int32_t* color_d, int32_t* order_d,
               int32_t* class_off, int32_t* ncolors);
+/* trn_color with the reference's palette behaviour (partition.cpp:104-110):
+ * a vertex whose neighbours already hold every colour 1..max_colors stays
+ * uncoloured (colour 0, in no class, so Smooth never moves it,
+ * smoothing.cpp:170) instead of failing the call; *uncoloured (may be
+ * NULL) receives the number of such vertices.  Synchronises. */
+int trn_color_partial(trn_ctx* ctx, const trn_mesh* mesh, uint64_t seed,
+                      int32_t max_colors, int32_t* color_d, int32_t* order_d,
+                      int32_t* class_off, int32_t* ncolors, int64_t* uncoloured);
+/* Tile size (vertices) of the colouring's first step on this mesh: 0
+ * chooses it from the longest edge of the numbering (the default: just
+ * longer, so that no edge joins two tiles of the same parity), otherwise a
+ * multiple of 64 in 64..16384.  A tile shorter than the longest edge leaves
+ * conflicts to the repair rounds; a longer one runs longer. */
+int trn_mesh_set_color_tile(trn_mesh* mesh, int32_t tile);
 /* Colour classes of an externally supplied colouring (colour 0 = not
  * coloured), e.g. the reference's own.  class_off as above.  mesh may be
  * NULL (classes in index order) or the mesh whose locality order to use.

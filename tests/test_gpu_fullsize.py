@@ -95,6 +95,9 @@ def test_full_size_pass_matches_oracle(ctx, wl, p, sweeps):
 
     col = T().color(ctx, dm, seed=1, max_colors=64)
     assert T().color_check(ctx, dm, col.color, col.ncolors) == (0, 0)
+    print(f"colours: {col.ncolors}", flush=True)
+    if wl.startswith("jitter"):
+        assert col.ncolors <= 6  # the tile first fit on a row-numbered grid
     cnp = col.color.cpu().numpy()
     xo, to, qo, co, tao = oracle.smooth_run(m.xy, m.tri, m.tags, st["metric"], cnp,
                                             col.ncolors, sweeps, topo=st["topo"])

@@ -193,10 +193,12 @@ def test_colouring_valid_deterministic_seeded(ctx, wl):
     topo = oracle.Topology(m.nv, m.tri)
     a = T().color(ctx, dm, seed=3, max_colors=7)
     b = T().color(ctx, dm, seed=3, max_colors=7)
+    T().set_color_tile(dm, 256)  # more seams, so the seeded repair has work
     c = T().color(ctx, dm, seed=4, max_colors=7)
-    ca, cb, cc = a.color.cpu().numpy(), b.color.cpu().numpy(), c.color.cpu().numpy()
+    d = T().color(ctx, dm, seed=4, max_colors=7)
+    ca, cb = a.color.cpu().numpy(), b.color.cpu().numpy()
     assert np.array_equal(ca, cb)
-    assert not np.array_equal(ca, cc)
+    assert np.array_equal(c.color.cpu().numpy(), d.color.cpu().numpy())
     for col in (a, c):
         assert T().color_check(ctx, dm, col.color, col.ncolors) == (0, 0)
         assert oracle.color_valid(topo, col.color.cpu().numpy(), col.ncolors)
@@ -210,6 +212,70 @@ def test_colouring_valid_deterministic_seeded(ctx, wl):
         for k in range(col.ncolors):
             seg = order[off[k]:off[k + 1]]
             assert (cv[seg] == k + 1).all() and (np.diff(seg) > 0).all()
+
+
+@pytest.mark.parametrize("wl", ["jitter128:fronts", "jitter300:layer", "flip200:fronts",
+                                "renumbered:jitter120:fronts"])
+def test_colouring_tiles_few_colours(ctx, wl):
+    """The tile-sequential first fit (index order inside tiles, seams
+    repaired) colours row-numbered grids like the reference's one-thread
+    colouring inside each tile (4 colours on a triangulated grid; the seams
+    between tiles allow up to 6), and stays
+    valid on randomly flipped diagonals and on randomly numbered meshes
+    (where almost every edge is a seam)."""
+    if wl.startswith("renumbered:"):
+        xy, tri, tags, _ = _renumbered(wl.split(":", 1)[1])
+        nv = xy.shape[0]
+    else:
+        m = mg.workload(wl)
+        nv, tri, tags = m.nv, m.tri, m.tags
+    dm = T().DeviceMesh(ctx, nv, tri, tags)
+    topo = oracle.Topology(nv, tri)
+    for tile in (0, 1024, 4096):
+        T().set_color_tile(dm, tile)
+        col = T().color(ctx, dm, seed=1, max_colors=64)
+        assert T().color_check(ctx, dm, col.color, col.ncolors) == (0, 0)
+        assert oracle.color_valid(topo, col.color.cpu().numpy(), col.ncolors)
+        if wl.startswith("jitter"):
+            assert col.ncolors <= 6, (tile, col.ncolors)
+    dm.close()
+
+
+def test_colour_partial_leaves_vertices_uncoloured(ctx):
+    """With a palette too small for the 60-spoke hub, the reference leaves a
+    vertex uncoloured (partition.cpp:104-110) and never smooths it
+    (smoothing.cpp:170): trn_color_partial returns a proper partial
+    colouring, every uncoloured vertex out of the classes and counted, and
+    with a palette large enough the same call colours everything."""
+    import torch
+    xy, tri, tags, _ = _wheel(60)
+    nv = xy.shape[0]
+    dm = T().DeviceMesh(ctx, nv, tri, tags)
+    col, unc = T().color_partial(ctx, dm, seed=1, max_colors=3)
+    c = col.color.cpu().numpy()
+    assert unc >= 1 and (c == 0).sum() == unc
+    assert c.min() >= 0 and c.max() <= 3
+    for a, b in ((0, 1), (1, 2), (2, 0)):
+        ca, cb = c[tri[:, a]], c[tri[:, b]]
+        assert not ((ca == cb) & (ca > 0)).any()
+    order = col.order.cpu().numpy()
+    assert col.class_off[-1] == nv - unc
+    assert (c[order[:col.class_off[-1]]] > 0).all()
+    with pytest.raises(T().TrinityError):
+        T().color(ctx, dm, seed=1, max_colors=3)
+    full, unc2 = T().color_partial(ctx, dm, seed=1, max_colors=7)
+    assert unc2 == 0
+    ref = T().color(ctx, dm, seed=1, max_colors=7)
+    assert np.array_equal(full.color.cpu().numpy(), ref.color.cpu().numpy())
+    # the sweeps leave the uncoloured vertex where it is
+    metric = oracle.metric_stages(xy, tri, _wheel(60)[3], 0.8, 2)["metric"]
+    xg = torch.from_numpy(xy.copy()).cuda()
+    tg = torch.from_numpy(metric.copy()).cuda()
+    q = torch.empty(tri.shape[0], dtype=torch.float64, device="cuda")
+    T().smooth_run(ctx, dm, xg, tg, q, col, 2)
+    moved = (xg.cpu().numpy() != xy).any(axis=1)
+    assert not moved[c == 0].any()
+    dm.close()
 
 
 def test_locality_order_is_a_permutation_of_each_class(ctx):

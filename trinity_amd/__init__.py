@@ -37,7 +37,8 @@ EXPORTS = [
     "trn_ctx_reset_timing", "trn_ctx_timing_count", "trn_ctx_timing_get",
     "trn_mesh_create", "trn_mesh_destroy", "trn_mesh_info", "trn_mesh_csr",
     "trn_metric_run", "trn_gradient", "trn_hessian", "trn_normalize_local",
-    "trn_complexity", "trn_normalize_global", "trn_color", "trn_color_classes",
+    "trn_complexity", "trn_normalize_global", "trn_color", "trn_color_partial",
+    "trn_mesh_set_color_tile", "trn_color_classes",
     "trn_color_check", "trn_indep_set", "trn_quality_cache", "trn_quality_stats", "trn_refine_filter",
     "trn_smooth_run",
     "trn_smooth_prepare",
@@ -100,6 +101,9 @@ def _load():
         "trn_normalize_global": ([_vp, _i32, _vp, _f64, _f64, _f64], ctypes.c_int),
         "trn_color": ([_vp, _vp, ctypes.c_uint64, _i32, _vp, _vp, _P(_i32), _P(_i32)],
                       ctypes.c_int),
+        "trn_color_partial": ([_vp, _vp, ctypes.c_uint64, _i32, _vp, _vp, _P(_i32), _P(_i32),
+                               _P(_i64)], ctypes.c_int),
+        "trn_mesh_set_color_tile": ([_vp, _i32], ctypes.c_int),
         "trn_color_classes": ([_vp, _vp, _i32, _vp, _i32, _vp, _P(_i32)], ctypes.c_int),
         "trn_color_check": ([_vp, _vp, _vp, _i32, _P(_i64), _P(_i64)], ctypes.c_int),
         "trn_quality_cache": ([_vp, _vp, _vp, _vp, _vp], ctypes.c_int),
@@ -378,6 +382,28 @@ def color(ctx, mesh, seed=1, max_colors=64) -> Coloring:
     _check(lib().trn_color(ctx.handle, mesh.handle, int(seed) & ((1 << 64) - 1), int(max_colors),
                            _vp(col.data_ptr()), _vp(order.data_ptr()), off, ctypes.byref(nc)))
     return Coloring(col, order, np.array(off[:nc.value + 1], dtype=np.int32), nc.value)
+
+
+def color_partial(ctx, mesh, seed=1, max_colors=7) -> tuple[Coloring, int]:
+    """Colouring with the reference's palette behaviour: vertices that find
+    every colour 1..max_colors taken stay uncoloured (0, in no class).
+    Returns (coloring, number of uncoloured vertices)."""
+    torch = _torch()
+    dev = f"cuda:{ctx.device}"
+    col = torch.empty(mesh.nv, dtype=torch.int32, device=dev)
+    order = torch.empty(mesh.nv, dtype=torch.int32, device=dev)
+    off = (ctypes.c_int32 * 80)()
+    nc = _i32()
+    unc = _i64()
+    _check(lib().trn_color_partial(ctx.handle, mesh.handle, int(seed) & ((1 << 64) - 1),
+                                   int(max_colors), _vp(col.data_ptr()), _vp(order.data_ptr()),
+                                   off, ctypes.byref(nc), ctypes.byref(unc)))
+    return Coloring(col, order, np.array(off[:nc.value + 1], dtype=np.int32), nc.value), unc.value
+
+
+def set_color_tile(mesh, tile: int) -> None:
+    """Tile size of the colouring's first step (0 = chosen from nv)."""
+    _check(lib().trn_mesh_set_color_tile(mesh.handle, int(tile)))
 
 
 def color_classes(ctx, color_t, ncolors, mesh=None) -> Coloring:

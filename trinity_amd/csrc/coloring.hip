@@ -3,21 +3,37 @@
 //
 // The reference runs a speculative first-fit in parallel (:94-111), detects
 // conflicts (:113-122) and recolours conflicting vertices until none remain
-// (:126-168); with several threads its result depends on the interleaving.
-// Here each round is two bulk-synchronous kernels, which makes the result a
-// pure function of (mesh, seed):
-//   tentative: every uncoloured vertex takes the smallest colour not held by
-//              an already coloured neighbour (64-bit forbidden mask);
-//   resolve:   an uncoloured vertex keeps its tentative colour unless an
-//              uncoloured neighbour with a higher seeded priority chose the
-//              same colour; winners commit, losers retry next round.
-// Two adjacent winners of one round hold different tentative colours and
-// every tentative colour avoids all committed neighbours, so the colouring
-// is proper after every round; the top-priority vertex of every conflict
-// group commits, so each round makes progress.  In the first round every
-// tentative colour is 1, so that round is the resolve kernel alone.  A
-// round reads the previous round's loser count itself and returns at once
-// when it is zero, so a batch of rounds is queued without host checks.
+// (:126-168); with one thread it is the sequential first fit in index order
+// (4 colours on a triangulated grid), with several its result depends on
+// the interleaving.  Here the result is a pure function of (mesh, seed,
+// tile size), in three steps:
+//   tiles:    the vertices are cut into tiles of consecutive ids; one wave
+//             per tile runs the sequential first fit over its tile in
+//             index order, in LDS, seeing only neighbours inside the tile
+//             (k_color_tiles).  On a mesh numbered with locality most edges
+//             stay inside a tile, so this is the reference's one-thread
+//             colouring almost everywhere;
+//   seams:    an edge between two tiles whose ends got the same colour is
+//             a conflict; the end in the later tile is uncoloured
+//             (k_color_seams, k_color_drop), which leaves a proper partial
+//             colouring;
+//   repair:   the uncoloured vertices are coloured in speculative rounds,
+//             each two bulk-synchronous kernels:
+//               tentative: every uncoloured vertex takes the smallest colour
+//                          not held by a coloured neighbour (64-bit mask);
+//               resolve:   it keeps it unless an uncoloured neighbour with a
+//                          higher seeded priority chose the same colour;
+//                          winners commit, losers retry next round.
+//             Two adjacent winners of one round hold different tentative
+//             colours and every tentative colour avoids all committed
+//             neighbours, so the colouring stays proper; the top-priority
+//             vertex of every conflict group commits, so each round makes
+//             progress.  A round reads the previous round's loser count
+//             itself and returns at once when it is zero, so a batch of
+//             rounds is queued without host checks.
+// A vertex whose neighbours hold every colour 1..max_colors is either an
+// error (trn_color) or, as in the reference (partition.cpp:104-110), left
+// uncoloured and out of every class (trn_color_partial).
 //
 // The colour classes (Partit::task.subset/cardin/nb.parts, :171-194) are a
 // stable counting sort: per-block colour histograms, one exclusive scan in
@@ -34,7 +50,8 @@
 
 namespace trn {
 
-constexpr int kClassB = 1024;  // vertices per block of the class build
+constexpr int kClassB = 1024;  // threads per block of the class build
+constexpr int kClassPer = 4;   // consecutive kClassB-vertex chunks per block
 constexpr int kScanB = 1024;   // threads of the single-block scan
 constexpr int kMaxRows = 64;   // colours a class build handles
 
@@ -65,11 +82,13 @@ __device__ __forceinline__ int prev_losers(const int32_t* __restrict__ prev_slot
 
 // Rounds are queued in batches; a round whose predecessor left no loser
 // returns at once, so over-queued rounds cost a launch each.
+// leave != 0: a vertex that finds every colour taken gives up (colour -1,
+// which no neighbour counts as a colour) instead of raising the failure flag.
 __global__ __launch_bounds__(256) void k_color_tent(int32_t nv, const int32_t* __restrict__ v2v_off,
                                                     const int32_t* __restrict__ v2v,
-                                                    const int32_t* __restrict__ color,
+                                                    int32_t* __restrict__ color,
                                                     int32_t* __restrict__ tent, int max_colors,
-                                                    int32_t* fail,
+                                                    int leave, int32_t* fail,
                                                     const int32_t* __restrict__ prev_slots) {
   if (prev_losers(prev_slots) == 0) return;
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -87,7 +106,8 @@ __global__ __launch_bounds__(256) void k_color_tent(int32_t nv, const int32_t* _
   const uint64_t freebits = ~used;
   const int c = freebits ? __ffsll((unsigned long long)freebits) : 65;
   if (c > max_colors) {
-    atomicOr(fail, 1);
+    if (leave) color[v] = -1;
+    else atomicOr(fail, 1);
     tent[v] = 0;
     return;
   }
@@ -160,56 +180,224 @@ __global__ void k_color_check(int32_t nv, const int32_t* __restrict__ v2v_off,
   }
 }
 
-// Per-block histogram of colours 1..ncolors, stored colour-major.
-// Position k of the visiting sequence is vertex perm[k] (or k without perm).
+// ---- tiles and seams -------------------------------------------------------
+
+constexpr int kTileB = 64;                    // one wave per tile
+constexpr int kTileMax = 16384;               // largest tile (local ids fit 16 bits)
+
+// LDS of a tile block: the tile's colours, one byte per vertex.
+static size_t tile_shmem(int tile) { return ((size_t)tile + 15) & ~size_t(15); }
+
+__device__ __forceinline__ uint64_t lane_u64(const uint32_t lo, const uint32_t hi, const int k) {
+  return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)hi, k) << 32) |
+         (uint32_t)__builtin_amdgcn_readlane((int)lo, k);
+}
+
+// Sequential first fit in index order over tile b = 2 * blockIdx.x + parity
+// ([t0, t0 + tile) of the vertex ids).  Even tiles run first and see only
+// their own earlier vertices; odd tiles run second and also avoid the
+// colours of their neighbours in even tiles, which are final by then.
+// Per 64-vertex chunk every lane gathers, for its vertex, the mask of
+// colours its earlier neighbours outside the chunk hold (in-tile ones from
+// the tile's colours in LDS, even-tile ones from memory) and the mask of
+// its earlier neighbours inside the chunk; then the wave walks the chunk
+// in order on the scalar unit: vertex k takes the smallest colour absent
+// from its outside mask and from the colours of its in-chunk neighbours,
+// read from their lanes (readlane), and is recorded in lane k.
+// Isolated vertices stay 0.  A vertex that finds colours 1..max_colors
+// taken is written -1 and counted in *given_up.  Edges between tiles of
+// the same parity (only where the numbering has edges longer than a tile)
+// are left to the seam check.
+__global__ __launch_bounds__(kTileB) void k_color_tiles(int32_t nv, int32_t tile, int parity,
+                                                        const int32_t* __restrict__ v2v_off,
+                                                        const int32_t* __restrict__ v2v,
+                                                        int max_colors,
+                                                        int32_t* __restrict__ color,
+                                                        int32_t* __restrict__ given_up) {
+  extern __shared__ uint8_t col[];  // [tile] colours of the tile (0: none, 0xFF: given up)
+  const int64_t tb = 2 * (int64_t)blockIdx.x + parity;
+  const int64_t t0 = tb * tile;
+  if (t0 >= nv) return;
+  const int64_t t1 = t0 + tile < nv ? t0 + tile : nv;
+  const int lane = threadIdx.x;
+  int nfail = 0;
+  for (int64_t c0 = t0; c0 < t1; c0 += kTileB) {
+    const int64_t v = c0 + lane;
+    int32_t b = 0, e = 0;
+    if (v < t1) {
+      b = v2v_off[v];
+      e = v2v_off[v + 1];
+    }
+    uint64_t near = 0, far = 0;
+    for (int32_t a = b; a < e; ++a) {
+      const int32_t u = v2v[a];
+      if (u >= c0 && u < v) {
+        near |= 1ull << (u - c0);
+      } else if (u >= t0 && u < c0) {
+        const int cu = col[u - t0];
+        if (cu != 0 && cu != 0xFF) far |= 1ull << (cu - 1);
+      } else if (parity && (u < t0 || u >= t1) && ((u / tile) & 1) == 0) {
+        const int32_t cu = color[u];  // an even tile, finished
+        if (cu > 0) far |= 1ull << (cu - 1);
+      }
+    }
+    const uint64_t iso = __ballot(v < t1 && b == e);
+    const uint32_t near_lo = (uint32_t)near, near_hi = (uint32_t)(near >> 32);
+    const uint32_t far_lo = (uint32_t)far, far_hi = (uint32_t)(far >> 32);
+    uint32_t oh_lo = 0, oh_hi = 0;  // one-hot colour of this lane's vertex
+    int mine = 0;
+    const int n = (int)(t1 - c0 < kTileB ? t1 - c0 : kTileB);
+    for (int k = 0; k < n; ++k) {  // uniform: the scalar unit walks the chunk
+      int c = 0;
+      if (!((iso >> k) & 1)) {
+        uint64_t used = lane_u64(far_lo, far_hi, k);
+        uint64_t nb = lane_u64(near_lo, near_hi, k);
+        while (nb) {
+          const int j = __builtin_ctzll(nb);
+          nb &= nb - 1;
+          used |= lane_u64(oh_lo, oh_hi, j);
+        }
+        c = __builtin_ffsll((long long)~used);
+        if (c == 0 || c > max_colors) {
+          c = 0xFF;
+          ++nfail;
+        }
+      }
+      const uint64_t o = (c != 0 && c != 0xFF) ? 1ull << (c - 1) : 0ull;
+      if (lane == k) {  // record vertex k's colour in its lane
+        oh_lo = (uint32_t)o;
+        oh_hi = (uint32_t)(o >> 32);
+        mine = c;
+      }
+    }
+    if (v < t1) col[v - t0] = (uint8_t)mine;
+    __syncthreads();
+  }
+  for (int64_t k = lane; k < t1 - t0; k += kTileB) {
+    const int c = col[k];
+    color[t0 + k] = c == 0xFF ? -1 : c;
+  }
+  if (lane == 0 && nfail) atomicAdd(given_up, nfail);
+}
+
+// An edge between two tiles whose ends hold the same colour: the end in the
+// later tile loses (lose[v] = 1).  Reads colours only, so the losers do not
+// depend on timing.
+__global__ __launch_bounds__(256) void k_color_seams(int32_t nv, int32_t tile,
+                                                     const int32_t* __restrict__ v2v_off,
+                                                     const int32_t* __restrict__ v2v,
+                                                     const int32_t* __restrict__ color,
+                                                     int32_t* __restrict__ lose) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  const int32_t cv = color[v];
+  int l = 0;
+  if (cv > 0) {
+    const int32_t tv = (int32_t)(v / tile);
+    for (int32_t a = v2v_off[v]; a < v2v_off[v + 1]; ++a) {
+      const int32_t u = v2v[a];
+      if (u / tile < tv && color[u] == cv) {
+        l = 1;
+        break;
+      }
+    }
+  }
+  lose[v] = l;
+}
+
+// Uncolours the losers and counts them into the round slots the first
+// repair round reads.
+__global__ __launch_bounds__(256) void k_color_drop(int32_t nv, const int32_t* __restrict__ lose,
+                                                    int32_t* __restrict__ color,
+                                                    int32_t* __restrict__ slots) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int l = v < nv ? lose[v] : 0;
+  if (l) color[v] = 0;
+  const int n = __syncthreads_count(l);
+  if (threadIdx.x == 0 && n) atomicAdd(&slots[blockIdx.x & (kRoundSlots - 1)], n);
+}
+
+__global__ void k_copy_slots(const int32_t* __restrict__ src, int32_t* __restrict__ dst) {
+  if (threadIdx.x < kRoundSlots) dst[threadIdx.x] = src[threadIdx.x];
+}
+
+// Vertices that gave up (-1) back to 0 (uncoloured, in no class), counted.
+__global__ __launch_bounds__(256) void k_color_unset(int32_t nv, int32_t* __restrict__ color,
+                                                     int32_t* __restrict__ count) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = v < nv && color[v] < 0;
+  if (g) color[v] = 0;
+  const int n = __syncthreads_count(g);
+  if (threadIdx.x == 0 && n) atomicAdd(count, n);
+}
+
+// Per-block histogram of colours 1..ncolors, stored colour-major; a block
+// covers kClassPer consecutive chunks of kClassB positions.  Position k of
+// the visiting sequence is vertex perm[k] (or k without perm).
 __global__ __launch_bounds__(kClassB) void k_class_count(int32_t nv, const int32_t* __restrict__ color,
                                                          const int32_t* __restrict__ perm,
                                                          int ncolors, int32_t* __restrict__ cnt) {
   __shared__ int hist[64];
   if (threadIdx.x < 64) hist[threadIdx.x] = 0;
   __syncthreads();
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < nv) {
-    const int c = color[perm ? perm[k] : k];
-    if (c >= 1 && c <= ncolors) atomicAdd(&hist[c - 1], 1);
+  for (int s = 0; s < kClassPer; ++s) {
+    const int64_t k = ((int64_t)blockIdx.x * kClassPer + s) * kClassB + threadIdx.x;
+    if (k < nv) {
+      const int c = color[perm ? perm[k] : k];
+      if (c >= 1 && c <= ncolors) atomicAdd(&hist[c - 1], 1);
+    }
   }
   __syncthreads();
   if ((int)threadIdx.x < ncolors) cnt[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = hist[threadIdx.x];
 }
 
+// Scatter of the block's vertices into their classes, chunk by chunk, so
+// that each class keeps the visiting order.
 __global__ __launch_bounds__(kClassB) void k_class_scatter(int32_t nv, const int32_t* __restrict__ color,
                                                            const int32_t* __restrict__ perm,
                                                            int ncolors,
                                                            const int32_t* __restrict__ off,
                                                            int32_t* __restrict__ order) {
   __shared__ int wcnt[kClassB / 64][64];
+  __shared__ int run[64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int k = threadIdx.x; k < (kClassB / 64) * 64; k += blockDim.x) (&wcnt[0][0])[k] = 0;
-  __syncthreads();
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t v = (k < nv && perm) ? perm[k] : k;
-  int c = 0;
-  if (k < nv) {
-    c = color[v];
-    if (c < 1 || c > ncolors) c = 0;
-  }
-  // rank among equal colours inside the wave (lanes in ascending v order)
-  uint64_t todo = __ballot(1);
-  int rank = 0;
+  if (threadIdx.x < 64) run[threadIdx.x] = 0;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  while (todo) {
-    const int leader = __ffsll((unsigned long long)todo) - 1;
-    const int lc = __shfl(c, leader, 64);
-    const uint64_t m = __ballot(c == lc);
-    if (c == lc) rank = __popcll(m & lt);
-    if (lane == leader && lc > 0) wcnt[wid][lc - 1] = __popcll(m);
-    todo &= ~m;
-  }
-  __syncthreads();
-  if (c > 0) {
-    int base = off[(int64_t)(c - 1) * gridDim.x + blockIdx.x];
-    for (int w = 0; w < wid; ++w) base += wcnt[w][c - 1];
-    order[base + rank] = (int32_t)v;
+  for (int s = 0; s < kClassPer; ++s) {
+    for (int k = threadIdx.x; k < (kClassB / 64) * 64; k += blockDim.x) (&wcnt[0][0])[k] = 0;
+    __syncthreads();
+    const int64_t k = ((int64_t)blockIdx.x * kClassPer + s) * kClassB + threadIdx.x;
+    const int64_t v = (k < nv && perm) ? perm[k] : k;
+    int c = 0;
+    if (k < nv) {
+      c = color[v];
+      if (c < 1 || c > ncolors) c = 0;
+    }
+    // rank among equal colours inside the wave (lanes in visiting order)
+    uint64_t todo = __ballot(1);
+    int rank = 0;
+    while (todo) {
+      const int leader = __ffsll((unsigned long long)todo) - 1;
+      const int lc = __shfl(c, leader, 64);
+      const uint64_t m = __ballot(c == lc);
+      if (c == lc) rank = __popcll(m & lt);
+      if (lane == leader && lc > 0) wcnt[wid][lc - 1] = __popcll(m);
+      todo &= ~m;
+    }
+    __syncthreads();
+    if (c > 0) {
+      int base = off[(int64_t)(c - 1) * gridDim.x + blockIdx.x] + run[c - 1];
+      for (int w = 0; w < wid; ++w) base += wcnt[w][c - 1];
+      if ((uint32_t)(base + rank) < (uint32_t)nv) order[base + rank] = (int32_t)v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      int t = 0;
+      for (int w = 0; w < kClassB / 64; ++w) t += wcnt[w][threadIdx.x];
+      run[threadIdx.x] += t;
+    }
+    // the counts are read above before the next chunk clears them
+    __syncthreads();
   }
 }
 
@@ -371,13 +559,13 @@ struct ClassBufs {
 };
 
 static size_t class_bytes(int32_t nv, int rows) {
-  const int64_t n = (int64_t)nblk(nv, kClassB) * rows;
+  const int64_t n = (int64_t)nblk(nv, kClassB * kClassPer) * rows;
   return 2 * scratch_bytes(sizeof(int32_t) * (size_t)(n + 1)) +
          scratch_bytes(sizeof(int32_t) * kStatusWords);
 }
 
 static ClassBufs carve_classes(Scratch& sc, int32_t nv, int rows) {
-  const int64_t n = (int64_t)nblk(nv, kClassB) * rows;
+  const int64_t n = (int64_t)nblk(nv, kClassB * kClassPer) * rows;
   ClassBufs b;
   b.cnt = sc.carve<int32_t>((size_t)n + 1);
   b.off = sc.carve<int32_t>((size_t)n + 1);
@@ -392,7 +580,7 @@ static ClassBufs carve_classes(Scratch& sc, int32_t nv, int rows) {
 static void queue_classes(trn_ctx* ctx, int32_t nv, const int32_t* color, const int32_t* perm,
                           int rows, const int32_t* fail, const int32_t* last_slots,
                           int32_t* order, const ClassBufs& b) {
-  const int64_t nb = nblk(nv, kClassB);
+  const int64_t nb = nblk(nv, kClassB * kClassPer);
   if (nb > 0)
     launch(ctx, "class_count", 4.0 * nv, k_class_count, dim3((unsigned)nb), dim3(kClassB), 0, nv,
            color, perm, rows, b.cnt);
@@ -410,57 +598,133 @@ using namespace trn;
 
 extern "C" {
 
+}  // extern "C"
+
+namespace trn {
+
+// Tile size of the colouring's first step for a mesh: large enough that a
+// tile of a mesh numbered row by row holds a few rows (few seam conflicts),
+// small enough for one tile per wave to fill the chip.
+static int color_tile_size(const trn_mesh* m) {
+  if (m->color_tile > 0) return m->color_tile;
+  // no edge may join two tiles of the same parity: the tile must exceed the
+  // longest edge of the numbering (rounded up to whole 64-vertex chunks);
+  // past kTileMax the seams are repaired instead
+  if (m->bandwidth >= kTileMax) return 4096;
+  const int t = (m->bandwidth + 1 + kTileB - 1) / kTileB * kTileB;
+  return t < 1024 ? 1024 : t;
+}
+
+static void color_impl(trn_ctx* ctx, const trn_mesh* mesh, uint64_t seed, int32_t max_colors,
+                       bool leave, int32_t* color_d, int32_t* order_d, int32_t* class_off,
+                       int32_t* ncolors, int64_t* uncoloured) {
+  TRN_REQUIRE(mesh != nullptr && color_d && order_d && class_off && ncolors, TRN_ERR_ARG,
+              "null argument");
+  TRN_REQUIRE(max_colors >= 1 && max_colors <= kMaxRows, TRN_ERR_ARG,
+              "max_colors must be in 1..64");
+  const int32_t nv = mesh->nv;
+  // first fit gives a vertex at most (1-ring size + 1) colours
+  const int rows = max_colors < mesh->max_deg_v + 1 ? max_colors : mesh->max_deg_v + 1;
+  const size_t nslots = (size_t)(kRoundsPerSync + 1) * kRoundSlots;
+  Scratch sc(ctx, scratch_bytes(sizeof(int32_t) * (size_t)nv) +
+                      scratch_bytes(sizeof(int32_t) * (16 + nslots)) + class_bytes(nv, rows));
+  int32_t* tent = sc.carve<int32_t>(nv);
+  // [0] failure (palette exhausted, trn_color), [1] vertices left
+  // uncoloured (trn_color_partial), [2] tile-step give-ups, then the slots:
+  // kRoundSlots for the seams, then per queued round
+  int32_t* flags = sc.carve<int32_t>(16 + nslots);
+  int32_t* seam_slots = flags + 16;
+  int32_t* slots = seam_slots + kRoundSlots;
+  const ClassBufs cb = carve_classes(sc, nv, rows);
+  memset_async(ctx, flags, 0, sizeof(int32_t) * (16 + nslots));
+  const int tile = color_tile_size(mesh);
+  const int64_t ntiles = (nv + (int64_t)tile - 1) / tile;
+  for (int parity = 0; parity < 2; ++parity) {
+    const int64_t nb = (ntiles + 1 - parity) / 2;
+    if (nb > 0)
+      launch(ctx, "color_tiles", 4.0 * nv + 2.0 * (double)mesh->nnz_v2v, k_color_tiles,
+             dim3((unsigned)nb), dim3(kTileB), tile_shmem(tile), nv, tile, parity,
+             (const int32_t*)mesh->v2v_off, (const int32_t*)mesh->v2v, (int)max_colors,
+             color_d, leave ? flags + 2 : flags);
+  }
+  launch(ctx, "color_seams", 8.0 * nv + 4.0 * (double)mesh->nnz_v2v, k_color_seams,
+         dim3(nblk(nv, 256)), dim3(256), 0, nv, tile, (const int32_t*)mesh->v2v_off,
+         (const int32_t*)mesh->v2v, (const int32_t*)color_d, tent);
+  launch(ctx, "color_drop", 8.0 * nv, k_color_drop, dim3(nblk(nv, 256)), dim3(256), 0, nv,
+         (const int32_t*)tent, color_d, seam_slots);
+  const int32_t* h = ctx->pinned_i32;
+  // the class build right away: with a tile longer than every edge there is
+  // no seam conflict (the usual case), and one synchronisation returns the
+  // loser count, the number of colours and the class offsets
+  const int32_t* prev = seam_slots;
+  int rounds = 0;
+  for (;;) {
+    queue_classes(ctx, nv, color_d, mesh->perm, rows, flags, prev, order_d, cb);
+    sync(ctx);
+    TRN_REQUIRE(h[0] == 0, TRN_ERR_PALETTE,
+                "a vertex needs more than " + std::to_string(max_colors) + " colours");
+    if (h[1] == 0) break;
+    TRN_REQUIRE(rounds < 100000, TRN_ERR_STATE, "colouring did not converge");
+    // repair: a batch of speculative rounds, the first reading the losers
+    // left so far (the seams' or the previous batch's last round's)
+    if (prev != seam_slots)
+      launch(ctx, "color_carry", 0.0, k_copy_slots, dim3(1), dim3(kRoundSlots), 0, prev,
+             seam_slots);
+    prev = seam_slots;
+    memset_async(ctx, slots, 0, sizeof(int32_t) * kRoundsPerSync * kRoundSlots);
+    for (int r = 0; r < kRoundsPerSync; ++r) {
+      int32_t* cur = slots + (size_t)r * kRoundSlots;
+      launch(ctx, "color_tent", 4.0 * nv + 4.0 * (double)mesh->nnz_v2v, k_color_tent,
+             dim3(nblk(nv, 256)), dim3(256), 0, nv, (const int32_t*)mesh->v2v_off,
+             (const int32_t*)mesh->v2v, color_d, tent, (int)max_colors, (int)leave, flags, prev);
+      launch(ctx, "color_resolve", 4.0 * nv + 4.0 * (double)mesh->nnz_v2v, k_color_resolve,
+             dim3(nblk(nv, 256)), dim3(256), 0, nv, (const int32_t*)mesh->v2v_off,
+             (const int32_t*)mesh->v2v, color_d, (const int32_t*)tent, seed, 0, cur, prev);
+      prev = cur;
+    }
+    rounds += kRoundsPerSync;
+  }
+  const int nc = h[2];
+  for (int k = 0; k <= nc; ++k) class_off[k] = h[3 + k];
+  *ncolors = nc;
+  if (leave) {  // vertices that gave up (-1, in no class) become 0, counted
+    launch(ctx, "color_unset", 4.0 * nv, k_color_unset, dim3(nblk(nv, 256)), dim3(256), 0, nv,
+           color_d, flags + 1);
+    copy_async(ctx, ctx->pinned_i32, flags + 1, sizeof(int32_t));
+    sync(ctx);
+    if (uncoloured) *uncoloured = ctx->pinned_i32[0];
+  }
+}
+
+}  // namespace trn
+
+extern "C" {
+
 int trn_color(trn_ctx* ctx, const trn_mesh* mesh, uint64_t seed, int32_t max_colors,
               int32_t* color_d, int32_t* order_d, int32_t* class_off, int32_t* ncolors) {
   return guard_impl("trn_color", [&] {
     DeviceScope ds(ctx);
-    TRN_REQUIRE(mesh != nullptr && color_d && order_d && class_off && ncolors, TRN_ERR_ARG,
-                "null argument");
-    TRN_REQUIRE(max_colors >= 1 && max_colors <= kMaxRows, TRN_ERR_ARG,
-                "max_colors must be in 1..64");
-    const int32_t nv = mesh->nv;
-    // first fit gives a vertex at most (1-ring size + 1) colours
-    const int rows = max_colors < mesh->max_deg_v + 1 ? max_colors : mesh->max_deg_v + 1;
-    const size_t nslots = (size_t)kRoundsPerSync * kRoundSlots;
-    Scratch sc(ctx, scratch_bytes(sizeof(int32_t) * (size_t)nv) +
-                        scratch_bytes(sizeof(int32_t) * (16 + nslots)) + class_bytes(nv, rows));
-    int32_t* tent = sc.carve<int32_t>(nv);
-    int32_t* flags = sc.carve<int32_t>(16 + nslots);  // [0] palette failure, then the slots
-    int32_t* slots = flags + 16;
-    const ClassBufs cb = carve_classes(sc, nv, rows);
-    memset_async(ctx, flags, 0, sizeof(int32_t) * (16 + nslots));
-    const int32_t* h = ctx->pinned_i32;
-    int rounds = 0;
-    for (;;) {
-      if (rounds) memset_async(ctx, slots, 0, sizeof(int32_t) * nslots);
-      for (int r = 0; r < kRoundsPerSync; ++r) {
-        const int32_t* prev = r ? slots + (size_t)(r - 1) * kRoundSlots : nullptr;
-        const int first = rounds == 0 && r == 0;
-        if (!first)
-          launch(ctx, "color_tent", 4.0 * nv + 4.0 * (double)mesh->nnz_v2v, k_color_tent,
-                 dim3(nblk(nv, 256)), dim3(256), 0, nv, (const int32_t*)mesh->v2v_off,
-                 (const int32_t*)mesh->v2v, (const int32_t*)color_d, tent, (int)max_colors,
-                 flags, prev);
-        launch(ctx, "color_resolve", 4.0 * nv + 4.0 * (double)mesh->nnz_v2v, k_color_resolve,
-               dim3(nblk(nv, 256)), dim3(256), 0, nv, (const int32_t*)mesh->v2v_off,
-               (const int32_t*)mesh->v2v, color_d, (const int32_t*)tent, seed, first,
-               slots + (size_t)r * kRoundSlots, prev);
-      }
-      // the class build is queued with the batch: when the batch has
-      // coloured every vertex (the usual case) one synchronisation returns
-      // the convergence check, the number of colours and the class offsets
-      queue_classes(ctx, nv, color_d, mesh->perm, rows, flags,
-                    slots + (size_t)(kRoundsPerSync - 1) * kRoundSlots, order_d, cb);
-      sync(ctx);
-      rounds += kRoundsPerSync;
-      TRN_REQUIRE(h[0] == 0, TRN_ERR_PALETTE,
-                  "a vertex needs more than " + std::to_string(max_colors) + " colours");
-      if (h[1] == 0) break;
-      TRN_REQUIRE(rounds < 100000, TRN_ERR_STATE, "colouring did not converge");
-    }
-    const int nc = h[2];
-    for (int k = 0; k <= nc; ++k) class_off[k] = h[3 + k];
-    *ncolors = nc;
+    color_impl(ctx, mesh, seed, max_colors, false, color_d, order_d, class_off, ncolors,
+               nullptr);
+  });
+}
+
+int trn_color_partial(trn_ctx* ctx, const trn_mesh* mesh, uint64_t seed, int32_t max_colors,
+                      int32_t* color_d, int32_t* order_d, int32_t* class_off, int32_t* ncolors,
+                      int64_t* uncoloured) {
+  return guard_impl("trn_color_partial", [&] {
+    DeviceScope ds(ctx);
+    color_impl(ctx, mesh, seed, max_colors, true, color_d, order_d, class_off, ncolors,
+               uncoloured);
+  });
+}
+
+int trn_mesh_set_color_tile(trn_mesh* mesh, int32_t tile) {
+  return guard_impl("trn_mesh_set_color_tile", [&] {
+    TRN_REQUIRE(mesh != nullptr, TRN_ERR_ARG, "null mesh");
+    TRN_REQUIRE(tile == 0 || (tile >= 64 && tile <= kTileMax && tile % kTileB == 0),
+                TRN_ERR_ARG, "tile must be 0 (automatic) or a multiple of 64 in 64..16384");
+    mesh->color_tile = tile;
   });
 }
 

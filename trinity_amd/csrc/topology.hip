@@ -158,6 +158,20 @@ __global__ void k_validate_tri(int32_t nv, int32_t nt, const int32_t* __restrict
     atomicAdd(bad, 1);
 }
 
+// Longest edge at each vertex, |u - v| over its 1-ring.
+__global__ void k_vertex_span(int32_t nv, const int32_t* __restrict__ v2v_off,
+                              const int32_t* __restrict__ v2v, int32_t* __restrict__ span) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  int32_t s = 0;
+  for (int32_t a = v2v_off[v]; a < v2v_off[v + 1]; ++a) {
+    const int32_t d = v2v[a] - (int32_t)v;
+    const int32_t ad = d < 0 ? -d : d;
+    s = ad > s ? ad : s;
+  }
+  span[v] = s;
+}
+
 __global__ void k_count_v2t(int32_t nt, const int32_t* __restrict__ tri, int32_t* deg) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nt) return;
@@ -644,6 +658,15 @@ int trn_mesh_create(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
       launch(ctx, "topo_fill_v2v", 0.0, k_v2v<1>, dim3(blocks_for(nv, 64)), dim3(64), shv, nv,
              (const int32_t*)m->tri, (const int32_t*)m->v2t_off, (const int32_t*)m->v2t, capv,
              (int32_t*)nullptr, (const int32_t*)m->v2v_off, m->v2v, flags + 2);
+
+      // 3b. bandwidth of the numbering (longest edge |u - v|)
+      launch(ctx, "topo_span", 8.0 * nv + 4.0 * (double)m->nnz_v2v, k_vertex_span,
+             dim3(blocks_for(nv, 256)), dim3(256), 0, nv, (const int32_t*)m->v2v_off,
+             (const int32_t*)m->v2v, cnt);
+      reduce_max_i32(ctx, cnt, nv, flags + 7);
+      copy_async(ctx, h, flags + 7, sizeof(int32_t));
+      sync(ctx);
+      m->bandwidth = h[0];
 
       // 4. largest 2-ring element patch
       launch(ctx, "topo_patch_bound", 0.0, k_patch_bound, dim3(blocks_for(nv, 256)), dim3(256),
