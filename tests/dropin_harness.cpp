@@ -9,6 +9,8 @@
  * is mapped to `public` in this translation unit only.
  *
  * Usage: dropin_harness MESH.mesh SOLU.bb OUTDIR tf p hmin hmax sweeps
+ *                       [max_part (8)] [verbosity (0)]
+ * Partit(mesh, max_part) gives the colouring a palette of max(6,max_part)-1.
  */
 #include <stack>
 #include <set>
@@ -66,12 +68,14 @@ int main(int argc, char** argv) {
   const int p = std::atoi(argv[5]);
   const double hmin = std::atof(argv[6]), hmax = std::atof(argv[7]);
   const int sweeps = std::atoi(argv[8]);
+  const int max_part = argc > 9 ? std::atoi(argv[9]) : 8;
+  const int verbosity = argc > 10 ? std::atoi(argv[10]) : 0;
 
   std::ifstream file(argv[1], std::ios::in);
   int size[] = {io::find("Vertices", file), io::find("Triangles", file)};
   file.close();
 
-  Mesh mesh(size, 64, 2, 0, 1);
+  Mesh mesh(size, 64, 2, verbosity, 1);
   mesh.load(argv[1], argv[2]);
   Stats stat[2];
   Metrics metric(&mesh, tf, p, hmin, hmax);
@@ -80,7 +84,7 @@ int main(int argc, char** argv) {
   dump(out + "/metric.bin", mesh.geom.tensor.data(), sizeof(double) * 3 * nv);
   dump(out + "/complexity.bin", &metric.field.complexity, sizeof(double));
 
-  Partit partit(&mesh, 8);
+  Partit partit(&mesh, max_part);
   Smooth smooth(&mesh, &partit, sweeps);
   smooth.run(stat + 1);
   std::vector<int32_t> color(nv, 0);
@@ -90,7 +94,11 @@ int main(int argc, char** argv) {
   dump(out + "/xy.bin", mesh.geom.points.data(), sizeof(double) * 2 * nv);
   dump(out + "/tensor.bin", mesh.geom.tensor.data(), sizeof(double) * 3 * nv);
   dump(out + "/qualit.bin", mesh.geom.qualit.data(), sizeof(double) * nt);
-  const int32_t counts[4] = {stat[0].eval, stat[0].task, stat[1].eval, stat[1].task};
+  // Metrics and Smooth counters, Smooth's four step times, the progress
+  // counter and the number of colour classes
+  const int32_t counts[10] = {stat[0].eval,    stat[0].task,    stat[1].eval,    stat[1].task,
+                              stat[1].step[0], stat[1].step[1], stat[1].step[2], stat[1].step[3],
+                              mesh.param.iter, partit.nb.parts};
   dump(out + "/stats.bin", counts, sizeof(counts));
   return 0;
 }

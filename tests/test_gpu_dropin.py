@@ -33,15 +33,35 @@ def _grid4(tmp_path):
     return m, mp, bp
 
 
+# the progress line Metrics::recap / Smooth::recap print at verbosity 0 with
+# one remeshing round (metric.cpp:249-251,281-286; smoothing.cpp:243-246)
+PROGRESS_V0 = "\n\r= Remeshing  ...   0 % =\r= Remeshing  ...  20 % =\r= Remeshing  ...  40 % ="
+
+
+def _harness(mp, bp, out, sweeps, max_part=8, verbosity=0, tf="1.0", p="2"):
+    r = subprocess.run([HARNESS, mp, bp, str(out), tf, p, "1e-9", "0.4", str(sweeps),
+                        str(max_part), str(verbosity)],
+                       timeout=120, capture_output=True,
+                       env=dict(os.environ, OMP_NUM_THREADS="4"))
+    err = r.stderr.decode("latin-1")
+    assert r.returncode == 0 and "trinity_amd:" not in err, err[-2000:]
+    return r.stdout.decode("latin-1")   # keeps the carriage returns
+
+
+def _stats_ok(stats, nv, nc):
+    assert stats[0] == nv and stats[1] == nv            # Metrics::recap
+    assert (stats[4:8] >= 0).all()                      # Stats::step[0..3] in ms
+    assert stats[8] == 2                                 # mesh->param.iter
+    assert stats[9] == nc                                # Partit nb.parts
+
+
 @pytest.mark.skipif(not os.path.exists(HARNESS), reason="drop-in harness not built")
 def test_dropin_classes_match_reference(tmp_path):
     m, mp, bp = _grid4(tmp_path)
     out = tmp_path / "out"
     out.mkdir()
     sweeps = 3
-    subprocess.run([HARNESS, mp, bp, str(out), "1.0", "2", "1e-9", "0.4", str(sweeps)],
-                   check=True, timeout=120, stdout=subprocess.DEVNULL,
-                   env=dict(os.environ, OMP_NUM_THREADS="4"))
+    assert _harness(mp, bp, out, sweeps).endswith(PROGRESS_V0)   # after Mesh::load's lines
 
     def rd(name, dt, shape):
         return np.fromfile(str(out / name), dtype=dt).reshape(shape)
@@ -68,9 +88,76 @@ def test_dropin_classes_match_reference(tmp_path):
     assert normrel(xy, xo).max() <= 1e-12
     assert normrel(rd("tensor.bin", "<f8", (nv, 3)), to).max() <= 1e-10
     assert normrel(rd("qualit.bin", "<f8", (nt,)), qo).max() <= 1e-10
-    stats = rd("stats.bin", "<i4", (4,))
-    assert stats[0] == nv and stats[1] == nv            # Metrics::recap
+    stats = rd("stats.bin", "<i4", (10,))
+    _stats_ok(stats, nv, nc)
     assert stats[3] == int(np.sum(co))                   # Smooth: commits
+
+
+def _high_degree_mesh(n=40):
+    """Jittered grid with random edge flips (vertex degrees 4..12), numbered
+    smallest degree first: a sequential first fit needs 7+ colours on it."""
+    m = mg.raise_degrees(mg.grid_mesh(n, 0.25, 1234), 12, rounds=60)
+    deg = mg.vertex_degrees(m.tri, m.nv)
+    m = mg.renumber(m, np.argsort(deg, kind="stable"))
+    m.solut, _ = mg.field_fronts(m.xy, seed=9)
+    return m, deg.max()
+
+
+@pytest.mark.skipif(not os.path.exists(HARNESS), reason="drop-in harness not built")
+def test_dropin_palette_overflow_leaves_vertices_unmoved(tmp_path):
+    """Partit(mesh, 6): a palette of 5 colours on a mesh of degree up to 12.
+    As in partition.cpp:104-110 the vertices that find no free colour stay at
+    colour 0, in no class, and are never moved (smoothing.cpp:170); the run
+    carries on.  Everything else equals the compiled reference fed the same
+    colouring, zeros included."""
+    m, dmax = _high_degree_mesh()
+    assert dmax >= 8
+    mp, bp = str(tmp_path / "h.mesh"), str(tmp_path / "h.bb")
+    mg.write_medit(mp, bp, m)
+    out = tmp_path / "out"
+    out.mkdir()
+    sweeps = 2
+    assert _harness(mp, bp, out, sweeps, max_part=6).endswith(PROGRESS_V0)
+
+    def rd(name, dt, shape):
+        return np.fromfile(str(out / name), dtype=dt).reshape(shape)
+    nv, nt = m.nv, m.nt
+    color = rd("color.bin", "<i4", (nv,))
+    st = oracle.metric_stages(m.xy, m.tri, m.solut, 1.0, 2, 1e-9, 0.4)
+    topo = st["topo"]
+    off, adj = topo.v2v_off, topo.v2v
+    src = np.repeat(np.arange(nv), np.diff(off))
+    both = (color[src] > 0) & (color[adj] > 0)
+    assert not (color[src] == color[adj])[both].any()   # valid where coloured
+    assert color.max() <= 5
+    blank = np.flatnonzero((color == 0) & (np.diff(off) > 0))
+    assert blank.size > 0                                # the palette overflowed
+    xy = rd("xy.bin", "<f8", (nv, 2))
+    assert np.array_equal(xy[blank], m.xy[blank])        # never moved
+    stats = rd("stats.bin", "<i4", (10,))
+    _stats_ok(stats, nv, int(color.max()))
+    if oracle.ref_available():
+        ref = oracle.ref_dump(m.xy, m.tri, m.tags, m.solut, 1.0, 2, 1e-9, 0.4, sweeps,
+                              color=color)
+        assert normrel(xy, ref["xy"]).max() <= 1e-12
+        assert normrel(rd("tensor.bin", "<f8", (nv, 3)), ref["tensor"]).max() <= 1e-10
+        assert normrel(rd("qualit.bin", "<f8", (nt,)), ref["qualit"]).max() <= 1e-10
+        assert stats[3] == int(np.sum(ref["commits"]))
+
+
+@pytest.mark.skipif(not os.path.exists(HARNESS), reason="drop-in harness not built")
+def test_dropin_verbose2_reports_steps(tmp_path):
+    """Verbosity 2: the reference's per-sweep lines and step summary
+    (smoothing.cpp:212-226,253-268), and the metric summary
+    (metric.cpp:287-302)."""
+    m, mp, bp = _grid4(tmp_path)
+    out = tmp_path / "out"
+    out.mkdir()
+    text = _harness(mp, bp, out, 3, verbosity=2)
+    assert "Computing multi-scale metric field ... \n= norm L^2\n= target : " in text
+    assert text.count("\n= round ") == 3
+    for step in ("primal", "color ", "qualit", "kernel"):
+        assert "%% %s \x1b[32m(" % step in text
 
 
 CPU_CLI = os.path.join(oracle.HERE, "_ref", "trinity_cpu")
@@ -100,3 +187,38 @@ def test_reference_main_runs_with_dropin(tmp_path):
                            capture_output=True, text=True, env=env)
         assert "Exporting" in c.stdout
         assert r.returncode == c.returncode
+
+
+def _strip_build_line(text):
+    # the banner names each binary's compile time
+    return "\n".join(l for l in text.splitlines() if "compiled with" not in l)
+
+
+@pytest.mark.skipif(not (os.path.exists(CLI) and os.path.exists(CPU_CLI)),
+                    reason="drop-in or reference CLI not built")
+def test_reference_main_high_degree_verbose0(tmp_path):
+    """src/main.cpp at verbosity 0 on GRID4 with a few random edge flips
+    (vertex degrees up to 8): the drop-in CLI prints the reference CLI's
+    progress lines and ends with its return code.  On every such mesh tried
+    (degree 8 and above) the reference's own refinement (src/refinement.cpp,
+    kept unchanged in both binaries) ends the run right after the metric
+    pass; the palette overflow of the smoothing is covered by
+    test_dropin_palette_overflow_leaves_vertices_unmoved."""
+    g = load_golden("grid4_shock4")
+    m = mg.Mesh(xy=g["xy_in"], tri=g["tri"], tags=g["tags"], solut=g["solut"])
+    m = mg.raise_degrees(m, 12, rounds=2, min_area=0.3)
+    assert mg.vertex_degrees(m.tri, m.nv).max() >= 8
+    mp, bp = str(tmp_path / "h.mesh"), str(tmp_path / "h.bb")
+    mg.write_medit(mp, bp, m)
+    args = ["-i", mp, "-s", bp, "-r", "1", "-d", "2", "-c", "4", "-v", "0"]
+    env = dict(os.environ, OMP_NUM_THREADS="4")
+    runs = []
+    for exe, name in ((CLI, "gpu"), (CPU_CLI, "cpu")):
+        r = subprocess.run([exe, *args, "-o", str(tmp_path / (name + ".mesh"))], timeout=300,
+                           capture_output=True, env=env)
+        runs.append((r.returncode, r.stdout.decode("latin-1"), r.stderr.decode("latin-1")))
+    (rc_g, out_g, err_g), (rc_c, out_c, _) = runs
+    assert "trinity_amd:" not in err_g, err_g[-2000:]
+    assert "= Remeshing  ...  20 % =" in out_g
+    assert _strip_build_line(out_g) == _strip_build_line(out_c)
+    assert rc_g == rc_c

@@ -1,6 +1,7 @@
-// trn_dropin.cpp -- device context and topology handles of the drop-in.
+// trn_dropin.cpp -- device context and the shared device mesh of the drop-in.
 #include "trn_dropin.hpp"
 
+#include <cstring>
 #include <mutex>
 
 namespace trinity_amd_dropin {
@@ -23,13 +24,28 @@ void check(int rc, const char* what) {
   }
 }
 
-DeviceTopology::DeviceTopology(int32_t nv, int32_t nt, const int32_t* tri, const uint8_t* tags) {
-  // tri and tags may be host pointers: the library copies them
-  check(trn_mesh_create(context(), nv, nt, tri, tags, nullptr, &m_), "trn_mesh_create");
+DeviceMesh& DeviceMesh::get() {
+  static DeviceMesh* self = new DeviceMesh();
+  return *self;
 }
 
-DeviceTopology::~DeviceTopology() {
-  if (m_) trn_mesh_destroy(m_);
+trn_mesh* DeviceMesh::topology(int32_t nv, int32_t nt, const int32_t* elems,
+                               const uint8_t* tags, bool* built) {
+  const size_t ne = 3 * (size_t)nt;
+  const bool same = mesh_ != nullptr && nv == nv_ && nt == nt_ &&
+                    std::memcmp(elems_.data(), elems, ne * sizeof(int32_t)) == 0 &&
+                    std::memcmp(tags_.data(), tags, (size_t)nv) == 0;
+  if (built) *built = !same;
+  if (same) return mesh_;
+  if (mesh_) trn_mesh_destroy(mesh_);
+  mesh_ = nullptr;
+  // elems and tags are host pointers: the library copies them
+  check(trn_mesh_create(context(), nv, nt, elems, tags, nullptr, &mesh_), "trn_mesh_create");
+  nv_ = nv;
+  nt_ = nt;
+  elems_.assign(elems, elems + ne);
+  tags_.assign(tags, tags + nv);
+  return mesh_;
 }
 
 }  // namespace trinity_amd_dropin

@@ -23,45 +23,70 @@ trn_ctx* context();
 // so a failing device call ends the process with the library's message.
 void check(int rc, const char* what);
 
-// Device buffer owned for the duration of one pass.
+// Device array that grows on demand and is kept between passes.
 template <typename T>
-class DeviceArray {
+class DeviceBuffer {
  public:
-  explicit DeviceArray(size_t n) : n_(n) {
-    if (n_ && hipMalloc(reinterpret_cast<void**>(&p_), n_ * sizeof(T)) != hipSuccess) {
-      std::fprintf(stderr, "trinity_amd: hipMalloc of %zu bytes failed\n", n_ * sizeof(T));
-      std::exit(EXIT_FAILURE);
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+
+  // At least n elements; the contents are not kept when it grows.
+  T* reserve(size_t n) {
+    if (n > cap_) {
+      if (p_) (void)hipFree(p_);
+      p_ = nullptr;
+      cap_ = 0;
+      if (hipMalloc(reinterpret_cast<void**>(&p_), n * sizeof(T)) != hipSuccess) {
+        std::fprintf(stderr, "trinity_amd: hipMalloc of %zu bytes failed\n", n * sizeof(T));
+        std::exit(EXIT_FAILURE);
+      }
+      cap_ = n;
     }
+    return p_;
   }
-  DeviceArray(const DeviceArray&) = delete;
-  DeviceArray& operator=(const DeviceArray&) = delete;
-  ~DeviceArray() {
-    if (p_) (void)hipFree(p_);
+  T* upload(const T* host, size_t n) {
+    reserve(n);
+    if (n) check(hipMemcpy(p_, host, n * sizeof(T), hipMemcpyHostToDevice) == hipSuccess ? 0 : -2,
+                 "hipMemcpy host->device");
+    return p_;
+  }
+  void download(T* host, size_t n) const {
+    if (n) check(hipMemcpy(host, p_, n * sizeof(T), hipMemcpyDeviceToHost) == hipSuccess ? 0 : -2,
+                 "hipMemcpy device->host");
   }
   T* get() const { return p_; }
-  void upload(const T* host) {
-    if (n_) check(hipMemcpy(p_, host, n_ * sizeof(T), hipMemcpyHostToDevice) == hipSuccess ? 0 : -2,
-                  "hipMemcpy host->device");
-  }
-  void download(T* host) const {
-    if (n_) check(hipMemcpy(host, p_, n_ * sizeof(T), hipMemcpyDeviceToHost) == hipSuccess ? 0 : -2,
-                  "hipMemcpy device->host");
-  }
 
  private:
   T* p_ = nullptr;
-  size_t n_ = 0;
+  size_t cap_ = 0;
 };
 
-// RAII handle of a device topology built from the mesh's element array.
-class DeviceTopology {
+// Device state shared by Metrics::run and Smooth::run of one process: the
+// topology built from the mesh's element array and tags, rebuilt only when
+// those changed since the previous pass (in the reference's loop a
+// remeshing round changes them between two smoothings; Metrics::run and the
+// next Smooth::run on an unchanged mesh share one build), and the vertex and
+// element arrays of the passes.  Created on first use and kept until exit
+// (never destroyed: the HIP runtime may be gone when static objects die).
+class DeviceMesh {
  public:
-  DeviceTopology(int32_t nv, int32_t nt, const int32_t* tri, const uint8_t* tags);
-  ~DeviceTopology();
-  trn_mesh* get() const { return m_; }
+  static DeviceMesh& get();
+
+  // Device topology of (elems [3nt], tags [nv]); *built (may be NULL) tells
+  // whether this call rebuilt it.
+  trn_mesh* topology(int32_t nv, int32_t nt, const int32_t* elems, const uint8_t* tags,
+                     bool* built = nullptr);
+
+  DeviceBuffer<double> xy, tensor, solut, qualit;
+  DeviceBuffer<int32_t> color, order;
 
  private:
-  trn_mesh* m_ = nullptr;
+  DeviceMesh() = default;
+  trn_mesh* mesh_ = nullptr;
+  int32_t nv_ = -1, nt_ = -1;
+  std::vector<int32_t> elems_;
+  std::vector<uint8_t> tags_;
 };
 
 }  // namespace trinity_amd_dropin

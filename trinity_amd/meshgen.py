@@ -94,6 +94,95 @@ def grid_mesh(n: int, jitter: float = 0.0, seed: int = 1234, flip: float = 0.0) 
                 meta={"n": n, "jitter": jitter, "seed": seed, "flip": flip})
 
 
+def _interior_edges(tri: np.ndarray, nv: int):
+    """Edges shared by two triangles: (a, b, c, d, t1, t2) with t1 = (a, b, c)
+    and t2 = (b, a, d) up to rotation, both counter-clockwise."""
+    t = np.asarray(tri, dtype=np.int64)
+    nt = t.shape[0]
+    a = np.concatenate([t[:, 0], t[:, 1], t[:, 2]])
+    b = np.concatenate([t[:, 1], t[:, 2], t[:, 0]])
+    c = np.concatenate([t[:, 2], t[:, 0], t[:, 1]])
+    tid = np.tile(np.arange(nt), 3)
+    key = np.minimum(a, b) * nv + np.maximum(a, b)
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    pair = np.flatnonzero(ks[1:] == ks[:-1])
+    h1, h2 = order[pair], order[pair + 1]
+    return a[h1], b[h1], c[h1], c[h2], tid[h1], tid[h2], np.unique(ks)
+
+
+def raise_degrees(m: Mesh, max_deg: int = 12, rounds: int = 40, seed: int = 5,
+                  min_area: float = 0.12) -> Mesh:
+    """Random edge flips that spread the vertex degrees of a triangulation up
+    to max_deg, the way anisotropic adaptation leaves them.  Each round flips
+    a vertex-disjoint set of interior edges (a, b) -> (c, d), favouring flips
+    onto vertices of high degree, where the quad a, d, b, c is convex, both
+    new triangles keep at least min_area of the pair's mean area, (c, d) is
+    not an edge yet, deg(c), deg(d) stay <= max_deg and deg(a), deg(b) >= 4.
+    Positions and tags are kept; the triangles stay counter-clockwise."""
+    rng = np.random.default_rng(seed)
+    tri = np.array(m.tri, dtype=np.int32, copy=True)
+    nv = m.nv
+    x, y = m.xy[:, 0], m.xy[:, 1]
+
+    def orient(p, q, r):
+        return (x[q] - x[p]) * (y[r] - y[p]) - (y[q] - y[p]) * (x[r] - x[p])
+
+    for _ in range(rounds):
+        a, b, c, d, t1, t2, keys = _interior_edges(tri, nv)
+        ends = np.concatenate([a, b])
+        deg = np.bincount(ends, minlength=nv) + 0  # every interior edge once
+        bnd_e = boundary_edges(tri)
+        if bnd_e:
+            be = np.asarray(bnd_e, dtype=np.int64).ravel()
+            deg += np.bincount(be, minlength=nv)
+        ok = (deg[c] < max_deg) & (deg[d] < max_deg) & (deg[a] > 4) & (deg[b] > 4)
+        o1, o2 = orient(c, a, d), orient(c, d, b)
+        mean = 0.5 * (orient(a, b, c) + orient(b, a, d))
+        ok &= (o1 > min_area * mean) & (o2 > min_area * mean)
+        new_key = np.minimum(c, d) * nv + np.maximum(c, d)
+        pos = np.searchsorted(keys, new_key)
+        pos = np.minimum(pos, keys.size - 1)
+        ok &= keys[pos] != new_key
+        cand = np.flatnonzero(ok)
+        if cand.size == 0:
+            break
+        pr = rng.random(cand.size) + 0.05 * (deg[c[cand]] + deg[d[cand]])
+        best = np.full(nv, -1.0)
+        for v in (a, b, c, d):
+            np.maximum.at(best, v[cand], pr)
+        win = np.ones(cand.size, dtype=bool)
+        for v in (a, b, c, d):
+            win &= best[v[cand]] == pr
+        e = cand[win]
+        tri[t1[e]] = np.stack([c[e], a[e], d[e]], axis=1)
+        tri[t2[e]] = np.stack([c[e], d[e], b[e]], axis=1)
+    out = Mesh(xy=m.xy.copy(), tri=tri, tags=m.tags.copy(),
+               solut=None if m.solut is None else m.solut.copy(), meta=dict(m.meta))
+    out.meta["max_deg"] = max_deg
+    return out
+
+
+def vertex_degrees(tri: np.ndarray, nv: int) -> np.ndarray:
+    """Number of distinct neighbours of every vertex."""
+    t = np.asarray(tri, dtype=np.int64)
+    a = np.concatenate([t[:, 0], t[:, 1], t[:, 2]])
+    b = np.concatenate([t[:, 1], t[:, 2], t[:, 0]])
+    key = np.unique(np.minimum(a, b) * nv + np.maximum(a, b))
+    return np.bincount(np.concatenate([key // nv, key % nv]), minlength=nv)
+
+
+def renumber(m: Mesh, perm: np.ndarray) -> Mesh:
+    """The same mesh with vertex perm[k] renamed k."""
+    perm = np.asarray(perm, dtype=np.int64)
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(perm.size)
+    return Mesh(xy=np.ascontiguousarray(m.xy[perm]), tri=inv[m.tri].astype(np.int32),
+                tags=m.tags[perm].copy(),
+                solut=None if m.solut is None else np.ascontiguousarray(m.solut[perm]),
+                meta=dict(m.meta))
+
+
 # --------------------------------------------------------------------------
 # nodal fields
 
