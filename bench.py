@@ -179,7 +179,10 @@ def run_plain(T, mg, ctx, workload, p, sweeps, steps, warmup, seed, color_seed,
     import torch
     m = mg.workload(workload, seed=seed)
     xy0 = torch.from_numpy(m.xy).cuda()
-    dm = T.DeviceMesh(ctx, m.nv, m.tri, m.tags, xy=xy0 if locality else None)
+    # coordinates at creation: the 2-ring patches are stored in Z-curve slabs
+    dm = T.DeviceMesh(ctx, m.nv, m.tri, m.tags, xy=xy0)
+    if locality:
+        T.set_class_order(dm, True)
     f = torch.from_numpy(m.solut).cuda()
     xy = torch.empty_like(xy0)
     tensor = torch.empty((m.nv, 3), dtype=torch.float64, device="cuda")
@@ -195,12 +198,12 @@ def run_plain(T, mg, ctx, workload, p, sweeps, steps, warmup, seed, color_seed,
         # and tags (CSR, stored patches and unions, star codes, records)
         tri_d = torch.from_numpy(m.tri.reshape(-1)).cuda()
         tags_d = torch.from_numpy(m.tags).cuda()
-        T.DeviceMesh(ctx, m.nv, tri_d, tags_d).close()
+        T.DeviceMesh(ctx, m.nv, tri_d, tags_d, xy=xy0).close()
         times = []
         for _ in range(create_reps):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            d2 = T.DeviceMesh(ctx, m.nv, tri_d, tags_d)
+            d2 = T.DeviceMesh(ctx, m.nv, tri_d, tags_d, xy=xy0)
             torch.cuda.synchronize()
             times.append((time.perf_counter() - t0) * 1e3)
             d2.close()
@@ -358,7 +361,7 @@ def run_host_boundary(T, mg, ctx, workload, p, sweeps, reps=3, seed=1234, color_
         tri_d, tags_d = torch.from_numpy(tri).cuda(), torch.from_numpy(tags).cuda()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        dm = T.DeviceMesh(ctx, m.nv, tri_d, tags_d)
+        dm = T.DeviceMesh(ctx, m.nv, tri_d, tags_d, xy=xy)
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         tensor = torch.empty((m.nv, 3), dtype=torch.float64, device="cuda")
@@ -402,7 +405,7 @@ def run_refine(T, mg, ctx, workload, p, reps=5, seed=1234):
     import torch
     m = mg.workload(workload, seed=seed)
     xy = torch.from_numpy(m.xy).cuda()
-    dm = T.DeviceMesh(ctx, m.nv, m.tri, m.tags)
+    dm = T.DeviceMesh(ctx, m.nv, m.tri, m.tags, xy=xy)
     tensor, _ = T.metric_run(ctx, dm, xy, torch.from_numpy(m.solut).cuda(), 0.8, p, 1e-9, 0.4,
                              want_complexity=False)
     r = T.refine_filter(ctx, dm, xy, tensor)
@@ -459,9 +462,9 @@ def run_adjoint(T, mg, ctx, workload, steps, p=2):
     of the energy sum_t (1 - q_M(t))^2."""
     import torch
     m = mg.workload(workload)
-    dm = T.DeviceMesh(ctx, m.nv, m.tri, m.tags)
-    adj = T.Adjoint(ctx, dm)
     xy = torch.from_numpy(m.xy).cuda()
+    dm = T.DeviceMesh(ctx, m.nv, m.tri, m.tags, xy=xy)
+    adj = T.Adjoint(ctx, dm)
     f = torch.from_numpy(m.solut).cuda()
     gx = torch.empty_like(xy)
     gf = torch.empty_like(f)
@@ -496,7 +499,7 @@ def compare_smoothing(T, mg, ctx, workload, iters):
     import numpy as np
     import torch
     m = mg.workload(workload)
-    dm = T.DeviceMesh(ctx, m.nv, m.tri, m.tags)
+    dm = T.DeviceMesh(ctx, m.nv, m.tri, m.tags, xy=torch.from_numpy(m.xy).cuda())
     adj = T.Adjoint(ctx, dm)
     f = torch.from_numpy(m.solut).cuda()
     xy = torch.from_numpy(m.xy).cuda()
@@ -696,7 +699,8 @@ def main():
                     help="colour on the pass's stream after the metric instead of on a "
                          "second stream beside it")
     ap.add_argument("--locality-order", action="store_true",
-                    help="visit colour classes along a Z-curve of the coordinates")
+                    help="visit colour classes along a Z-curve of the coordinates "
+                         "(the patch slabs follow it in any case)")
     ap.add_argument("--cpu-workload", default=None,
                     help="mesh of the CPU baseline (default: the headline's)")
     ap.add_argument("--cpu-reps", type=int, default=1)

@@ -84,12 +84,19 @@ int trn_ctx_timing_get(trn_ctx* ctx, int32_t index, const char** name,
  * mesh.cpp:200) and tags (nv) may be host or device pointers; they are
  * copied into the mesh.  Also measures the largest 2-ring element patch
  * (Mesh::getVicinity(i,2), src/mesh.cpp:390-434) the recovery will gather.
- * xy_d (device, optional): coordinates used only to order colour classes
- * along a Z-curve for memory locality; NULL keeps index order.  Results do
- * not depend on it. */
+ * xy_d (device, optional): coordinates used only for memory locality: the
+ * stored 2-ring patches are grouped along a Z-curve of them (64 vertices
+ * that are close in the plane share their staged terms), and
+ * trn_mesh_set_class_order can visit colour classes in that order; NULL
+ * keeps index order.  Results do not depend on it. */
 int trn_mesh_create(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
                     const uint8_t* tags, const double* xy_d, trn_mesh** out);
 int trn_mesh_destroy(trn_mesh* mesh);
+/* Order in which the colour classes (trn_color, trn_color_classes) list
+ * their vertices: zcurve = 0 ascending (the default), 1 along the Z-curve of
+ * the coordinates given to trn_mesh_create (TRN_ERR_ARG if none were).  The
+ * classes hold the same vertices either way. */
+int trn_mesh_set_class_order(trn_mesh* mesh, int32_t zcurve);
 /* Sizes: nnz of v2v, max triangles per vertex, max 1-ring, max 2-ring patch. */
 int trn_mesh_info(const trn_mesh* mesh, int64_t* nnz_v2v, int32_t* max_deg_t,
                   int32_t* max_deg_v, int32_t* max_patch);

@@ -115,6 +115,27 @@ def test_metric_full_mesh_matches_oracle(ctx):
     assert normrel(r["metric"], st["metric"]).max() <= 1e-10
 
 
+@pytest.mark.parametrize("wl", ["jitter300:layer", "hub160:fronts"])
+def test_metric_with_zcurve_slabs_is_bit_identical(ctx, wl):
+    """A mesh created with coordinates stores its 2-ring patches in slabs of
+    Z-curve-consecutive vertices instead of index-consecutive ones: the
+    gathers then run over other slabs and unions, and every stage output is
+    bit-identical to the index-order slabs'."""
+    import torch
+    m = mg.workload(wl)
+    xy, f = dev(m.xy), dev(m.solut)
+    out = []
+    for coords in (None, m.xy):
+        dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags, xy=coords)
+        grad = torch.empty((m.nv, 2), dtype=torch.float64, device="cuda")
+        hess = torch.empty((m.nv, 3), dtype=torch.float64, device="cuda")
+        ten, c = T().metric_run(ctx, dm, xy, f, 0.8, 4, 1e-9, 0.4, gradient=grad, hessian=hess)
+        out.append((grad.cpu(), hess.cpu(), ten.cpu(), c))
+        dm.close()
+    (g0, h0, t0, c0), (g1, h1, t1, c1) = out
+    assert torch.equal(g0, g1) and torch.equal(h0, h1) and torch.equal(t0, t1) and c0 == c1
+
+
 def _wheel(n=60):
     """A hub joined to a rim of n vertices, and an outer ring of n boundary
     vertices joined to the rim: the hub has n incident triangles (generic
@@ -282,8 +303,13 @@ def test_locality_order_is_a_permutation_of_each_class(ctx):
     m = mg.workload("jitter90:fronts")
     plain = T().DeviceMesh(ctx, m.nv, m.tri, m.tags)
     local = T().DeviceMesh(ctx, m.nv, m.tri, m.tags, xy=m.xy)
+    with pytest.raises(T().TrinityError):
+        T().set_class_order(plain, True)          # no coordinates to order by
+    same = T().color(ctx, local, seed=5)          # ascending unless asked
+    T().set_class_order(local, True)
     a = T().color(ctx, plain, seed=5)
     b = T().color(ctx, local, seed=5)
+    assert np.array_equal(same.order.cpu().numpy(), a.order.cpu().numpy())
     assert np.array_equal(a.color.cpu().numpy(), b.color.cpu().numpy())
     assert np.array_equal(a.class_off, b.class_off)
     oa, ob = a.order.cpu().numpy(), b.order.cpu().numpy()
@@ -535,8 +561,10 @@ def test_quality_cache_matches_oracle(ctx):
 def test_smoothing_with_gpu_colouring_matches_oracle(ctx, wl, p, sweeps):
     m = mg.workload(wl)
     st = oracle.metric_stages(m.xy, m.tri, m.solut, 0.8, p)
-    # with coordinates: classes are visited in Z-curve order (same results)
+    # with coordinates: Z-curve patch slabs, classes visited in Z-curve
+    # order (same results)
     dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags, xy=m.xy)
+    T().set_class_order(dm, True)
     col = T().color(ctx, dm, seed=11, max_colors=8)
     cnp = col.color.cpu().numpy()
     xo, to, qo, co, tao = oracle.smooth_run(m.xy, m.tri, m.tags, st["metric"], cnp, col.ncolors,
@@ -603,3 +631,30 @@ def test_device_exp_pow_bit_identical_to_glibc_log_correctly_rounded(ctx):
     got = T().debug_math(ctx, 0, dev(sample)).cpu().numpy()
     want = np.array([float(mpmath.log(mpmath.mpf(float(v)))) for v in sample])
     assert bits_equal(got, want)
+
+
+@pytest.mark.parametrize("wl", ["hub256:fronts", "renumbered:hub160:fronts"])
+def test_colouring_degree12_palette13(ctx, wl):
+    """A vertex graph of degree up to 12 (random edge flips of a jittered
+    grid, as adaptation leaves it) with the palette 1..13 that Partit(mesh,
+    14) gives: valid, deterministic, within the palette, also when the
+    numbering is random."""
+    if wl.startswith("renumbered:"):
+        xy, tri, tags, _ = _renumbered(wl.split(":", 1)[1])
+        nv = xy.shape[0]
+    else:
+        m = mg.workload(wl)
+        nv, tri, tags = m.nv, m.tri, m.tags
+    assert mg.vertex_degrees(tri, nv).max() == 12
+    dm = T().DeviceMesh(ctx, nv, tri, tags)
+    topo = oracle.Topology(nv, tri)
+    for tile in (0, 256):
+        T().set_color_tile(dm, tile)
+        a = T().color(ctx, dm, seed=5, max_colors=13)
+        b = T().color(ctx, dm, seed=5, max_colors=13)
+        assert np.array_equal(a.color.cpu().numpy(), b.color.cpu().numpy())
+        assert T().color_check(ctx, dm, a.color, a.ncolors) == (0, 0)
+        assert oracle.color_valid(topo, a.color.cpu().numpy(), a.ncolors)
+        assert 1 <= a.ncolors <= 13
+        print(wl, "tile", tile, "colours", a.ncolors)
+    dm.close()

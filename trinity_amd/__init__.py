@@ -38,7 +38,7 @@ EXPORTS = [
     "trn_mesh_create", "trn_mesh_destroy", "trn_mesh_info", "trn_mesh_csr",
     "trn_metric_run", "trn_gradient", "trn_hessian", "trn_normalize_local",
     "trn_complexity", "trn_normalize_global", "trn_color", "trn_color_partial",
-    "trn_mesh_set_color_tile", "trn_color_classes",
+    "trn_mesh_set_color_tile", "trn_mesh_set_class_order", "trn_color_classes",
     "trn_color_check", "trn_indep_set", "trn_quality_cache", "trn_quality_stats", "trn_refine_filter",
     "trn_smooth_run",
     "trn_smooth_prepare",
@@ -104,6 +104,7 @@ def _load():
         "trn_color_partial": ([_vp, _vp, ctypes.c_uint64, _i32, _vp, _vp, _P(_i32), _P(_i32),
                                _P(_i64)], ctypes.c_int),
         "trn_mesh_set_color_tile": ([_vp, _i32], ctypes.c_int),
+        "trn_mesh_set_class_order": ([_vp, _i32], ctypes.c_int),
         "trn_color_classes": ([_vp, _vp, _i32, _vp, _i32, _vp, _P(_i32)], ctypes.c_int),
         "trn_color_check": ([_vp, _vp, _vp, _i32, _P(_i64), _P(_i64)], ctypes.c_int),
         "trn_quality_cache": ([_vp, _vp, _vp, _vp, _vp], ctypes.c_int),
@@ -404,6 +405,12 @@ def color_partial(ctx, mesh, seed=1, max_colors=7) -> tuple[Coloring, int]:
 def set_color_tile(mesh, tile: int) -> None:
     """Tile size of the colouring's first step (0 = chosen from nv)."""
     _check(lib().trn_mesh_set_color_tile(mesh.handle, int(tile)))
+
+
+def set_class_order(mesh, zcurve: bool) -> None:
+    """Colour classes in ascending order (False) or along the Z-curve of the
+    coordinates the mesh was created with (True)."""
+    _check(lib().trn_mesh_set_class_order(mesh.handle, int(bool(zcurve))))
 
 
 def color_classes(ctx, color_t, ncolors, mesh=None) -> Coloring:

@@ -58,6 +58,7 @@ __global__ __launch_bounds__(kBatchCplxB) void k_batch_cplx_part(
   const int64_t g = tile0[m + 1] - tile0[m];
   const int64_t t0 = toff[m], ntm = (int64_t)toff[m + 1] - t0;
   double phi = 0.;
+#pragma unroll 4
   for (int64_t t = (int64_t)b * kBatchCplxB + threadIdx.x; t < ntm; t += g * kBatchCplxB)
     phi += cplx_one(t0 + t, tri, xy, tensor);
   const double s = block_sum(phi, lds);

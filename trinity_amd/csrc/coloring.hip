@@ -659,7 +659,7 @@ static void color_impl(trn_ctx* ctx, const trn_mesh* mesh, uint64_t seed, int32_
   const int32_t* prev = seam_slots;
   int rounds = 0;
   for (;;) {
-    queue_classes(ctx, nv, color_d, mesh->perm, rows, flags, prev, order_d, cb);
+    queue_classes(ctx, nv, color_d, mesh->class_v, rows, flags, prev, order_d, cb);
     sync(ctx);
     TRN_REQUIRE(h[0] == 0, TRN_ERR_PALETTE,
                 "a vertex needs more than " + std::to_string(max_colors) + " colours");
@@ -737,7 +737,7 @@ int trn_color_classes(trn_ctx* ctx, const trn_mesh* mesh, int32_t nv, const int3
     TRN_REQUIRE(!mesh || mesh->nv == nv, TRN_ERR_ARG, "nv differs from the mesh");
     Scratch sc(ctx, class_bytes(nv, ncolors));
     const ClassBufs cb = carve_classes(sc, nv, ncolors);
-    queue_classes(ctx, nv, color_d, mesh ? mesh->perm : nullptr, ncolors, nullptr, nullptr,
+    queue_classes(ctx, nv, color_d, mesh ? mesh->class_v : nullptr, ncolors, nullptr, nullptr,
                   order_d, cb);
     sync(ctx);
     for (int k = 0; k <= ncolors; ++k) class_off[k] = ctx->pinned_i32[3 + k];

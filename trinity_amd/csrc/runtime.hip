@@ -111,6 +111,8 @@ void flush_timing(trn_ctx* ctx) {
 }
 
 Scratch::Scratch(trn_ctx* ctx, size_t total) : ctx_(ctx) {
+  TRN_REQUIRE(ctx->ws.live == 0, TRN_ERR_STATE,
+              "internal: a second workspace user while one is live");
   if (total > ctx->ws.size) {
     // the previous workspace may still be read by queued work
     TRN_HIP(hipStreamSynchronize(ctx->stream));
@@ -121,7 +123,10 @@ Scratch::Scratch(trn_ctx* ctx, size_t total) : ctx_(ctx) {
     TRN_HIP(hipMalloc(&ctx->ws.base, sz));
     ctx->ws.size = sz;
   }
+  ctx->ws.live = 1;
 }
+
+Scratch::~Scratch() { ctx_->ws.live = 0; }
 
 void sync(trn_ctx* ctx) { TRN_HIP(hipStreamSynchronize(ctx->stream)); }
 

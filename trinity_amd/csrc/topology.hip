@@ -289,10 +289,11 @@ __global__ __launch_bounds__(64) void k_patch_size(int32_t nv, const int32_t* __
 // Rows of each patch slab: the largest patch among its kPatchSlab vertices
 // (one wave per slab).
 __global__ __launch_bounds__(kPatchSlab) void k_patch_slab_rows(int32_t nv,
+                                                               const int32_t* __restrict__ slab_v,
                                                                const int32_t* __restrict__ size,
                                                                int32_t* __restrict__ rows) {
-  const int64_t i = (int64_t)blockIdx.x * kPatchSlab + threadIdx.x;
-  int m = i < nv ? size[i] : 0;
+  const int64_t pos = (int64_t)blockIdx.x * kPatchSlab + threadIdx.x;
+  int m = pos < nv ? size[slab_v ? slab_v[pos] : pos] : 0;
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) {
     const int o = __shfl_xor(m, d, 64);
@@ -302,15 +303,18 @@ __global__ __launch_bounds__(kPatchSlab) void k_patch_slab_rows(int32_t nv,
 }
 
 // Stored patches: the ascending 2-ring element set of every vertex, as
-// k_patch_size builds it, written column-wise into its slab, padded with -1.
-// The columns past the last vertex of the last slab are all padding, so
-// every entry of the array is written.
+// k_patch_size builds it, written column-wise into its slab, padded with -1;
+// slab position pos holds vertex slab_v[pos] (pos without slab_v).  The
+// columns past the last vertex of the last slab are all padding, so every
+// entry of the array is written.
 __global__ __launch_bounds__(kPatchSlab) void k_patch_fill(
-    int32_t nv, const int32_t* __restrict__ v2t_off, const int32_t* __restrict__ v2t,
-    const int32_t* __restrict__ v2v_off, const int32_t* __restrict__ v2v, int cap,
-    const int32_t* __restrict__ row, int32_t* __restrict__ ell) {
+    int32_t nv, const int32_t* __restrict__ slab_v, const int32_t* __restrict__ v2t_off,
+    const int32_t* __restrict__ v2t, const int32_t* __restrict__ v2v_off,
+    const int32_t* __restrict__ v2v, int cap, const int32_t* __restrict__ row,
+    int32_t* __restrict__ ell) {
   extern __shared__ int lds[];
-  const int64_t i = (int64_t)blockIdx.x * kPatchSlab + threadIdx.x;
+  const int64_t pos = (int64_t)blockIdx.x * kPatchSlab + threadIdx.x;
+  const int64_t i = pos < nv ? (slab_v ? slab_v[pos] : pos) : nv;
   int* s = lds + threadIdx.x;
   int m = 0;
   bool ok = true;
@@ -600,6 +604,16 @@ int trn_mesh_create(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
       copy_async(ctx, m->tri, tri, sizeof(int32_t) * 3 * (size_t)nt);
       copy_async(ctx, m->tags, tags, (size_t)nv);
 
+      // 0. locality order (Z-curve of the coordinates): the vertices of a
+      //    patch slab, and on request the visiting order of the colour
+      //    classes.  It takes the context workspace, so it runs before the
+      //    workspace below is carved.
+      if (xy_d) {
+        TRN_HIP(hipMalloc(&m->zorder, sizeof(int32_t) * (size_t)nv));
+        spatial_order(ctx, nv, xy_d, m->zorder);
+        m->slab_v = m->zorder;
+      }
+
       Scratch sc(ctx, 4 * scratch_bytes(sizeof(int32_t) * ((size_t)nv + 1)) +
                           exclusive_scan_scratch(nv) + 256 * 4);
       int32_t* cnt = sc.carve<int32_t>((size_t)nv + 1);
@@ -696,14 +710,16 @@ int trn_mesh_create(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
       TRN_HIP(hipMalloc(&m->patch_row, sizeof(int32_t) * (size_t)(nslab + 1)));
       copy_async(ctx, m->patch_n, bound, sizeof(int32_t) * (size_t)nv);
       launch(ctx, "topo_patch_rows", 0.0, k_patch_slab_rows, dim3((unsigned)nslab),
-             dim3(kPatchSlab), 0, nv, (const int32_t*)bound, m->patch_row);
+             dim3(kPatchSlab), 0, nv, (const int32_t*)m->slab_v, (const int32_t*)bound,
+             m->patch_row);
       exclusive_scan_i32(ctx, m->patch_row, m->patch_row, nslab, true, scan_tmp);
       copy_async(ctx, h, m->patch_row + nslab, sizeof(int32_t));
       sync(ctx);
       const int64_t ell_n = (int64_t)h[0] * kPatchSlab;
       TRN_HIP(hipMalloc(&m->patch_ell, sizeof(int32_t) * (size_t)(ell_n > 0 ? ell_n : 1)));
       launch(ctx, "topo_patch_fill", 0.0, k_patch_fill, dim3((unsigned)nslab), dim3(kPatchSlab),
-             sizeof(int) * kPatchSlab * (size_t)m->patch_cap, nv, (const int32_t*)m->v2t_off,
+             sizeof(int) * kPatchSlab * (size_t)m->patch_cap, nv, (const int32_t*)m->slab_v,
+             (const int32_t*)m->v2t_off,
              (const int32_t*)m->v2t, (const int32_t*)m->v2v_off, (const int32_t*)m->v2v,
              m->patch_cap, (const int32_t*)m->patch_row, m->patch_ell);
 
@@ -736,12 +752,6 @@ int trn_mesh_create(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
              (const int32_t*)m->star, (const int32_t*)m->v2v_off, (const int32_t*)m->v2v,
              m->vrec);
 
-      // 6. locality order of the colour classes
-      if (xy_d) {
-        TRN_HIP(hipMalloc(&m->perm, sizeof(int32_t) * (size_t)nv));
-        spatial_order(ctx, nv, xy_d, m->perm);
-        sync(ctx);
-      }
     } catch (...) {
       cleanup();
       throw;
@@ -762,7 +772,7 @@ int trn_mesh_destroy(trn_mesh* m) {
     if (m->v2v_off) (void)hipFree(m->v2v_off);
     if (m->v2v) (void)hipFree(m->v2v);
     if (m->star) (void)hipFree(m->star);
-    if (m->perm) (void)hipFree(m->perm);
+    if (m->zorder) (void)hipFree(m->zorder);
     if (m->vrec) (void)hipFree(m->vrec);
     if (m->patch_n) (void)hipFree(m->patch_n);
     if (m->patch_row) (void)hipFree(m->patch_row);
@@ -772,6 +782,15 @@ int trn_mesh_destroy(trn_mesh* m) {
     if (m->union_ids) (void)hipFree(m->union_ids);
     if (m->patch_local) (void)hipFree(m->patch_local);
     delete m;
+  });
+}
+
+int trn_mesh_set_class_order(trn_mesh* mesh, int32_t zcurve) {
+  return guard_impl("trn_mesh_set_class_order", [&] {
+    TRN_REQUIRE(mesh != nullptr, TRN_ERR_ARG, "null mesh");
+    TRN_REQUIRE(!zcurve || mesh->zorder, TRN_ERR_ARG,
+                "a Z-curve class order needs coordinates at mesh creation");
+    mesh->class_v = zcurve ? mesh->zorder : nullptr;
   });
 }
 

@@ -58,6 +58,12 @@ __device__ __forceinline__ void tri_grad_one(const int64_t t, const int32_t* __r
   ag_t[t] = make_double2(a * g0, a * g1);
 }
 
+// Patch entries are visited in batches of kPatchBatch: the batch's indices
+// are loaded first, then its terms, so that a lane has that many
+// independent loads in flight instead of one dependent pair at a time; the
+// terms are still added one by one in patch order.
+constexpr int kPatchBatch = 8;
+
 // Per-vertex gradient (src/hessian.cpp:75-82) from the patch in the LDS set:
 // total area and area-weighted element gradients summed in patch order.
 template <bool WITH_GRAD, class Idx>
@@ -67,15 +73,52 @@ __device__ __forceinline__ void vtx_grad_one(const int i, const Idx* s, const in
                                              double2* __restrict__ grad,
                                              double* __restrict__ tarea) {
   double ta = 0., gx = 0., gy = 0.;
-  for (int e = 0; e < m; ++e) {
-    const int t = s[e * stride];
-    ta += area_t[t];  // hessian.cpp:53
-    if (WITH_GRAD) {
-      const double2 r = ag_t[t];
-      gx += r.x;  // hessian.cpp:76-78
-      gy += r.y;
+  for (int e0 = 0; e0 < m; e0 += kPatchBatch) {
+    int t[kPatchBatch];
+#pragma unroll
+    for (int c = 0; c < kPatchBatch; ++c) t[c] = e0 + c < m ? (int)s[(e0 + c) * stride] : 0;
+    double a[kPatchBatch];
+    double2 r[kPatchBatch];
+#pragma unroll
+    for (int c = 0; c < kPatchBatch; ++c) {  // t = 0 past the patch end: in bounds
+      a[c] = area_t[t[c]];
+      if (WITH_GRAD) r[c] = ag_t[t[c]];
     }
+#pragma unroll
+    for (int c = 0; c < kPatchBatch; ++c)
+      if (e0 + c < m) {
+        ta += a[c];  // hessian.cpp:53
+        if (WITH_GRAD) {
+          gx += r[c].x;  // hessian.cpp:76-78
+          gy += r[c].y;
+        }
+      }
   }
+  if (WITH_GRAD) grad[i] = make_double2(gx / ta, gy / ta);  // hessian.cpp:81-82
+  tarea[i] = ta;
+}
+
+// Stored patches of at most kRegRows entries keep their local indices in
+// registers: a lane loads its whole column at once, while the union is being
+// staged, and the sums then read LDS only.  Entries are added in patch order.
+constexpr int kRegRows = 32;
+
+template <bool WITH_GRAD>
+__device__ __forceinline__ void vtx_grad_reg(const int i, const int (&loc)[kRegRows],
+                                             const int m, const double* area,
+                                             const double2* ag, double2* __restrict__ grad,
+                                             double* __restrict__ tarea) {
+  double ta = 0., gx = 0., gy = 0.;
+#pragma unroll
+  for (int e = 0; e < kRegRows; ++e)
+    if (e < m) {
+      ta += area[loc[e]];  // hessian.cpp:53
+      if (WITH_GRAD) {
+        const double2 r = ag[loc[e]];
+        gx += r.x;  // hessian.cpp:76-78
+        gy += r.y;
+      }
+    }
   if (WITH_GRAD) grad[i] = make_double2(gx / ta, gy / ta);  // hessian.cpp:81-82
   tarea[i] = ta;
 }
@@ -119,22 +162,13 @@ __device__ __forceinline__ void normalize_local(double& t0, double& t1, double& 
   t2 = h[3];
 }
 
-// Per-vertex Hessian (src/hessian.cpp:141-152) from the patch in the LDS set,
-// optionally followed by the local normalisation.
-template <bool NORMALIZE, class Idx>
-__device__ __forceinline__ void vtx_hess_one(const int i, const Idx* s, const int stride,
-                                             const int m, const double4* __restrict__ ah_t,
-                                             const double* __restrict__ tarea,
-                                             double* __restrict__ hess, double* __restrict__ out,
-                                             const double ex) {
-  double h0 = 0., h1 = 0., h2 = 0., h3 = 0.;  // hessian.cpp:93
-  for (int e = 0; e < m; ++e) {
-    const double4 r = ah_t[s[e * stride]];
-    h0 += r.x;
-    h1 += r.y;
-    h2 += r.z;
-    h3 += r.w;
-  }
+// Patch sums -> symmetric Hessian (src/hessian.cpp:141-152), optionally
+// followed by the local normalisation, stored at out (and hess if given).
+template <bool NORMALIZE>
+__device__ __forceinline__ void hess_finish(const int i, double h0, double h1, double h2,
+                                            double h3, const double* __restrict__ tarea,
+                                            double* __restrict__ hess, double* __restrict__ out,
+                                            const double ex) {
   const double ta = tarea[i];  // same sum as the gradient's (hessian.cpp:120)
   h0 /= ta;
   h1 /= ta;
@@ -150,6 +184,53 @@ __device__ __forceinline__ void vtx_hess_one(const int i, const Idx* s, const in
   out[3 * (int64_t)i] = t0;
   out[3 * (int64_t)i + 1] = t1;
   out[3 * (int64_t)i + 2] = t2;
+}
+
+// Per-vertex Hessian (src/hessian.cpp:141-152) from the patch in the LDS set,
+// optionally followed by the local normalisation.
+template <bool NORMALIZE, class Idx>
+__device__ __forceinline__ void vtx_hess_one(const int i, const Idx* s, const int stride,
+                                             const int m, const double4* __restrict__ ah_t,
+                                             const double* __restrict__ tarea,
+                                             double* __restrict__ hess, double* __restrict__ out,
+                                             const double ex) {
+  double h0 = 0., h1 = 0., h2 = 0., h3 = 0.;  // hessian.cpp:93
+  for (int e0 = 0; e0 < m; e0 += kPatchBatch) {
+    int t[kPatchBatch];
+#pragma unroll
+    for (int c = 0; c < kPatchBatch; ++c) t[c] = e0 + c < m ? (int)s[(e0 + c) * stride] : 0;
+    double4 r[kPatchBatch];
+#pragma unroll
+    for (int c = 0; c < kPatchBatch; ++c) r[c] = ah_t[t[c]];  // t = 0 past the end: in bounds
+#pragma unroll
+    for (int c = 0; c < kPatchBatch; ++c)
+      if (e0 + c < m) {
+        h0 += r[c].x;
+        h1 += r[c].y;
+        h2 += r[c].z;
+        h3 += r[c].w;
+      }
+  }
+  hess_finish<NORMALIZE>(i, h0, h1, h2, h3, tarea, hess, out, ex);
+}
+
+template <bool NORMALIZE>
+__device__ __forceinline__ void vtx_hess_reg(const int i, const int (&loc)[kRegRows],
+                                             const int m, const double4* ah,
+                                             const double* __restrict__ tarea,
+                                             double* __restrict__ hess, double* __restrict__ out,
+                                             const double ex) {
+  double h0 = 0., h1 = 0., h2 = 0., h3 = 0.;  // hessian.cpp:93
+#pragma unroll
+  for (int e = 0; e < kRegRows; ++e)
+    if (e < m) {
+      const double4 r = ah[loc[e]];
+      h0 += r.x;
+      h1 += r.y;
+      h2 += r.z;
+      h3 += r.w;
+    }
+  hess_finish<NORMALIZE>(i, h0, h1, h2, h3, tarea, hess, out, ex);
 }
 
 // One triangle's term of Metrics::computeComplexity (src/metric.cpp:161-180).

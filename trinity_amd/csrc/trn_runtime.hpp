@@ -55,6 +55,7 @@ struct PendingTiming {
 struct Workspace {
   char* base = nullptr;
   size_t size = 0;
+  int live = 0;  // Scratch objects carving it (at most one)
 };
 
 }  // namespace trn
@@ -129,15 +130,22 @@ struct trn_mesh {
   // packed r | a << 2 | b << 10 with tri[t][r] = i, tri[t][(r+1)%3] =
   // v2v[v2v_off[i]+a], tri[t][(r+2)%3] = v2v[v2v_off[i]+b].
   int32_t* star = nullptr;
-  // Vertices along a Z-curve of their coordinates (NULL: index order);
-  // colour classes list their vertices in this order.
-  int32_t* perm = nullptr;
+  // Vertices along a Z-curve of the coordinates given at creation (NULL:
+  // none were given).
+  int32_t* zorder = nullptr;
+  // Vertex at each position of the patch slabs (slab b holds positions
+  // b * kPatchSlab ..): the Z-curve order when there is one, so that a slab
+  // is a compact patch of the mesh (NULL: index order).  Not owned.
+  const int32_t* slab_v = nullptr;
+  // Visiting order of the vertices of a colour class: NULL (ascending, the
+  // default) or zorder (trn_mesh_set_class_order).  Not owned.
+  const int32_t* class_v = nullptr;
   // [nv] working copy of the smoothing sweeps (topology filled at
   // creation); smoothing calls on one mesh must not overlap.
   trn::VRecord* vrec = nullptr;
   // 2-ring element patches (Mesh::getVicinity(i, 2), ascending) in sliced
   // ELL form: vertices in slabs of kPatchSlab; entry e of the patch of
-  // vertex i = b * kPatchSlab + l is patch_ell[(patch_row[b] + e) *
+  // vertex i = slab_v[b * kPatchSlab + l] is patch_ell[(patch_row[b] + e) *
   // kPatchSlab + l], e < patch_n[i]; slab b has patch_row[b+1] -
   // patch_row[b] rows (its largest patch), shorter patches padded with -1.
   int32_t* patch_n = nullptr;    // [nv]
@@ -203,9 +211,16 @@ inline void launch(trn_ctx* ctx, const char* name, double bytes, K kernel,
 // ctx->stream, so reuse across calls is safe.
 inline size_t scratch_bytes(size_t bytes) { return (bytes + 255) & ~size_t(255); }
 
+// Slices of the context workspace for one entry point's launches.  Every
+// Scratch carves from the start of the workspace (and may reallocate it),
+// so at most one may be live per context: a second one while another is
+// live is refused (TRN_ERR_STATE) instead of aliasing the first's buffers.
 class Scratch {
  public:
   Scratch(trn_ctx* ctx, size_t total);
+  ~Scratch();
+  Scratch(const Scratch&) = delete;
+  Scratch& operator=(const Scratch&) = delete;
   template <typename T>
   T* carve(size_t count) {
     size_t bytes = scratch_bytes(count * sizeof(T));

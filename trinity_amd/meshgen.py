@@ -130,8 +130,7 @@ def raise_degrees(m: Mesh, max_deg: int = 12, rounds: int = 40, seed: int = 5,
 
     for _ in range(rounds):
         a, b, c, d, t1, t2, keys = _interior_edges(tri, nv)
-        ends = np.concatenate([a, b])
-        deg = np.bincount(ends, minlength=nv) + 0  # every interior edge once
+        deg = np.bincount(np.concatenate([a, b]), minlength=nv)  # interior edges
         bnd_e = boundary_edges(tri)
         if bnd_e:
             be = np.asarray(bnd_e, dtype=np.int64).ravel()
@@ -244,6 +243,8 @@ def workload(name: str, seed: int = 1234) -> Mesh:
     jitter4096   : 4096^2 jittered grid, boundary layer     (16,785,409 V)
     flipN        : N^2 jittered grid, half the cells split along the other
                    diagonal (vertex degrees 4..8), 3 random fronts
+    hubN         : N^2 jittered grid after 10 rounds of random edge flips
+                   (vertex degrees 4..12), 3 random fronts
     Other sizes: 'jitterN' / 'uniformN' with the field picked as above
     ('fronts' for jittered, 'tanh' for uniform) or suffixes ':tanh',
     ':fronts', ':fronts+quad', ':layer', ':linear', ':quadratic'.
@@ -260,6 +261,10 @@ def workload(name: str, seed: int = 1234) -> Mesh:
     elif base.startswith("flip"):
         n = int(base[len("flip"):])
         m = grid_mesh(n, 0.25, seed, flip=0.5)
+        default = "fronts"
+    elif base.startswith("hub"):
+        n = int(base[len("hub"):])
+        m = raise_degrees(grid_mesh(n, 0.25, seed), 12, rounds=10, seed=seed + 3)
         default = "fronts"
     else:
         raise ValueError(f"unknown workload {name!r}")
