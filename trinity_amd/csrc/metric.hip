@@ -74,6 +74,76 @@ __device__ __forceinline__ void stage_union(const int lane, const int32_t* __res
   }
 }
 
+// Fused staging: the union's per-triangle terms computed in place from the
+// triangles and their corners (the same arithmetic as k_tri_grad /
+// k_tri_hess, so the sums are bit-identical), kFusedBatch entries per lane
+// in flight.  A union never holds a deleted triangle; lanes past the end
+// read triangle 0's corners and store nothing.
+constexpr int kFusedBatch = 4;
+
+__device__ __forceinline__ void stage_grad_terms(const int lane, const int32_t* __restrict__ U,
+                                                 const int nu, const int32_t* __restrict__ tri,
+                                                 const double2* __restrict__ xy,
+                                                 const double* __restrict__ f, double* lds_a,
+                                                 double2* lds_g) {
+  for (int k0 = lane; k0 < nu; k0 += kFusedBatch * kGatherB) {
+    int32_t t[kFusedBatch];
+#pragma unroll
+    for (int c = 0; c < kFusedBatch; ++c) {
+      const int k = k0 + c * kGatherB;
+      t[c] = k < nu ? U[k] : 0;
+    }
+    int32_t n[kFusedBatch][3];
+#pragma unroll
+    for (int c = 0; c < kFusedBatch; ++c) {
+      const bool in = k0 + c * kGatherB < nu;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) n[c][j] = in ? tri[3 * (int64_t)t[c] + j] : 0;
+    }
+#pragma unroll
+    for (int c = 0; c < kFusedBatch; ++c) {
+      double a;
+      double2 ag;
+      grad_terms(xy[n[c][0]], xy[n[c][1]], xy[n[c][2]], f[n[c][0]], f[n[c][1]], f[n[c][2]], a,
+                 ag);
+      const int k = k0 + c * kGatherB;
+      if (k < nu) {
+        lds_a[k] = a;
+        lds_g[k] = ag;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void stage_hess_terms(const int lane, const int32_t* __restrict__ U,
+                                                 const int nu, const int32_t* __restrict__ tri,
+                                                 const double2* __restrict__ xy,
+                                                 const double2* __restrict__ grad,
+                                                 double4* lds_h) {
+  for (int k0 = lane; k0 < nu; k0 += kFusedBatch * kGatherB) {
+    int32_t t[kFusedBatch];
+#pragma unroll
+    for (int c = 0; c < kFusedBatch; ++c) {
+      const int k = k0 + c * kGatherB;
+      t[c] = k < nu ? U[k] : 0;
+    }
+    int32_t n[kFusedBatch][3];
+#pragma unroll
+    for (int c = 0; c < kFusedBatch; ++c) {
+      const bool in = k0 + c * kGatherB < nu;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) n[c][j] = in ? tri[3 * (int64_t)t[c] + j] : 0;
+    }
+#pragma unroll
+    for (int c = 0; c < kFusedBatch; ++c) {
+      const double4 r = hess_terms(xy[n[c][0]], xy[n[c][1]], xy[n[c][2]], grad[n[c][0]],
+                                   grad[n[c][1]], grad[n[c][2]]);
+      const int k = k0 + c * kGatherB;
+      if (k < nu) lds_h[k] = r;
+    }
+  }
+}
+
 // The lane's column of its slab's local patch indices, rows < `rows` (the
 // slab's row count, wave-uniform, <= kRegRows); entries past the lane's
 // own patch are padding and never used.
@@ -104,13 +174,17 @@ __device__ __forceinline__ GatherSlot gather_slot() {
 // from LDS in patch order; a slab without one gathers from global memory.
 // wave_bytes: LDS bytes per wave, ag terms (16 B) then areas (8 B) of up to
 // max_union entries.
-template <bool WITH_GRAD>
+// FUSED: union slabs compute their terms from tri, xy and f instead of
+// reading area_t / ag_t (which only slabs without a union then need).
+template <bool WITH_GRAD, bool FUSED>
 __global__ __launch_bounds__(kGatherB * kGatherWaves) void k_vtx_grad(
     int32_t nv, int32_t nslab, const int32_t* __restrict__ slab_v,
     const int32_t* __restrict__ patch_n, const int32_t* __restrict__ patch_row,
     const int32_t* __restrict__ patch_ell, const int32_t* __restrict__ union_n,
     const int32_t* __restrict__ union_off, const int32_t* __restrict__ union_ids,
-    const uint16_t* __restrict__ patch_local, int max_union, int wave_bytes, const double* __restrict__ area_t,
+    const uint16_t* __restrict__ patch_local, int max_union, int wave_bytes,
+    const int32_t* __restrict__ tri, const double2* __restrict__ xy,
+    const double* __restrict__ f, const double* __restrict__ area_t,
     const double2* __restrict__ ag_t, double2* __restrict__ grad, double* __restrict__ tarea) {
   extern __shared__ double2 lds_all[];
   const GatherSlot g = gather_slot();
@@ -126,8 +200,12 @@ __global__ __launch_bounds__(kGatherB * kGatherWaves) void k_vtx_grad(
     r0 = patch_row[g.b];
     rows = patch_row[g.b + 1] - r0;
     if (rows <= kRegRows) load_column(patch_local + (int64_t)r0 * kPatchSlab + g.lane, rows, loc);
-    stage_union(g.lane, U, nu, area_t, lds_a);
-    if (WITH_GRAD) stage_union(g.lane, U, nu, ag_t, lds_g);
+    if (FUSED) {
+      stage_grad_terms(g.lane, U, nu, tri, xy, f, lds_a, lds_g);
+    } else {
+      stage_union(g.lane, U, nu, area_t, lds_a);
+      if (WITH_GRAD) stage_union(g.lane, U, nu, ag_t, lds_g);
+    }
   }
   __syncthreads();
   if (!live || pos >= nv) return;
@@ -156,14 +234,17 @@ __global__ __launch_bounds__(256) void k_tri_hess(int32_t nt, const int32_t* __r
 }
 
 // wave_bytes: LDS bytes per wave, the ah terms (32 B) of up to max_union
-// entries.
+// entries.  Union slabs compute their terms from tri, xy and grad; ah_t is
+// read by slabs without a union only.
 template <bool NORMALIZE>
 __global__ __launch_bounds__(kGatherB * kGatherWaves) void k_vtx_hess(
     int32_t nv, int32_t nslab, const int32_t* __restrict__ slab_v,
     const int32_t* __restrict__ patch_n, const int32_t* __restrict__ patch_row,
     const int32_t* __restrict__ patch_ell, const int32_t* __restrict__ union_n,
     const int32_t* __restrict__ union_off, const int32_t* __restrict__ union_ids,
-    const uint16_t* __restrict__ patch_local, int wave_bytes, const double4* __restrict__ ah_t, const double* __restrict__ tarea,
+    const uint16_t* __restrict__ patch_local, int wave_bytes, const int32_t* __restrict__ tri,
+    const double2* __restrict__ xy, const double2* __restrict__ grad,
+    const double4* __restrict__ ah_t, const double* __restrict__ tarea,
     double* __restrict__ hess, double* __restrict__ out, double ex) {
   extern __shared__ double4 lds_all4[];
   const GatherSlot g = gather_slot();
@@ -178,7 +259,7 @@ __global__ __launch_bounds__(kGatherB * kGatherWaves) void k_vtx_hess(
     r0 = patch_row[g.b];
     rows = patch_row[g.b + 1] - r0;
     if (rows <= kRegRows) load_column(patch_local + (int64_t)r0 * kPatchSlab + g.lane, rows, loc);
-    stage_union(g.lane, U, nu, ah_t, lds_h);
+    stage_hess_terms(g.lane, U, nu, tri, xy, grad, lds_h);
   }
   __syncthreads();
   if (!live || pos >= nv) return;
@@ -304,35 +385,44 @@ static GatherShape gather_shape(const trn_mesh* m, size_t per_entry) {
   return g;
 }
 
-// The gradient gathers' LDS holds the union's ag (16 B) and area (8 B) terms.
+// The gradient gathers' LDS holds the union's ag (16 B) and area (8 B)
+// terms.  fused: union slabs compute them from tri, xy and f.
 static void launch_vtx_grad(trn_ctx* ctx, const char* name, double bytes, bool with_grad,
-                            const trn_mesh* m, const double* area_t, const double2* ag_t,
-                            double2* grad, double* tarea) {
+                            bool fused, const trn_mesh* m, const double* xy, const double* f,
+                            const double* area_t, const double2* ag_t, double2* grad,
+                            double* tarea) {
   const GatherShape g = gather_shape(m, sizeof(double2) + sizeof(double));
-  auto kern = with_grad ? k_vtx_grad<true> : k_vtx_grad<false>;
+  auto kern = fused ? k_vtx_grad<true, true>
+                    : (with_grad ? k_vtx_grad<true, false> : k_vtx_grad<false, false>);
   launch(ctx, name, bytes, kern, g.grid, g.block, g.shmem, m->nv, g.nslab, m->slab_v,
          (const int32_t*)m->patch_n, (const int32_t*)m->patch_row,
          (const int32_t*)m->patch_ell, (const int32_t*)m->union_n,
          (const int32_t*)m->union_off, (const int32_t*)m->union_ids,
-         (const uint16_t*)m->patch_local, m->max_union, g.wave_bytes, area_t, ag_t, grad, tarea);
+         (const uint16_t*)m->patch_local, m->max_union, g.wave_bytes, (const int32_t*)m->tri,
+         (const double2*)xy, f, area_t, ag_t, grad, tarea);
 }
 
 void patch_area_sums(trn_ctx* ctx, const trn_mesh* m, const double* area_t, double* tarea) {
-  launch_vtx_grad(ctx, "vtx_area", 0.0, false, m, area_t, nullptr, nullptr, tarea);
+  launch_vtx_grad(ctx, "vtx_area", 0.0, false, false, m, nullptr, nullptr, area_t, nullptr,
+                  nullptr, tarea);
 }
 
+// The per-triangle kernels run only for meshes with slabs too large for a
+// union (the union slabs compute their terms while staging).
 static void run_gradient(trn_ctx* ctx, const trn_mesh* m, const double* xy, const double* f,
                          double* grad, GradBuffers& b) {
-  launch(ctx, "tri_grad", 0.0, k_tri_grad, dim3(nblk(m->nt, 256)), dim3(256), 0, m->nt,
-         (const int32_t*)m->tri, (const double2*)xy, f, b.area_t, b.ag_t);
-  launch_vtx_grad(ctx, "vtx_grad", kBytesGradPerV * m->nv, true, m, b.area_t, b.ag_t,
-                  (double2*)grad, b.tarea);
+  if (!m->union_all)
+    launch(ctx, "tri_grad", 0.0, k_tri_grad, dim3(nblk(m->nt, 256)), dim3(256), 0, m->nt,
+           (const int32_t*)m->tri, (const double2*)xy, f, b.area_t, b.ag_t);
+  launch_vtx_grad(ctx, "vtx_grad", kBytesGradPerV * m->nv, true, true, m, xy, f, b.area_t,
+                  b.ag_t, (double2*)grad, b.tarea);
 }
 
 static void run_hessian(trn_ctx* ctx, const trn_mesh* m, const double* xy, const double* grad,
                         double* hess, double* out, bool normalize, double ex, GradBuffers& b) {
-  launch(ctx, "tri_hess", 0.0, k_tri_hess, dim3(nblk(m->nt, 256)), dim3(256), 0, m->nt,
-         (const int32_t*)m->tri, (const double2*)xy, (const double2*)grad, b.ah_t);
+  if (!m->union_all)
+    launch(ctx, "tri_hess", 0.0, k_tri_hess, dim3(nblk(m->nt, 256)), dim3(256), 0, m->nt,
+           (const int32_t*)m->tri, (const double2*)xy, (const double2*)grad, b.ah_t);
   const GatherShape g = gather_shape(m, sizeof(double4));
   auto kern = normalize ? k_vtx_hess<true> : k_vtx_hess<false>;
   launch(ctx, normalize ? "vtx_hess_norm" : "vtx_hess", kBytesHessPerV * m->nv, kern, g.grid,
@@ -340,6 +430,7 @@ static void run_hessian(trn_ctx* ctx, const trn_mesh* m, const double* xy, const
          (const int32_t*)m->patch_row, (const int32_t*)m->patch_ell,
          (const int32_t*)m->union_n, (const int32_t*)m->union_off,
          (const int32_t*)m->union_ids, (const uint16_t*)m->patch_local, g.wave_bytes,
+         (const int32_t*)m->tri, (const double2*)xy, (const double2*)grad,
          (const double4*)b.ah_t, (const double*)b.tarea, hess, out, ex);
 }
 
@@ -437,7 +528,8 @@ int trn_hessian(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d, const do
     launch(ctx, "tri_area", 0.0, k_tri_grad, dim3(nblk(mesh->nt, 256)), dim3(256), 0, mesh->nt,
            (const int32_t*)mesh->tri, (const double2*)xy_d, (const double*)nullptr, b.area_t,
            b.ag_t);
-    launch_vtx_grad(ctx, "vtx_area", 0.0, false, mesh, b.area_t, b.ag_t, nullptr, b.tarea);
+    launch_vtx_grad(ctx, "vtx_area", 0.0, false, false, mesh, xy_d, nullptr, b.area_t, b.ag_t,
+                    nullptr, b.tarea);
     run_hessian(ctx, mesh, xy_d, gradient_d, nullptr, tensor_d, false, 0.0, b);
   });
 }

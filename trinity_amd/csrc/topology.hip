@@ -247,6 +247,14 @@ __global__ __launch_bounds__(64) void k_v2v(int32_t nv, const int32_t* __restric
   }
 }
 
+// Counts the entries of a below zero (slabs without a union).
+__global__ void k_count_negative(int64_t n, const int32_t* __restrict__ a, int32_t* count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool neg = i < n && a[i] < 0;
+  const uint64_t m = __ballot(neg);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(count, (int32_t)__popcll(m));
+}
+
 // Upper bound of the 2-ring candidates: deg(i) + sum_{j in N(i)} deg(j).
 __global__ void k_patch_bound(int32_t nv, const int32_t* __restrict__ v2t_off,
                               const int32_t* __restrict__ v2v_off,
@@ -735,9 +743,13 @@ int trn_mesh_create(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
              0, nv, (const int32_t*)m->patch_row, (const int32_t*)m->patch_ell, m->union_n,
              (const int32_t*)m->union_off, m->union_ids, m->patch_local);
       reduce_max_i32(ctx, m->union_n, nslab, flags + 7);
-      copy_async(ctx, h + 1, flags + 7, sizeof(int32_t));
+      memset_async(ctx, flags + 8, 0, sizeof(int32_t));
+      launch(ctx, "topo_union_count", 0.0, k_count_negative, dim3(blocks_for(nslab, 256)),
+             dim3(256), 0, nslab, (const int32_t*)m->union_n, flags + 8);
+      copy_async(ctx, h + 1, flags + 7, 2 * sizeof(int32_t));
       sync(ctx);
       m->max_union = h[1] > 0 ? h[1] : 0;
+      m->union_all = h[2] == 0;
 
       // 5. star codes of the incident triangles (smoothing reads corners
       //    through the 1-ring instead of the triangle array)

@@ -33,21 +33,14 @@ __device__ __forceinline__ int build_patch(const int i, const int32_t* __restric
   return m;
 }
 
-// Per-triangle gradient terms (src/hessian.cpp:42-67): area and the products
-// area * element gradient that the vertex sums add (:78).  f == nullptr
-// stores the area only.
-__device__ __forceinline__ void tri_grad_one(const int64_t t, const int32_t* __restrict__ tri,
-                                             const double2* __restrict__ xy,
-                                             const double* __restrict__ f,
-                                             double* __restrict__ area_t,
-                                             double2* __restrict__ ag_t) {
-  const int32_t n0 = tri[3 * t], n1 = tri[3 * t + 1], n2 = tri[3 * t + 2];
-  if (n0 < 0) return;  // deleted triangles never appear in a patch
-  double a, d[6];
-  tri_geometry(xy[n0], xy[n1], xy[n2], a, d);
-  area_t[t] = a;
-  if (f == nullptr) return;
-  const double f0 = f[n0], f1 = f[n1], f2 = f[n2];
+// Per-triangle gradient terms (src/hessian.cpp:42-67) from the corners'
+// coordinates and field values: area and the products area * element
+// gradient that the vertex sums add (:78).
+__device__ __forceinline__ void grad_terms(const double2 p0, const double2 p1, const double2 p2,
+                                           const double f0, const double f1, const double f2,
+                                           double& a, double2& ag) {
+  double d[6];
+  tri_geometry(p0, p1, p2, a, d);
   double g0 = 0., g1 = 0.;  // hessian.cpp:36, accumulated j = 0,1,2 (:65-67)
   g0 += f0 * d[0];
   g1 += f0 * d[1];
@@ -55,7 +48,29 @@ __device__ __forceinline__ void tri_grad_one(const int64_t t, const int32_t* __r
   g1 += f1 * d[3];
   g0 += f2 * d[4];
   g1 += f2 * d[5];
-  ag_t[t] = make_double2(a * g0, a * g1);
+  ag = make_double2(a * g0, a * g1);
+}
+
+// grad_terms of triangle t into area_t / ag_t.  f == nullptr stores the
+// area only.
+__device__ __forceinline__ void tri_grad_one(const int64_t t, const int32_t* __restrict__ tri,
+                                             const double2* __restrict__ xy,
+                                             const double* __restrict__ f,
+                                             double* __restrict__ area_t,
+                                             double2* __restrict__ ag_t) {
+  const int32_t n0 = tri[3 * t], n1 = tri[3 * t + 1], n2 = tri[3 * t + 2];
+  if (n0 < 0) return;  // deleted triangles never appear in a patch
+  if (f == nullptr) {
+    double a, d[6];
+    tri_geometry(xy[n0], xy[n1], xy[n2], a, d);
+    area_t[t] = a;
+    return;
+  }
+  double a;
+  double2 ag;
+  grad_terms(xy[n0], xy[n1], xy[n2], f[n0], f[n1], f[n2], a, ag);
+  area_t[t] = a;
+  ag_t[t] = ag;
 }
 
 // Patch entries are visited in batches of kPatchBatch: the batch's indices
@@ -123,16 +138,13 @@ __device__ __forceinline__ void vtx_grad_reg(const int i, const int (&loc)[kRegR
   tarea[i] = ta;
 }
 
-// Per-triangle Hessian terms: area * delta (src/hessian.cpp:108-143).
-__device__ __forceinline__ void tri_hess_one(const int64_t t, const int32_t* __restrict__ tri,
-                                             const double2* __restrict__ xy,
-                                             const double2* __restrict__ grad,
-                                             double4* __restrict__ ah_t) {
-  const int32_t n0 = tri[3 * t], n1 = tri[3 * t + 1], n2 = tri[3 * t + 2];
-  if (n0 < 0) return;
+// Per-triangle Hessian terms: area * delta (src/hessian.cpp:108-143) from
+// the corners' coordinates and recovered gradients.
+__device__ __forceinline__ double4 hess_terms(const double2 p0, const double2 p1,
+                                              const double2 p2, const double2 g0,
+                                              const double2 g1, const double2 g2) {
   double a, d[6];
-  tri_geometry(xy[n0], xy[n1], xy[n2], a, d);
-  const double2 g0 = grad[n0], g1 = grad[n1], g2 = grad[n2];
+  tri_geometry(p0, p1, p2, a, d);
   double l0 = 0., l1 = 0., l2 = 0., l3 = 0.;  // hessian.cpp:99, :131-136
   l0 += g0.x * d[0]; l1 += g0.x * d[1]; l2 += g0.y * d[0]; l3 += g0.y * d[1];
   l0 += g1.x * d[2]; l1 += g1.x * d[3]; l2 += g1.y * d[2]; l3 += g1.y * d[3];
@@ -142,7 +154,16 @@ __device__ __forceinline__ void tri_hess_one(const int64_t t, const int32_t* __r
   r.y = a * l1;
   r.z = a * l2;
   r.w = a * l3;
-  ah_t[t] = r;
+  return r;
+}
+
+__device__ __forceinline__ void tri_hess_one(const int64_t t, const int32_t* __restrict__ tri,
+                                             const double2* __restrict__ xy,
+                                             const double2* __restrict__ grad,
+                                             double4* __restrict__ ah_t) {
+  const int32_t n0 = tri[3 * t], n1 = tri[3 * t + 1], n2 = tri[3 * t + 2];
+  if (n0 < 0) return;
+  ah_t[t] = hess_terms(xy[n0], xy[n1], xy[n2], grad[n0], grad[n1], grad[n2]);
 }
 
 // Metrics::normalizeLocally for one tensor (src/metric.cpp:113-148).

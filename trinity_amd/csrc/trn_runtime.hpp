@@ -162,6 +162,7 @@ struct trn_mesh {
   int32_t* union_ids = nullptr;      // [patch_row[nslab] * kPatchSlab]
   uint16_t* patch_local = nullptr;   // [patch_row[nslab] * kPatchSlab]
   int32_t max_union = 0;             // largest union_n
+  bool union_all = false;            // every slab has a union
   int32_t color_tile = 0;            // colouring tile size (0: chosen from bandwidth)
   int32_t bandwidth = 0;             // largest |u - v| over the edges
 };
