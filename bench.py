@@ -98,7 +98,19 @@ def cpu_arithmetic_only(args, m, threads) -> dict:
                     "stencils; topology and colouring untimed"}
 
 
-def cpu_baseline(args) -> dict | None:
+def reference_load(mp, bp, threads) -> dict:
+    """The reference's Mesh::load (text parse + rebuildTopology) of the
+    ingest files (oracle/_ref/ref_harness `load`)."""
+    import oracle
+    with tempfile.TemporaryDirectory() as td:
+        js = os.path.join(td, "r.json")
+        subprocess.run([oracle.REF_HARNESS, "load", mp, bp, td, js], check=True,
+                       env=dict(os.environ, OMP_NUM_THREADS=str(threads)),
+                       stdout=subprocess.DEVNULL)
+        return json.load(open(js))
+
+
+def cpu_baseline(args, ingest_files=None) -> dict | None:
     """The reference OpenMP path (oracle/_ref/ref_bench: the unmodified
     hobywan/trinity sources, -O3 -funroll-loops) timed on this host on a
     bounded sample of the same workload family; falls back to the oracle
@@ -135,6 +147,17 @@ def cpu_baseline(args) -> dict | None:
             out["arithmetic_only"] = cpu_arithmetic_only(args, m, threads)
         except Exception as e:  # the reference figure stands on its own
             log(f"arithmetic-only cpu split failed: {e!r}")
+        if ingest_files is not None:
+            try:
+                rl = reference_load(*ingest_files, threads)
+                log(f"reference Mesh::load: {rl}")
+                out["reference_text_load"] = {
+                    "ms": rl["load_ms"], "threads": rl["threads"],
+                    "note": "Mesh::load of the ingest .mesh/.bb files of the same mesh: "
+                            "iostream parse and rebuildTopology (compare "
+                            "extras.ingest.text_to_device_mesh_ms)"}
+            except Exception as e:
+                log(f"reference load timing failed: {e!r}")
         return out
     oracle.set_threads(threads)
     ts = []
@@ -457,6 +480,60 @@ def run_indep_set(T, mg, ctx, workload, frac=0.5, reps=5, seed=1234):
             "Mnodes_per_s": round(n / (ms * 1e-3) / 1e6, 1)}
 
 
+def run_ingest(T, mg, ctx, workload, seed, td):
+    """Input to a device mesh, end to end, from the text files the reference
+    reads (Mesh::load, src/io.cpp:53-161) and from the binary form: file ->
+    host parse (text) or direct read (binary) -> device arrays ->
+    trn_mesh_create.  The files are written to td first (untimed) and read
+    twice; the second, page-cached read is the one reported."""
+    import numpy as np
+    import torch
+    m = mg.workload(workload, seed=seed)
+    mp, bp, tp = (os.path.join(td, n) for n in ("m.mesh", "m.bb", "m.trnm"))
+    t0 = time.perf_counter()
+    mg.write_medit(mp, bp, m)
+    mg.write_trnm(tp, m)
+    log(f"ingest: wrote {os.path.getsize(mp) / 1e6:.0f} MB text, "
+        f"{os.path.getsize(tp) / 1e6:.0f} MB binary in {time.perf_counter() - t0:.1f} s")
+    nv, nt = m.nv, m.nt
+    del m
+    out = {"workload": workload, "vertices": nv,
+           "text_bytes": os.path.getsize(mp) + os.path.getsize(bp),
+           "binary_bytes": os.path.getsize(tp)}
+    for rep in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xy, tri, tags, f = T.read_medit(mp, bp)
+        t1 = time.perf_counter()
+        xy_d = torch.from_numpy(xy).cuda()
+        f_d = torch.from_numpy(f).cuda()
+        dm = T.DeviceMesh(ctx, nv, torch.from_numpy(tri.reshape(-1)).cuda(),
+                          torch.from_numpy(tags).cuda(), xy=xy_d)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        dm.close()
+        del xy, tri, tags, f, xy_d, f_d
+        t3 = time.perf_counter()
+        xy_d, tri_d, tags_d, f_d = T.read_trnm(tp, ctx)
+        t4 = time.perf_counter()
+        dm = T.DeviceMesh(ctx, nv, tri_d.reshape(-1), tags_d, xy=xy_d)
+        torch.cuda.synchronize()
+        t5 = time.perf_counter()
+        dm.close()
+        del xy_d, tri_d, tags_d, f_d
+        free_gpu()
+    out.update({"text_parse_ms": round((t1 - t0) * 1e3, 1),
+                "text_to_device_mesh_ms": round((t2 - t0) * 1e3, 1),
+                "binary_read_ms": round((t4 - t3) * 1e3, 1),
+                "binary_to_device_mesh_ms": round((t5 - t3) * 1e3, 1),
+                "io_threads": int(os.environ.get("TRN_IO_THREADS", "0")) or min(16, os.cpu_count() or 1),
+                "note": "trn_medit_read (multi-threaded parse with Mesh::load's semantics) or "
+                        "trn_trnm_read (pinned, overlapped copies), then trn_mesh_create "
+                        "(CSR, 2-ring patches, records); second of two reads (page cache warm)"})
+    log(f"ingest: {out}")
+    return out, (mp, bp)
+
+
 def run_adjoint(T, mg, ctx, workload, steps, p=2):
     """Forward (metric + qualities + energy) and backward (dE/dxy, dE/df)
     of the energy sum_t (1 - q_M(t))^2."""
@@ -704,6 +781,9 @@ def main():
     ap.add_argument("--cpu-workload", default=None,
                     help="mesh of the CPU baseline (default: the headline's)")
     ap.add_argument("--cpu-reps", type=int, default=1)
+    ap.add_argument("--ingest", action="store_true",
+                    help="also time file input to a device mesh (text vs binary) on the "
+                         "workload, and the reference's Mesh::load beside it")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 512 MiB device-to-device 3 times (scales the rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE counters against known traffic)")
@@ -755,6 +835,11 @@ def main():
     roof = roofline_for(dname, dstat, args.workload, stats)
 
     extras = {}
+    ingest_files = None
+    ingest_dir = tempfile.TemporaryDirectory() if args.ingest else None
+    if args.ingest:
+        extras["ingest"], ingest_files = run_ingest(T, mg, ctx, args.workload, args.seed,
+                                                    ingest_dir.name)
     if not args.no_extras:
         wl2 = "jitter2048:fronts"
         for p in (2, 1, 4):
@@ -787,9 +872,11 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline(args)
+            cpu = cpu_baseline(args, ingest_files)
         except Exception as e:  # the GPU number stands on its own
             log(f"cpu baseline failed: {e!r}")
+    if ingest_dir is not None:
+        ingest_dir.cleanup()
 
     out = {
         "metric": "Mvertices/s per recovery+metric+smooth pass",

@@ -75,6 +75,35 @@ int trn_ctx_timing_count(trn_ctx* ctx, int32_t* count);
 int trn_ctx_timing_get(trn_ctx* ctx, int32_t index, const char** name,
                        double* total_ms, int64_t* launches, double* bytes);
 
+/* ---- input: the parse of Mesh::load, and a binary format ------------------
+ * trn_medit_info / trn_medit_read replace the parse of Mesh::load
+ * (src/io.cpp:25-48 io::find, :53-114) with a multi-threaded one of the same
+ * text files: sections searched in the order Vertices, Edges, Triangles,
+ * Corners; "x y ref" vertices; "a b ref" edges whose ends get tag 4
+ * (boundary); "a b c ref" triangles, 1-based in the file, 0-based in tri;
+ * corners tagged 4|8 after the edges; the .bb field from its second line
+ * (bb_path and solut may both be NULL).  Coordinates and values are the
+ * reference's bit for bit (correctly rounded conversions).  Records must be
+ * one per line; another layout, an index outside 1..nv or counts other than
+ * nv / nt return TRN_ERR_ARG.  Host arrays: xy [2nv], tri [3nt], tags [nv],
+ * solut [nv].  The topology is then built by trn_mesh_create
+ * (Mesh::rebuildTopology).  No GPU is needed for these two.
+ *
+ * TRNM is this library's binary form of the same data: "TRNM", int32
+ * version 1, int32 nv, int32 nt, f64 xy[2nv], int32 tri[3nt] (0-based),
+ * u8 tags[nv], f64 solut[nv], little-endian.  trn_trnm_read fills host
+ * arrays (ctx NULL) or device arrays (ctx given: read through pinned
+ * buffers, each copy overlapping the next read; returns when the copies are
+ * done); solut may be NULL.  trn_trnm_write writes host arrays. */
+int trn_medit_info(const char* mesh_path, int32_t* nv, int32_t* nt);
+int trn_medit_read(const char* mesh_path, const char* bb_path, int32_t nv, int32_t nt,
+                   double* xy, int32_t* tri, uint8_t* tags, double* solut);
+int trn_trnm_info(const char* path, int32_t* nv, int32_t* nt);
+int trn_trnm_read(trn_ctx* ctx, const char* path, int32_t nv, int32_t nt, double* xy,
+                  int32_t* tri, uint8_t* tags, double* solut);
+int trn_trnm_write(const char* path, int32_t nv, int32_t nt, const double* xy,
+                   const int32_t* tri, const uint8_t* tags, const double* solut);
+
 /* ---- topology: Mesh::load + Mesh::rebuildTopology ------------------------
  * Replaces the topology part of Mesh::load (src/io.cpp:97-99, :128-129) and
  * Mesh::rebuildTopology (src/mesh.cpp:191-249): builds, on the device,

@@ -17,6 +17,8 @@
  * Usage:
  *   ref_harness dump  MESH OUTDIR tf p hmin hmax sweeps [COLOR]
  *   ref_harness bench MESH JSON   tf p hmin hmax sweeps reps
+ *   ref_harness load  MEDIT BB OUTDIR [JSON]   (Mesh::load of the text files:
+ *                     xy, tri (0-based), tags, solut dumps; JSON: its time)
  */
 #include <stack>
 #include <set>
@@ -241,8 +243,38 @@ static int benchMode(int argc, char** argv) {
   return 0;
 }
 
+// Mesh::load (src/io.cpp:53-161) of a .mesh/.bb pair, as src/main.cpp
+// sizes it (io::find of the two counts), dumping what it parsed.
+static int loadMode(int argc, char** argv) {
+  if (argc < 5) die("usage: ref_harness load MEDIT BB OUTDIR [JSON]");
+  std::ifstream file(argv[2], std::ios::in);
+  int size[] = {io::find("Vertices", file), io::find("Triangles", file)};
+  file.close();
+  if (size[0] <= 0 || size[1] <= 0) die("no Vertices or Triangles");
+  Mesh mesh(size, 64, 2, 0, 1);  // depth 2: node/element capacities of 3x
+  const auto t0 = std::chrono::steady_clock::now();
+  mesh.load(argv[2], argv[3]);
+  const double ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  const std::string out = argv[4];
+  const int nv = mesh.nb.nodes, nt = mesh.nb.elems;
+  writeBin(out + "/xy.bin", mesh.geom.points.data(), sizeof(double) * 2 * nv);
+  writeBin(out + "/tri.bin", mesh.topo.elems.data(), sizeof(int) * 3 * nt);
+  writeBin(out + "/tags.bin", mesh.sync.tags, (size_t)nv);
+  writeBin(out + "/solut.bin", mesh.geom.solut.data(), sizeof(double) * nv);
+  if (argc > 5) {
+    FILE* f = std::fopen(argv[5], "w");
+    if (!f) die("cannot write json");
+    std::fprintf(f, "{\"nv\": %d, \"nt\": %d, \"threads\": %d, \"load_ms\": %.3f}\n", nv, nt,
+                 omp_get_max_threads(), ms);
+    std::fclose(f);
+  }
+  return 0;
+}
+
 int main(int argc, char** argv) {
-  if (argc < 2) die("usage: ref_harness dump|bench ...");
+  if (argc < 2) die("usage: ref_harness dump|bench|load ...");
+  if (!std::strcmp(argv[1], "load")) return loadMode(argc, argv);
   if (!std::strcmp(argv[1], "dump")) return dumpMode(argc, argv);
   if (!std::strcmp(argv[1], "bench")) return benchMode(argc, argv);
   die("unknown mode");

@@ -39,6 +39,7 @@ EXPORTS = [
     "trn_metric_run", "trn_gradient", "trn_hessian", "trn_normalize_local",
     "trn_complexity", "trn_normalize_global", "trn_color", "trn_color_partial",
     "trn_mesh_set_color_tile", "trn_mesh_set_class_order", "trn_color_classes",
+    "trn_medit_info", "trn_medit_read", "trn_trnm_info", "trn_trnm_read", "trn_trnm_write",
     "trn_color_check", "trn_indep_set", "trn_quality_cache", "trn_quality_stats", "trn_refine_filter",
     "trn_smooth_run",
     "trn_smooth_prepare",
@@ -105,6 +106,12 @@ def _load():
                                _P(_i64)], ctypes.c_int),
         "trn_mesh_set_color_tile": ([_vp, _i32], ctypes.c_int),
         "trn_mesh_set_class_order": ([_vp, _i32], ctypes.c_int),
+        "trn_medit_info": ([ctypes.c_char_p, _P(_i32), _P(_i32)], ctypes.c_int),
+        "trn_medit_read": ([ctypes.c_char_p, ctypes.c_char_p, _i32, _i32, _vp, _vp, _vp, _vp],
+                           ctypes.c_int),
+        "trn_trnm_info": ([ctypes.c_char_p, _P(_i32), _P(_i32)], ctypes.c_int),
+        "trn_trnm_read": ([_vp, ctypes.c_char_p, _i32, _i32, _vp, _vp, _vp, _vp], ctypes.c_int),
+        "trn_trnm_write": ([ctypes.c_char_p, _i32, _i32, _vp, _vp, _vp, _vp], ctypes.c_int),
         "trn_color_classes": ([_vp, _vp, _i32, _vp, _i32, _vp, _P(_i32)], ctypes.c_int),
         "trn_color_check": ([_vp, _vp, _vp, _i32, _P(_i64), _P(_i64)], ctypes.c_int),
         "trn_quality_cache": ([_vp, _vp, _vp, _vp, _vp], ctypes.c_int),
@@ -307,6 +314,68 @@ def _d2h_i32(ptr: int, n: int) -> np.ndarray:
         if rc != 0:
             raise TrinityError(TRN_ERR_HIP, f"hipMemcpy failed ({rc})")
     return out
+
+
+# ---------------------------------------------------------------------------
+# input
+
+def _np_ptr(a):
+    return _vp(a.ctypes.data) if a is not None else None
+
+
+def read_medit(mesh_path: str, bb_path: str | None = None):
+    """Mesh::load's parse of a .mesh (and .bb) file on the host: returns
+    (xy [nv,2] f64, tri [nt,3] int32 0-based, tags [nv] u8, solut [nv] f64 or
+    None) as numpy arrays."""
+    nv, nt = _i32(), _i32()
+    _check(lib().trn_medit_info(mesh_path.encode(), ctypes.byref(nv), ctypes.byref(nt)))
+    xy = np.empty((nv.value, 2), dtype=np.float64)
+    tri = np.empty((nt.value, 3), dtype=np.int32)
+    tags = np.empty(nv.value, dtype=np.uint8)
+    solut = np.empty(nv.value, dtype=np.float64) if bb_path else None
+    _check(lib().trn_medit_read(mesh_path.encode(), bb_path.encode() if bb_path else None,
+                                nv.value, nt.value, _np_ptr(xy), _np_ptr(tri), _np_ptr(tags),
+                                _np_ptr(solut)))
+    return xy, tri, tags, solut
+
+
+def trnm_info(path: str) -> tuple[int, int]:
+    nv, nt = _i32(), _i32()
+    _check(lib().trn_trnm_info(path.encode(), ctypes.byref(nv), ctypes.byref(nt)))
+    return nv.value, nt.value
+
+
+def read_trnm(path: str, ctx: "Context | None" = None, with_solut: bool = True):
+    """A TRNM file into host numpy arrays (ctx None) or device torch tensors
+    (ctx given): (xy, tri, tags, solut or None)."""
+    nv, nt = trnm_info(path)
+    if ctx is None:
+        xy = np.empty((nv, 2), dtype=np.float64)
+        tri = np.empty((nt, 3), dtype=np.int32)
+        tags = np.empty(nv, dtype=np.uint8)
+        solut = np.empty(nv, dtype=np.float64) if with_solut else None
+        _check(lib().trn_trnm_read(None, path.encode(), nv, nt, _np_ptr(xy), _np_ptr(tri),
+                                   _np_ptr(tags), _np_ptr(solut)))
+        return xy, tri, tags, solut
+    torch = _torch()
+    xy = torch.empty((nv, 2), dtype=torch.float64, device="cuda")
+    tri = torch.empty((nt, 3), dtype=torch.int32, device="cuda")
+    tags = torch.empty(nv, dtype=torch.uint8, device="cuda")
+    solut = torch.empty(nv, dtype=torch.float64, device="cuda") if with_solut else None
+    _check(lib().trn_trnm_read(ctx.handle, path.encode(), nv, nt, _vp(xy.data_ptr()),
+                               _vp(tri.data_ptr()), _vp(tags.data_ptr()),
+                               _vp(solut.data_ptr()) if solut is not None else None))
+    return xy, tri, tags, solut
+
+
+def write_trnm(path: str, xy, tri, tags, solut) -> None:
+    """Host arrays into a TRNM file."""
+    xy = np.ascontiguousarray(xy, dtype=np.float64)
+    tri = np.ascontiguousarray(tri, dtype=np.int32)
+    tags = np.ascontiguousarray(tags, dtype=np.uint8)
+    solut = np.ascontiguousarray(solut, dtype=np.float64)
+    _check(lib().trn_trnm_write(path.encode(), xy.shape[0], tri.size // 3, _np_ptr(xy),
+                                _np_ptr(tri), _np_ptr(tags), _np_ptr(solut)))
 
 
 # ---------------------------------------------------------------------------

@@ -33,8 +33,12 @@ CFLAGS = [
 
 
 def _compile(src: str) -> str:
-    obj = os.path.join(OBJ, os.path.basename(src).replace(".hip", ".o"))
-    cmd = [HIPCC, *CFLAGS, "-c", src, "-o", obj]
+    base = os.path.basename(src)
+    obj = os.path.join(OBJ, base.rsplit(".", 1)[0] + ("_host" if base.endswith(".cpp") else "")
+                       + ".o")
+    # .cpp sources are host code (file input); they include the runtime
+    # headers, so they go through hipcc as HIP too
+    cmd = [HIPCC, *CFLAGS, "-x", "hip", "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
@@ -43,7 +47,7 @@ def _compile(src: str) -> str:
 
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
-    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
     workers = min(len(srcs), int(os.environ.get("MAX_JOBS", "8")))
     with cf.ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
         objs = list(ex.map(_compile, srcs))

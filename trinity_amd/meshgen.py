@@ -347,29 +347,31 @@ def boundary_edges(tri: np.ndarray) -> list[tuple[int, int]]:
     return [(int(a), int(b)) for a, b in e[once]]
 
 
+def _write_rows(fh, fmt: str, rows: np.ndarray, chunk: int = 1 << 16) -> None:
+    """fmt % row for every row, formatted a chunk at a time in one call."""
+    for s in range(0, rows.shape[0], chunk):
+        part = rows[s:s + chunk]
+        fh.write((fmt * part.shape[0]) % tuple(part.reshape(-1).tolist()))
+
+
 def write_medit(mesh_path: str, bb_path: str, m: Mesh) -> None:
     """Write the .mesh/.bb pair parsed by Mesh::load (src/io.cpp:53-161)."""
     with open(mesh_path, "w") as fh:
         fh.write("MeshVersionFormatted 1\nDimension\n2\nVertices\n%d\n" % m.nv)
-        for (x, y) in m.xy:
-            fh.write("%.17g %.17g 1\n" % (x, y))
+        _write_rows(fh, "%.17g %.17g 1\n", np.asarray(m.xy, dtype=np.float64))
         # boundary edges: those of a single triangle
-        edges = boundary_edges(m.tri)
-        fh.write("Edges\n%d\n" % len(edges))
-        for a, b in edges:
-            fh.write("%d %d 1\n" % (a + 1, b + 1))
+        edges = np.asarray(boundary_edges(m.tri), dtype=np.int64).reshape(-1, 2)
+        fh.write("Edges\n%d\n" % edges.shape[0])
+        _write_rows(fh, "%d %d 1\n", edges + 1)
         fh.write("Triangles\n%d\n" % m.nt)
-        for t in m.tri:
-            fh.write("%d %d %d 1\n" % (t[0] + 1, t[1] + 1, t[2] + 1))
+        _write_rows(fh, "%d %d %d 1\n", np.asarray(m.tri, dtype=np.int64) + 1)
         corners = np.flatnonzero(m.tags & TAG_CORNER)
         fh.write("Corners\n%d\n" % corners.size)
-        for c in corners:
-            fh.write("%d\n" % (c + 1))
+        _write_rows(fh, "%d\n", corners + 1)
         fh.write("End\n")
     with open(bb_path, "w") as fh:
         fh.write("2 1 %d 2\n" % m.nv)
-        for v in m.solut:
-            fh.write("%.17g\n" % v)
+        _write_rows(fh, "%.17g\n", np.asarray(m.solut, dtype=np.float64))
 
 
 def read_medit(mesh_path: str, bb_path: str) -> Mesh:
