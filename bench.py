@@ -182,7 +182,7 @@ def free_gpu():
 
 def run_plain(T, mg, ctx, workload, p, sweeps, steps, warmup, seed, color_seed,
               locality=False, cctx=None, cstream=None, stat_steps=3,
-              timed_kernel="smooth_phase", create_reps=0):
+              timed_kernel="smooth_phase", create_reps=0, autotune=False):
     """Times `steps` passes (coordinates restored, metric, colouring, sweeps)
     and returns the wall time per pass and the per-kernel statistics.
 
@@ -206,6 +206,12 @@ def run_plain(T, mg, ctx, workload, p, sweeps, steps, warmup, seed, color_seed,
     dm = T.DeviceMesh(ctx, m.nv, m.tri, m.tags, xy=xy0)
     if locality:
         T.set_class_order(dm, True)
+    tuned = None
+    if autotune:
+        t0 = time.perf_counter()
+        tuned = T.autotune(ctx, dm, xy0, torch.from_numpy(m.solut).cuda(), p)
+        tuned["tuner_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        log(f"{workload}: autotuned shapes {tuned}")
     f = torch.from_numpy(m.solut).cuda()
     xy = torch.empty_like(xy0)
     tensor = torch.empty((m.nv, 3), dtype=torch.float64, device="cuda")
@@ -283,7 +289,7 @@ def run_plain(T, mg, ctx, workload, p, sweeps, steps, warmup, seed, color_seed,
     free_gpu()
     return {"nv": nv, "nt": nt, "ms": ms, "stats": stats, "timed_stats": timed_stats,
             "colours": int(np.median(ncolors)), "steps": steps, "stat_steps": stat_steps,
-            "colouring_overlapped": cctx is not None, "create_ms": create_ms}
+            "colouring_overlapped": cctx is not None, "create_ms": create_ms, "autotune": tuned}
 
 
 def summarize(r, sweeps, label=None):
@@ -781,6 +787,8 @@ def main():
     ap.add_argument("--cpu-workload", default=None,
                     help="mesh of the CPU baseline (default: the headline's)")
     ap.add_argument("--cpu-reps", type=int, default=1)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes instead of tuning them on the mesh")
     ap.add_argument("--ingest", action="store_true",
                     help="also time file input to a device mesh (text vs binary) on the "
                          "workload, and the reference's Mesh::load beside it")
@@ -815,7 +823,8 @@ def main():
         cctx = T.Context(0, use_torch_stream=False)
         cctx.set_stream(cstream.cuda_stream)
     r = run_plain(T, mg, ctx, args.workload, args.p, args.sweeps, args.steps, args.warmup,
-                  args.seed, args.color_seed, args.locality_order, cctx, cstream, create_reps=3)
+                  args.seed, args.color_seed, args.locality_order, cctx, cstream, create_reps=3,
+                  autotune=not args.no_autotune)
     ms = r["ms"]
     value = r["nv"] / (ms * 1e-3) / 1e6
     kern, stages = summarize(r, args.sweeps, f"{args.workload} p={args.p}")
@@ -893,10 +902,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "vertices": r["nv"], "triangles": r["nt"],
                    "lp_norm": args.p, "target_factor": 0.8, "h_min": 1e-9, "h_max": 0.4,
-                   "sweeps": args.sweeps, "colouring": "gpu seeded speculative greedy" +
+                   "sweeps": args.sweeps,
+                   "colouring": "gpu tile-sequential first fit, seams repaired by seeded "
+                                "speculative rounds" +
                    ("" if args.serial_colouring else ", on a second stream beside the metric"),
-                   "parallelism": "single GPU"},
+                   "patch_slabs": "z-curve", "parallelism": "single GPU"},
         "roofline": roof,
+        "autotune": r["autotune"],
         "stages": stages,
         "extras": extras,
         "cpu_baseline": cpu,

@@ -200,6 +200,24 @@ int trn_color_partial(trn_ctx* ctx, const trn_mesh* mesh, uint64_t seed,
  * multiple of 64 in 64..16384.  A tile shorter than the longest edge leaves
  * conflicts to the repair rounds; a longer one runs longer. */
 int trn_mesh_set_color_tile(trn_mesh* mesh, int32_t tile);
+/* ---- launch shapes: the autotuner ------------------------------------------
+ * No counterpart in the reference (its loops have no launch shapes).
+ * trn_mesh_autotune measures, on this mesh with the caller's coordinates and
+ * field (lp_norm for the metric the smoothing runs on), the slabs per block
+ * (1, 2, 4) of the gradient and Hessian gathers and the register shape (2,
+ * 3, 4 waves per SIMD) of the smoothing phase kernel (on the largest colour
+ * class, first sweep), and keeps the fastest in the mesh for every later
+ * call.  shapes [3] (may be NULL) receives {gradient slabs, Hessian slabs,
+ * phase waves}; times_ms [6] (may be NULL) the default and the chosen
+ * shape's time of each of the three.  No shape changes a result; the
+ * mesh's smoothing records are restored.  Per-kernel timing must be off.
+ * Synchronises.  trn_mesh_shapes / trn_mesh_set_shapes read and set the
+ * same three values (defaults 1, 1, 3). */
+int trn_mesh_autotune(trn_ctx* ctx, trn_mesh* mesh, const double* xy_d, const double* solut_d,
+                      int32_t lp_norm, int32_t* shapes, double* times_ms);
+int trn_mesh_shapes(const trn_mesh* mesh, int32_t* shapes);
+int trn_mesh_set_shapes(trn_mesh* mesh, const int32_t* shapes);
+
 /* Colour classes of an externally supplied colouring (colour 0 = not
  * coloured), e.g. the reference's own.  class_off as above.  mesh may be
  * NULL (classes in index order) or the mesh whose locality order to use.

@@ -658,3 +658,43 @@ def test_colouring_degree12_palette13(ctx, wl):
         assert 1 <= a.ncolors <= 13
         print(wl, "tile", tile, "colours", a.ncolors)
     dm.close()
+
+
+@pytest.mark.parametrize("wl", ["jitter256:layer", "hub128:fronts"])
+def test_launch_shapes_do_not_change_results(ctx, wl):
+    """Every launch shape the autotuner chooses from (gather slabs per block,
+    phase register shape) gives bit-identical metric, coordinates, qualities
+    and counters; the tuner keeps valid shapes and restores the records."""
+    import torch
+    m = mg.workload(wl)
+    xy0, f = dev(m.xy), dev(m.solut)
+    dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags, xy=xy0)
+    col = T().color(ctx, dm, seed=3, max_colors=16)
+
+    def run():
+        xy = xy0.clone()
+        ten, c = T().metric_run(ctx, dm, xy, f, 0.8, 2, 1e-9, 0.4)
+        q = torch.empty(m.nt, dtype=torch.float64, device="cuda")
+        com, tas = T().smooth_run(ctx, dm, xy, ten, q, col, 2)
+        return ten.cpu(), c, xy.cpu(), q.cpu(), com, tas
+
+    assert T().mesh_shapes(dm) == (1, 1, 3)
+    base = run()
+    for shapes in ((2, 4, 2), (4, 2, 4), (1, 1, 3)):
+        T().set_shapes(dm, shapes)
+        assert T().mesh_shapes(dm) == shapes
+        r = run()
+        for a, b in zip(base, r):
+            assert (torch.equal(a, b) if isinstance(a, torch.Tensor) else a == b), shapes
+    tuned = T().autotune(ctx, dm, xy0, f, 2)
+    sh = tuple(tuned["shapes"].values())
+    assert sh == T().mesh_shapes(dm)
+    assert sh[0] in (1, 2, 4) and sh[1] in (1, 2, 4) and sh[2] in (2, 3, 4)
+    for k, t in tuned["ms"].items():
+        assert 0 < t["tuned"] <= t["default"], (k, t)
+    r = run()
+    for a, b in zip(base, r):
+        assert (torch.equal(a, b) if isinstance(a, torch.Tensor) else a == b)
+    with pytest.raises(T().TrinityError):
+        T().set_shapes(dm, (3, 1, 3))
+    dm.close()

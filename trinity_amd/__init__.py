@@ -40,6 +40,7 @@ EXPORTS = [
     "trn_complexity", "trn_normalize_global", "trn_color", "trn_color_partial",
     "trn_mesh_set_color_tile", "trn_mesh_set_class_order", "trn_color_classes",
     "trn_medit_info", "trn_medit_read", "trn_trnm_info", "trn_trnm_read", "trn_trnm_write",
+    "trn_mesh_autotune", "trn_mesh_shapes", "trn_mesh_set_shapes",
     "trn_color_check", "trn_indep_set", "trn_quality_cache", "trn_quality_stats", "trn_refine_filter",
     "trn_smooth_run",
     "trn_smooth_prepare",
@@ -106,6 +107,9 @@ def _load():
                                _P(_i64)], ctypes.c_int),
         "trn_mesh_set_color_tile": ([_vp, _i32], ctypes.c_int),
         "trn_mesh_set_class_order": ([_vp, _i32], ctypes.c_int),
+        "trn_mesh_autotune": ([_vp, _vp, _vp, _vp, _i32, _P(_i32), _P(_f64)], ctypes.c_int),
+        "trn_mesh_shapes": ([_vp, _P(_i32)], ctypes.c_int),
+        "trn_mesh_set_shapes": ([_vp, _P(_i32)], ctypes.c_int),
         "trn_medit_info": ([ctypes.c_char_p, _P(_i32), _P(_i32)], ctypes.c_int),
         "trn_medit_read": ([ctypes.c_char_p, ctypes.c_char_p, _i32, _i32, _vp, _vp, _vp, _vp],
                            ctypes.c_int),
@@ -474,6 +478,33 @@ def color_partial(ctx, mesh, seed=1, max_colors=7) -> tuple[Coloring, int]:
 def set_color_tile(mesh, tile: int) -> None:
     """Tile size of the colouring's first step (0 = chosen from nv)."""
     _check(lib().trn_mesh_set_color_tile(mesh.handle, int(tile)))
+
+
+SHAPE_NAMES = ("gradient_slabs_per_block", "hessian_slabs_per_block", "phase_waves_per_simd")
+
+
+def autotune(ctx, mesh, xy, solut, lp_norm=2) -> dict:
+    """trn_mesh_autotune: the launch shapes it keeps, and per tuned kernel
+    the default shape's time and the chosen one's (ms)."""
+    sh = (_i32 * 3)()
+    tm = (_f64 * 6)()
+    _check(lib().trn_mesh_autotune(ctx.handle, mesh.handle, _vp(xy.data_ptr()),
+                                   _vp(solut.data_ptr()), int(lp_norm), sh, tm))
+    kernels = ("vtx_grad", "vtx_hess_norm", "smooth_phase")
+    return {"shapes": dict(zip(SHAPE_NAMES, list(sh))),
+            "ms": {k: {"default": round(tm[2 * i], 4), "tuned": round(tm[2 * i + 1], 4)}
+                   for i, k in enumerate(kernels)}}
+
+
+def mesh_shapes(mesh) -> tuple[int, int, int]:
+    sh = (_i32 * 3)()
+    _check(lib().trn_mesh_shapes(mesh.handle, sh))
+    return tuple(sh)
+
+
+def set_shapes(mesh, shapes) -> None:
+    sh = (_i32 * 3)(*[int(x) for x in shapes])
+    _check(lib().trn_mesh_set_shapes(mesh.handle, sh))
 
 
 def set_class_order(mesh, zcurve: bool) -> None:

@@ -21,6 +21,7 @@
 // the triangle, so computing them once per triangle and summing the stored
 // products in patch order reproduces the reference's roundings exactly.
 // The element bodies live in trn_recovery.hpp (shared with batch.hip).
+#include <algorithm>
 #include <cmath>
 
 #include "trn_recovery.hpp"
@@ -43,12 +44,13 @@ __global__ __launch_bounds__(256) void k_tri_grad(int32_t nt, const int32_t* __r
 
 static_assert(kGatherB == kPatchSlab, "a gather wave reads one slab of stored patches");
 
-// The gathers run one wave per slab, up to kGatherWaves slabs per block
-// (as many as fit kGatherLdsMax bytes of LDS); each wave stages its slab's
-// union in its own LDS region and waits only at the block's one barrier.
-// One slab per block measured fastest (4 per block: 1.19 -> 1.21 ms for the
-// Hessian gather of a 4096^2 grid, 0.86 -> 0.95 ms for the gradient's).
-constexpr int kGatherWaves = 1;
+// The gathers run one wave per slab, 1, 2 or 4 slabs per block (the mesh's
+// tuned shape, as many as fit kGatherLdsMax bytes of LDS); each wave stages
+// its slab's union in its own LDS region and waits only at the block's one
+// barrier.  One slab per block is the default (4 per block: 1.19 -> 1.21 ms
+// for the Hessian gather of a 4096^2 grid, 0.86 -> 0.95 ms for the
+// gradient's).
+constexpr int kGatherWaves = 4;  // the most slabs per block
 constexpr size_t kGatherLdsMax = 64 * 1024;
 
 // lds[k] = src[U[k]] for the nu union entries of a slab, by its 64 lanes in
@@ -373,11 +375,11 @@ struct GatherShape {
   dim3 grid, block;
 };
 
-static GatherShape gather_shape(const trn_mesh* m, size_t per_entry) {
+static GatherShape gather_shape(const trn_mesh* m, size_t per_entry, int waves) {
   GatherShape g;
   g.nslab = (int)nblk(m->nv, kPatchSlab);
   g.wave_bytes = (int)(((per_entry * (size_t)m->max_union) + 15) & ~(size_t)15);
-  g.waves = kGatherWaves;
+  g.waves = std::min(std::max(waves, 1), kGatherWaves);
   while (g.waves > 1 && (size_t)g.waves * g.wave_bytes > kGatherLdsMax) g.waves >>= 1;
   g.shmem = (size_t)g.waves * g.wave_bytes;
   g.grid = dim3(nblk(g.nslab, g.waves));
@@ -390,8 +392,8 @@ static GatherShape gather_shape(const trn_mesh* m, size_t per_entry) {
 static void launch_vtx_grad(trn_ctx* ctx, const char* name, double bytes, bool with_grad,
                             bool fused, const trn_mesh* m, const double* xy, const double* f,
                             const double* area_t, const double2* ag_t, double2* grad,
-                            double* tarea) {
-  const GatherShape g = gather_shape(m, sizeof(double2) + sizeof(double));
+                            double* tarea, int waves) {
+  const GatherShape g = gather_shape(m, sizeof(double2) + sizeof(double), waves);
   auto kern = fused ? k_vtx_grad<true, true>
                     : (with_grad ? k_vtx_grad<true, false> : k_vtx_grad<false, false>);
   launch(ctx, name, bytes, kern, g.grid, g.block, g.shmem, m->nv, g.nslab, m->slab_v,
@@ -404,7 +406,7 @@ static void launch_vtx_grad(trn_ctx* ctx, const char* name, double bytes, bool w
 
 void patch_area_sums(trn_ctx* ctx, const trn_mesh* m, const double* area_t, double* tarea) {
   launch_vtx_grad(ctx, "vtx_area", 0.0, false, false, m, nullptr, nullptr, area_t, nullptr,
-                  nullptr, tarea);
+                  nullptr, tarea, m->tune.grad_waves);
 }
 
 // The per-triangle kernels run only for meshes with slabs too large for a
@@ -415,7 +417,21 @@ static void run_gradient(trn_ctx* ctx, const trn_mesh* m, const double* xy, cons
     launch(ctx, "tri_grad", 0.0, k_tri_grad, dim3(nblk(m->nt, 256)), dim3(256), 0, m->nt,
            (const int32_t*)m->tri, (const double2*)xy, f, b.area_t, b.ag_t);
   launch_vtx_grad(ctx, "vtx_grad", kBytesGradPerV * m->nv, true, true, m, xy, f, b.area_t,
-                  b.ag_t, (double2*)grad, b.tarea);
+                  b.ag_t, (double2*)grad, b.tarea, m->tune.grad_waves);
+}
+
+static void launch_vtx_hess(trn_ctx* ctx, const trn_mesh* m, const double* xy,
+                            const double* grad, const double4* ah_t, const double* tarea,
+                            double* hess, double* out, bool normalize, double ex, int waves) {
+  const GatherShape g = gather_shape(m, sizeof(double4), waves);
+  auto kern = normalize ? k_vtx_hess<true> : k_vtx_hess<false>;
+  launch(ctx, normalize ? "vtx_hess_norm" : "vtx_hess", kBytesHessPerV * m->nv, kern, g.grid,
+         g.block, g.shmem, m->nv, g.nslab, m->slab_v, (const int32_t*)m->patch_n,
+         (const int32_t*)m->patch_row, (const int32_t*)m->patch_ell,
+         (const int32_t*)m->union_n, (const int32_t*)m->union_off,
+         (const int32_t*)m->union_ids, (const uint16_t*)m->patch_local, g.wave_bytes,
+         (const int32_t*)m->tri, (const double2*)xy, (const double2*)grad, ah_t, tarea, hess,
+         out, ex);
 }
 
 static void run_hessian(trn_ctx* ctx, const trn_mesh* m, const double* xy, const double* grad,
@@ -423,15 +439,46 @@ static void run_hessian(trn_ctx* ctx, const trn_mesh* m, const double* xy, const
   if (!m->union_all)
     launch(ctx, "tri_hess", 0.0, k_tri_hess, dim3(nblk(m->nt, 256)), dim3(256), 0, m->nt,
            (const int32_t*)m->tri, (const double2*)xy, (const double2*)grad, b.ah_t);
-  const GatherShape g = gather_shape(m, sizeof(double4));
-  auto kern = normalize ? k_vtx_hess<true> : k_vtx_hess<false>;
-  launch(ctx, normalize ? "vtx_hess_norm" : "vtx_hess", kBytesHessPerV * m->nv, kern, g.grid,
-         g.block, g.shmem, m->nv, g.nslab, m->slab_v, (const int32_t*)m->patch_n,
-         (const int32_t*)m->patch_row, (const int32_t*)m->patch_ell,
-         (const int32_t*)m->union_n, (const int32_t*)m->union_off,
-         (const int32_t*)m->union_ids, (const uint16_t*)m->patch_local, g.wave_bytes,
-         (const int32_t*)m->tri, (const double2*)xy, (const double2*)grad,
-         (const double4*)b.ah_t, (const double*)b.tarea, hess, out, ex);
+  launch_vtx_hess(ctx, m, xy, grad, b.ah_t, b.tarea, hess, out, normalize, ex,
+                  m->tune.hess_waves);
+}
+
+// ---- shapes the autotuner measures ----------------------------------------
+
+size_t tune_gather_scratch(const trn_mesh* m) {
+  return scratch_bytes(sizeof(double) * m->nt) + scratch_bytes(sizeof(double2) * m->nt) +
+         scratch_bytes(sizeof(double4) * m->nt) + scratch_bytes(sizeof(double) * m->nv) +
+         scratch_bytes(sizeof(double2) * m->nv) + scratch_bytes(sizeof(double) * 3 * m->nv);
+}
+
+// One gradient and one Hessian gather (with the local normalisation) at the
+// given shapes, bracketed by ev[0..1] and ev[2..3]; the per-triangle
+// kernels that meshes with union-less slabs need run outside the brackets.
+void tune_gathers(trn_ctx* ctx, const trn_mesh* m, const double* xy, const double* f,
+                  int lp_norm, void* scratch, size_t bytes, int grad_waves, int hess_waves,
+                  hipEvent_t ev[4]) {
+  Carve c(scratch, bytes);
+  GradBuffers b;
+  b.area_t = c.carve<double>(m->nt);
+  b.ag_t = c.carve<double2>(m->nt);
+  b.ah_t = c.carve<double4>(m->nt);
+  b.tarea = c.carve<double>(m->nv);
+  double2* grad = c.carve<double2>(m->nv);
+  double* tensor = c.carve<double>(3 * (size_t)m->nv);
+  if (!m->union_all)
+    launch(ctx, "tri_grad", 0.0, k_tri_grad, dim3(nblk(m->nt, 256)), dim3(256), 0, m->nt,
+           (const int32_t*)m->tri, (const double2*)xy, f, b.area_t, b.ag_t);
+  TRN_HIP(hipEventRecord(ev[0], ctx->stream));
+  launch_vtx_grad(ctx, "vtx_grad", 0.0, true, true, m, xy, f, b.area_t, b.ag_t, grad, b.tarea,
+                  grad_waves);
+  TRN_HIP(hipEventRecord(ev[1], ctx->stream));
+  if (!m->union_all)
+    launch(ctx, "tri_hess", 0.0, k_tri_hess, dim3(nblk(m->nt, 256)), dim3(256), 0, m->nt,
+           (const int32_t*)m->tri, (const double2*)xy, (const double2*)grad, b.ah_t);
+  TRN_HIP(hipEventRecord(ev[2], ctx->stream));
+  launch_vtx_hess(ctx, m, xy, (const double*)grad, b.ah_t, b.tarea, nullptr, tensor, true,
+                  metric_exponent(lp_norm), hess_waves);
+  TRN_HIP(hipEventRecord(ev[3], ctx->stream));
 }
 
 static void run_complexity(trn_ctx* ctx, const trn_mesh* m, const double* xy,
@@ -529,7 +576,7 @@ int trn_hessian(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d, const do
            (const int32_t*)mesh->tri, (const double2*)xy_d, (const double*)nullptr, b.area_t,
            b.ag_t);
     launch_vtx_grad(ctx, "vtx_area", 0.0, false, false, mesh, xy_d, nullptr, b.area_t, b.ag_t,
-                    nullptr, b.tarea);
+                    nullptr, b.tarea, mesh->tune.grad_waves);
     run_hessian(ctx, mesh, xy_d, gradient_d, nullptr, tensor_d, false, 0.0, b);
   });
 }

@@ -90,8 +90,10 @@ __global__ __launch_bounds__(256) void k_unpack_records(int32_t nv,
 // (the whole class); otherwise it is a live list of *count_d <= count
 // vertices written by k_class_live, the grid is sized for `count`, and the
 // blocks past the live ones exit at once while the others take XCD-chunked
-// ranges of the live blocks.
-__global__ __launch_bounds__(64, 3) void k_smooth_phase(
+// ranges of the live blocks.  MINW: waves per SIMD the register allocation
+// is made for (2, 3 or 4; the mesh's tuned shape), the same arithmetic.
+template <int MINW>
+__global__ __launch_bounds__(64, MINW) void k_smooth_phase(
     int32_t count, const int32_t* __restrict__ count_d, const int32_t* __restrict__ order,
     VRecord* __restrict__ rec, double* __restrict__ qualit, uint8_t* __restrict__ stale,
     const int32_t* __restrict__ v2t_off, const int32_t* __restrict__ v2t,
@@ -215,6 +217,30 @@ static void phase_shape(const trn_mesh* m, int* block, size_t* shmem) {
   while (b > 16 && (size_t)b * per_thread * sizeof(double) > 65536) b >>= 1;
   *block = b;
   *shmem = (size_t)b * per_thread * sizeof(double);
+}
+
+using PhaseKernel = void (*)(int32_t, const int32_t*, const int32_t*, VRecord*, double*,
+                            uint8_t*, const int32_t*, const int32_t*, const int32_t*,
+                            const int32_t*, const int32_t*, int32_t*, unsigned long long*);
+
+static PhaseKernel phase_kernel(int minw) {
+  return minw <= 2 ? k_smooth_phase<2> : (minw >= 4 ? k_smooth_phase<4> : k_smooth_phase<3>);
+}
+
+// One first-sweep phase over `count` vertices of `list` at the given shape,
+// bracketed by ev[0..1] (the autotuner's measurement).
+void tune_phase(trn_ctx* ctx, const trn_mesh* m, int32_t count, const int32_t* list,
+                double* qualit, uint8_t* stale, int32_t* counters, int minw, hipEvent_t ev[2]) {
+  int block;
+  size_t shmem;
+  phase_shape(m, &block, &shmem);
+  TRN_HIP(hipEventRecord(ev[0], ctx->stream));
+  launch(ctx, "smooth_phase", 0.0, phase_kernel(minw), dim3(nblk(count, block)), dim3(block),
+         shmem, count, (const int32_t*)nullptr, list, m->vrec, qualit, stale,
+         (const int32_t*)m->v2t_off, (const int32_t*)m->v2t, (const int32_t*)m->star,
+         (const int32_t*)m->v2v_off, (const int32_t*)m->v2v, counters,
+         (unsigned long long*)nullptr);
+  TRN_HIP(hipEventRecord(ev[1], ctx->stream));
 }
 
 static void tensor_logs(trn_ctx* ctx, const trn_mesh* m, const double* tensor, double4* logm) {
@@ -361,10 +387,11 @@ int32_t* smooth_sweeps_launch(trn_ctx* ctx, const trn_mesh* m, double* xy, doubl
       // on the device only, so its launch reports it as work units
       unsigned long long* units =
           count_d ? timing_units(ctx, "smooth_phase", kBytesSweepPerV) : nullptr;
-      launch(ctx, "smooth_phase", count_d ? 0.0 : kBytesSweepPerV * count, k_smooth_phase,
-             dim3(nblk(count, block)), dim3(block), shmem, count, count_d, list, rec, qualit,
-             stale, (const int32_t*)m->v2t_off, (const int32_t*)m->v2t, (const int32_t*)m->star,
-             (const int32_t*)m->v2v_off, (const int32_t*)m->v2v, cs, units);
+      launch(ctx, "smooth_phase", count_d ? 0.0 : kBytesSweepPerV * count,
+             phase_kernel(m->tune.phase_minw), dim3(nblk(count, block)), dim3(block), shmem,
+             count, count_d, list, rec, qualit, stale, (const int32_t*)m->v2t_off,
+             (const int32_t*)m->v2t, (const int32_t*)m->star, (const int32_t*)m->v2v_off,
+             (const int32_t*)m->v2v, cs, units);
     }
   }
   launch(ctx, "unpack_records", 80.0 * m->nv, k_unpack_records, dim3(nblk(m->nv, 256)),

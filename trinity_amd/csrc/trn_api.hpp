@@ -63,6 +63,18 @@ int32_t* smooth_sweeps_launch(trn_ctx* ctx, const trn_mesh* m, double* xy, doubl
                               int ncolors, int sweeps, int max_class, void* scratch,
                               size_t scratch_size, bool want_totals);
 
+// Shapes the autotuner (tune.hip) measures.  metric.hip: one gradient and
+// one Hessian gather at the given slabs per block, bracketed by ev[0..1] and
+// ev[2..3]; scratch of tune_gather_scratch bytes.  smoothing.hip: one
+// first-sweep phase over a class at the given register shape, bracketed by
+// ev[0..1]; the mesh's records must be prepared.
+size_t tune_gather_scratch(const trn_mesh* m);
+void tune_gathers(trn_ctx* ctx, const trn_mesh* m, const double* xy, const double* f,
+                  int lp_norm, void* scratch, size_t bytes, int grad_waves, int hess_waves,
+                  hipEvent_t ev[4]);
+void tune_phase(trn_ctx* ctx, const trn_mesh* m, int32_t count, const int32_t* list,
+                double* qualit, uint8_t* stale, int32_t* counters, int minw, hipEvent_t ev[2]);
+
 // Reductions used by several passes.
 void reduce_max_i32(trn_ctx* ctx, const int32_t* in, int64_t n, int32_t* out_d);
 

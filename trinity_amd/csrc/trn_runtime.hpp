@@ -163,6 +163,14 @@ struct trn_mesh {
   uint16_t* patch_local = nullptr;   // [patch_row[nslab] * kPatchSlab]
   int32_t max_union = 0;             // largest union_n
   bool union_all = false;            // every slab has a union
+  // Launch shapes of the plain kernels (trn_mesh_autotune measures them on
+  // the mesh): slabs per block of the gradient and Hessian gathers, and the
+  // waves per SIMD the smoothing phase kernel is compiled for (its register
+  // budget).  None of them changes a result.
+  struct {
+    int32_t grad_waves = 1, hess_waves = 1, phase_minw = 3;
+    bool tuned = false;
+  } tune;
   int32_t color_tile = 0;            // colouring tile size (0: chosen from bandwidth)
   int32_t bandwidth = 0;             // largest |u - v| over the edges
 };
