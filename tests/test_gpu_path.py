@@ -262,6 +262,62 @@ def test_colouring_tiles_few_colours(ctx, wl):
     dm.close()
 
 
+def _tile_first_fit(nv, off, adj, tile, max_colors):
+    """The tile-sequential first fit in plain Python: even tiles first, each
+    vertex in index order taking the smallest colour absent among its
+    earlier neighbours in its tile; then odd tiles, which also avoid their
+    neighbours in even tiles.  -1 marks a vertex past the palette, 0 an
+    isolated one."""
+    col = np.zeros(nv, dtype=np.int64)
+    ntiles = (nv + tile - 1) // tile
+    for parity in (0, 1):
+        for tb in range(parity, ntiles, 2):
+            t0, t1 = tb * tile, min(tb * tile + tile, nv)
+            for v in range(t0, t1):
+                nb = adj[off[v]:off[v + 1]]
+                if nb.size == 0:
+                    continue
+                used = set()
+                for u in nb:
+                    if t0 <= u < v or (parity and not t0 <= u < t1 and (u // tile) % 2 == 0):
+                        if col[u] > 0:
+                            used.add(int(col[u]))
+                c = 1
+                while c in used:
+                    c += 1
+                col[v] = c if c <= max_colors else -1
+    return col
+
+
+@pytest.mark.parametrize("wl", ["jitter40:fronts", "jitter150:fronts", "flip120:fronts",
+                                "hub96:fronts"])
+def test_colouring_tiles_equal_sequential_model(ctx, wl):
+    """With tiles longer than every edge of the numbering (no seams), the
+    colouring is exactly the tile-sequential first fit: pinned against a
+    plain sequential restatement, for rows shorter than a 64-vertex chunk
+    (chunks with several earlier in-chunk neighbours: the scalar walk) and
+    longer (one earlier in-chunk neighbour: the pointer-jumping chains),
+    on grids, flipped diagonals and degree-12 hubs, with a palette that
+    fits and one that does not (partial colouring)."""
+    m = mg.workload(wl)
+    dm = T().DeviceMesh(ctx, m.nv, m.tri, m.tags)
+    topo = oracle.Topology(m.nv, m.tri)
+    span = max(int(np.abs(topo.v2v[topo.v2v_off[v]:topo.v2v_off[v + 1]] - v).max(initial=0))
+               for v in range(m.nv))
+    for tile in (0, (span // 64 + 1) * 64, (span // 64 + 3) * 64):
+        T().set_color_tile(dm, tile)
+        t = tile if tile else max(1024, (span // 64 + 1) * 64)
+        want = _tile_first_fit(m.nv, topo.v2v_off, topo.v2v, t, 64)
+        col = T().color(ctx, dm, seed=1, max_colors=64)
+        assert np.array_equal(col.color.cpu().numpy(), want), (wl, tile)
+        want3 = _tile_first_fit(m.nv, topo.v2v_off, topo.v2v, t, 3)
+        part, unc = T().color_partial(ctx, dm, seed=1, max_colors=3)
+        got3 = part.color.cpu().numpy()
+        assert np.array_equal(got3, np.where(want3 < 0, 0, want3)), (wl, tile)
+        assert unc == int((want3 < 0).sum())
+    dm.close()
+
+
 def test_colour_partial_leaves_vertices_uncoloured(ctx):
     """With a palette too small for the 60-spoke hub, the reference leaves a
     vertex uncoloured (partition.cpp:104-110) and never smooths it

@@ -200,14 +200,31 @@ __device__ __forceinline__ uint64_t lane_u64(const uint32_t lo, const uint32_t h
 // Per 64-vertex chunk every lane gathers, for its vertex, the mask of
 // colours its earlier neighbours outside the chunk hold (in-tile ones from
 // the tile's colours in LDS, even-tile ones from memory) and the mask of
-// its earlier neighbours inside the chunk; then the wave walks the chunk
-// in order on the scalar unit: vertex k takes the smallest colour absent
-// from its outside mask and from the colours of its in-chunk neighbours,
-// read from their lanes (readlane), and is recorded in lane k.
+// its earlier neighbours inside the chunk; then the chunk is coloured in
+// index order, in one of two ways with the same result:
+//   chains: when no vertex of the chunk has two earlier neighbours inside
+//           it (a mesh numbered row by row: the only one is v - 1), the
+//           colour of vertex k is a function of its in-chunk parent's
+//           colour x alone, x == m1 ? m2 : m1, with m1 and m2 the two
+//           smallest colours absent from its outside mask.  Such functions
+//           compose into the same form, so pointer jumping over the parent
+//           links resolves the chunk in at most 7 shuffle rounds;
+//   walk:   otherwise the wave walks the chunk in order on the scalar unit:
+//           vertex k takes the smallest colour absent from its outside
+//           mask and from the colours of its in-chunk neighbours, read
+//           from their lanes (readlane), and is recorded in lane k.
 // Isolated vertices stay 0.  A vertex that finds colours 1..max_colors
 // taken is written -1 and counted in *given_up.  Edges between tiles of
 // the same parity (only where the numbering has edges longer than a tile)
 // are left to the seam check.
+constexpr uint32_t kGaveUp = 0xFF;
+
+// The smallest colour absent from `used`, or kGaveUp past the palette.
+__device__ __forceinline__ uint32_t first_free(const uint64_t used, const int max_colors) {
+  const int c = __builtin_ffsll((long long)~used);
+  return (c == 0 || c > max_colors) ? kGaveUp : (uint32_t)c;
+}
+
 __global__ __launch_bounds__(kTileB) void k_color_tiles(int32_t nv, int32_t tile, int parity,
                                                         const int32_t* __restrict__ v2v_off,
                                                         const int32_t* __restrict__ v2v,
@@ -241,33 +258,64 @@ __global__ __launch_bounds__(kTileB) void k_color_tiles(int32_t nv, int32_t tile
         if (cu > 0) far |= 1ull << (cu - 1);
       }
     }
-    const uint64_t iso = __ballot(v < t1 && b == e);
-    const uint32_t near_lo = (uint32_t)near, near_hi = (uint32_t)(near >> 32);
-    const uint32_t far_lo = (uint32_t)far, far_hi = (uint32_t)(far >> 32);
-    uint32_t oh_lo = 0, oh_hi = 0;  // one-hot colour of this lane's vertex
+    const bool live = v < t1 && b != e;  // coloured here (not isolated, inside the tile)
     int mine = 0;
-    const int n = (int)(t1 - c0 < kTileB ? t1 - c0 : kTileB);
-    for (int k = 0; k < n; ++k) {  // uniform: the scalar unit walks the chunk
-      int c = 0;
-      if (!((iso >> k) & 1)) {
-        uint64_t used = lane_u64(far_lo, far_hi, k);
-        uint64_t nb = lane_u64(near_lo, near_hi, k);
-        while (nb) {
-          const int j = __builtin_ctzll(nb);
-          nb &= nb - 1;
-          used |= lane_u64(oh_lo, oh_hi, j);
-        }
-        c = __builtin_ffsll((long long)~used);
-        if (c == 0 || c > max_colors) {
-          c = 0xFF;
-          ++nfail;
+    if (__ballot(__popcll(near) > 1) == 0) {
+      // chains: g(x) = x == a ? b : c applied to the colour x of lane q
+      const uint32_t m1 = first_free(far, max_colors);
+      const uint32_t m2 = m1 == kGaveUp ? kGaveUp
+                                        : first_free(far | (1ull << (m1 - 1)), max_colors);
+      uint32_t ga = m1, gb = m2, gc = m1;
+      int q = near ? __builtin_ctzll(near) : -1;
+      bool done = !live || q < 0;
+      uint32_t x = live ? m1 : 0u;  // the colour, once done
+      while (__ballot(!done)) {
+        const uint32_t word = done ? (0x80000000u | x)
+                                   : (ga | (gb << 8) | (gc << 16) | ((uint32_t)q << 24));
+        const uint32_t other = (uint32_t)__shfl((int)word, done ? lane : q, kTileB);
+        if (!done) {
+          if (other & 0x80000000u) {
+            const uint32_t xp = other & 0xFFu;
+            x = xp == ga ? gb : gc;
+            done = true;
+          } else {
+            const uint32_t oa = other & 0xFFu, ob = (other >> 8) & 0xFFu,
+                           oc = (other >> 16) & 0xFFu;
+            const uint32_t nb = ob == ga ? gb : gc, nc = oc == ga ? gb : gc;
+            ga = oa;
+            gb = nb;
+            gc = nc;
+            q = (int)((other >> 24) & 63u);
+          }
         }
       }
-      const uint64_t o = (c != 0 && c != 0xFF) ? 1ull << (c - 1) : 0ull;
-      if (lane == k) {  // record vertex k's colour in its lane
-        oh_lo = (uint32_t)o;
-        oh_hi = (uint32_t)(o >> 32);
-        mine = c;
+      mine = (int)x;
+      nfail += __popcll(__ballot(live && x == kGaveUp));
+    } else {
+      const uint64_t iso = __ballot(v < t1 && b == e);
+      const uint32_t near_lo = (uint32_t)near, near_hi = (uint32_t)(near >> 32);
+      const uint32_t far_lo = (uint32_t)far, far_hi = (uint32_t)(far >> 32);
+      uint32_t oh_lo = 0, oh_hi = 0;  // one-hot colour of this lane's vertex
+      const int n = (int)(t1 - c0 < kTileB ? t1 - c0 : kTileB);
+      for (int k = 0; k < n; ++k) {  // uniform: the scalar unit walks the chunk
+        int c = 0;
+        if (!((iso >> k) & 1)) {
+          uint64_t used = lane_u64(far_lo, far_hi, k);
+          uint64_t nb = lane_u64(near_lo, near_hi, k);
+          while (nb) {
+            const int j = __builtin_ctzll(nb);
+            nb &= nb - 1;
+            used |= lane_u64(oh_lo, oh_hi, j);
+          }
+          c = (int)first_free(used, max_colors);
+          if (c == (int)kGaveUp) ++nfail;
+        }
+        const uint64_t o = (c != 0 && c != (int)kGaveUp) ? 1ull << (c - 1) : 0ull;
+        if (lane == k) {  // record vertex k's colour in its lane
+          oh_lo = (uint32_t)o;
+          oh_hi = (uint32_t)(o >> 32);
+          mine = c;
+        }
       }
     }
     if (v < t1) col[v - t0] = (uint8_t)mine;
@@ -340,11 +388,23 @@ __global__ __launch_bounds__(kClassB) void k_class_count(int32_t nv, const int32
   __shared__ int hist[64];
   if (threadIdx.x < 64) hist[threadIdx.x] = 0;
   __syncthreads();
+  const int lane = threadIdx.x & 63;
   for (int s = 0; s < kClassPer; ++s) {
     const int64_t k = ((int64_t)blockIdx.x * kClassPer + s) * kClassB + threadIdx.x;
+    int c = 0;
     if (k < nv) {
-      const int c = color[perm ? perm[k] : k];
-      if (c >= 1 && c <= ncolors) atomicAdd(&hist[c - 1], 1);
+      c = color[perm ? perm[k] : k];
+      if (c < 1 || c > ncolors) c = 0;
+    }
+    // one LDS atomic per colour present in the wave (lanes of one colour
+    // are counted with a ballot)
+    uint64_t todo = __ballot(c > 0);
+    while (todo) {
+      const int leader = __ffsll((unsigned long long)todo) - 1;
+      const int lc = __shfl(c, leader, 64);
+      const uint64_t m = __ballot(c == lc);
+      if (lane == leader) atomicAdd(&hist[lc - 1], __popcll(m));
+      todo &= ~m;
     }
   }
   __syncthreads();
@@ -647,15 +707,19 @@ static void color_impl(trn_ctx* ctx, const trn_mesh* mesh, uint64_t seed, int32_
              (const int32_t*)mesh->v2v_off, (const int32_t*)mesh->v2v, (int)max_colors,
              color_d, leave ? flags + 2 : flags);
   }
-  launch(ctx, "color_seams", 8.0 * nv + 4.0 * (double)mesh->nnz_v2v, k_color_seams,
-         dim3(nblk(nv, 256)), dim3(256), 0, nv, tile, (const int32_t*)mesh->v2v_off,
-         (const int32_t*)mesh->v2v, (const int32_t*)color_d, tent);
-  launch(ctx, "color_drop", 8.0 * nv, k_color_drop, dim3(nblk(nv, 256)), dim3(256), 0, nv,
-         (const int32_t*)tent, color_d, seam_slots);
+  // a tile longer than every edge of the numbering leaves no edge between
+  // two tiles of the same parity, hence no seam conflict to look for
+  if (tile <= mesh->bandwidth) {
+    launch(ctx, "color_seams", 8.0 * nv + 4.0 * (double)mesh->nnz_v2v, k_color_seams,
+           dim3(nblk(nv, 256)), dim3(256), 0, nv, tile, (const int32_t*)mesh->v2v_off,
+           (const int32_t*)mesh->v2v, (const int32_t*)color_d, tent);
+    launch(ctx, "color_drop", 8.0 * nv, k_color_drop, dim3(nblk(nv, 256)), dim3(256), 0, nv,
+           (const int32_t*)tent, color_d, seam_slots);
+  }
   const int32_t* h = ctx->pinned_i32;
-  // the class build right away: with a tile longer than every edge there is
-  // no seam conflict (the usual case), and one synchronisation returns the
-  // loser count, the number of colours and the class offsets
+  // the class build right away: with no seam conflict (the usual case) one
+  // synchronisation returns the loser count, the number of colours and the
+  // class offsets
   const int32_t* prev = seam_slots;
   int rounds = 0;
   for (;;) {
