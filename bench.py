@@ -220,6 +220,7 @@ def run_plain(T, mg, ctx, workload, p, sweeps, steps, warmup, seed, color_seed,
         f"{sweeps} sweeps")
     ncolors = []
     create_ms = None
+    create_kernels = None
     if create_reps:
         # the topology the reference builds inside each call (2-ring patches
         # in Metrics::run, mesh.cpp:390-434; the primal graph in Smooth::run,
@@ -237,8 +238,17 @@ def run_plain(T, mg, ctx, workload, p, sweeps, steps, warmup, seed, color_seed,
             times.append((time.perf_counter() - t0) * 1e3)
             d2.close()
         create_ms = float(np.median(times))
+        # one more creation with every launch bracketed: its kernel breakdown
+        ctx.set_timing(True, only=None)
+        ctx.reset_timing()
+        T.DeviceMesh(ctx, m.nv, tri_d, tags_d, xy=xy0).close()
+        create_kernels = {n: round(v["ms"], 3) for n, v in
+                          sorted(ctx.timings().items(), key=lambda kv: -kv[1]["ms"])[:10]}
+        ctx.set_timing(False)
+        ctx.reset_timing()
         del tri_d, tags_d
-        log(f"{workload}: device mesh creation {create_ms:.3f} ms (median of {create_reps})")
+        log(f"{workload}: device mesh creation {create_ms:.3f} ms (median of {create_reps}); "
+            f"kernels {create_kernels}")
 
     def step():
         xy.copy_(xy0)
@@ -289,7 +299,8 @@ def run_plain(T, mg, ctx, workload, p, sweeps, steps, warmup, seed, color_seed,
     free_gpu()
     return {"nv": nv, "nt": nt, "ms": ms, "stats": stats, "timed_stats": timed_stats,
             "colours": int(np.median(ncolors)), "steps": steps, "stat_steps": stat_steps,
-            "colouring_overlapped": cctx is not None, "create_ms": create_ms, "autotune": tuned}
+            "colouring_overlapped": cctx is not None, "create_ms": create_ms,
+            "create_kernels": create_kernels, "autotune": tuned}
 
 
 def summarize(r, sweeps, label=None):
@@ -833,6 +844,7 @@ def main():
         stages["per_call_with_topology"] = {
             "ms": round(ms + r["create_ms"], 4), "mesh_create_ms": round(r["create_ms"], 4),
             "Mvertices_per_s": round(r["nv"] / ((ms + r["create_ms"]) * 1e-3) / 1e6, 2),
+            "mesh_create_kernels_ms": r["create_kernels"],
             "note": "the pass plus trn_mesh_create from device-resident triangles and tags "
                     "(v2t/v2v CSR, stored 2-ring patches and their unions, star codes, vertex "
                     "records), which the reference rebuilds inside Metrics::run (getVicinity) "

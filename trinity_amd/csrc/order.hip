@@ -67,25 +67,40 @@ __device__ __forceinline__ uint32_t quantize(double v, double lo, double scale) 
   return q >= 65535. ? 65535u : (uint32_t)q;
 }
 
+// The bounding box from the per-block partials, once (one block).
+__global__ __launch_bounds__(256) void k_bbox_final(const double* __restrict__ part, int nparts,
+                                                    double* __restrict__ box) {
+  __shared__ double s[4][4];
+  const int c = threadIdx.x & 3;
+  double v = c < 2 ? INFINITY : -INFINITY;
+  for (int b = threadIdx.x >> 2; b < nparts; b += blockDim.x >> 2)
+    v = c < 2 ? fmin(v, part[4 * b + c]) : fmax(v, part[4 * b + c]);
+  // lanes c, c + 4, c + 8, ... of a wave hold the same corner
+#pragma unroll
+  for (int o = 32; o >= 4; o >>= 1) {
+    const double w = __shfl_xor(v, o, 64);
+    v = c < 2 ? fmin(v, w) : fmax(v, w);
+  }
+  if ((threadIdx.x & 63) < 4) s[threadIdx.x >> 6][c] = v;
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double r = s[0][c];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = c < 2 ? fmin(r, s[w][c]) : fmax(r, s[w][c]);
+    box[c] = r;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_morton(int32_t nv, const double2* __restrict__ xy,
-                                                const double* __restrict__ part, int nparts,
+                                                const double* __restrict__ box,
                                                 uint32_t* __restrict__ key,
                                                 int32_t* __restrict__ idx) {
-  __shared__ double box[4];
-  if (threadIdx.x < 4) {
-    double v = part[threadIdx.x];
-    for (int b = 1; b < nparts; ++b)
-      v = threadIdx.x < 2 ? fmin(v, part[4 * b + threadIdx.x]) : fmax(v, part[4 * b + threadIdx.x]);
-    box[threadIdx.x] = v;
-  }
-  __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nv) return;
-  const double ext = fmax(box[2] - box[0], box[3] - box[1]);
+  const double x0 = box[0], y0 = box[1];
+  const double ext = fmax(box[2] - x0, box[3] - y0);
   const double scale = ext > 0. ? 65536. / ext : 0.;
   const double2 p = xy[i];
-  key[i] = spread_bits(quantize(p.x, box[0], scale)) |
-           (spread_bits(quantize(p.y, box[1], scale)) << 1);
+  key[i] = spread_bits(quantize(p.x, x0, scale)) | (spread_bits(quantize(p.y, y0, scale)) << 1);
   idx[i] = (int32_t)i;
 }
 
@@ -94,10 +109,11 @@ void spatial_order(trn_ctx* ctx, int32_t nv, const double* xy, int32_t* perm) {
   TRN_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const uint32_t*)nullptr,
                                              (uint32_t*)nullptr, (const int32_t*)nullptr,
                                              (int32_t*)nullptr, nv, 0, 32, ctx->stream));
-  Scratch sc(ctx, scratch_bytes(sizeof(double) * 4 * kBoxBlocks) +
+  Scratch sc(ctx, scratch_bytes(sizeof(double) * 4 * (kBoxBlocks + 1)) +
                       2 * scratch_bytes(sizeof(uint32_t) * (size_t)nv) +
                       scratch_bytes(sizeof(int32_t) * (size_t)nv) + scratch_bytes(sort_bytes));
   double* part = sc.carve<double>(4 * kBoxBlocks);
+  double* box = sc.carve<double>(4);
   uint32_t* key = sc.carve<uint32_t>(nv);
   uint32_t* key2 = sc.carve<uint32_t>(nv);
   int32_t* idx = sc.carve<int32_t>(nv);
@@ -106,8 +122,10 @@ void spatial_order(trn_ctx* ctx, int32_t nv, const double* xy, int32_t* perm) {
   if (blocks > kBoxBlocks) blocks = kBoxBlocks;
   launch(ctx, "order_bbox", 16.0 * nv, k_bbox_partial, dim3(blocks), dim3(256), 0, nv,
          (const double2*)xy, part);
+  launch(ctx, "order_bbox_final", 0.0, k_bbox_final, dim3(1), dim3(256), 0, (const double*)part,
+         blocks, box);
   launch(ctx, "order_morton", 24.0 * nv, k_morton, dim3((unsigned)((nv + 255) / 256)), dim3(256),
-         0, nv, (const double2*)xy, (const double*)part, blocks, key, idx);
+         0, nv, (const double2*)xy, (const double*)box, key, idx);
   TRN_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, sort_bytes, (const uint32_t*)key, key2,
                                              (const int32_t*)idx, perm, nv, 0, 32, ctx->stream));
 }

@@ -348,23 +348,34 @@ __global__ void k_union_offsets(int32_t nslab, const int32_t* __restrict__ row,
   if (b <= nslab) out[b] = row[b] * kPatchSlab;
 }
 
-// Union of the patches of one slab (one block of kUnionB threads per slab):
-// the slab's stored entries are sorted in LDS (bitonic), duplicates dropped.
+// Union of the patches of one slab (one block of kUnionB threads per slab).
 // Writes the union size (or -1 when the slab is larger than kUnionKeys
-// entries or its union larger than kUnionCap), the union at union_off[b] =
-// the slab's ELL offset, and, for every patch entry, its position in the
-// union.
+// entries or its union larger than kUnionCap), the ascending union at
+// union_off[b] = the slab's ELL offset, and, for every patch entry, its
+// position in the union.  The slab's entries go into an open-addressing
+// hash set in LDS (the distinct ids, whatever the numbering's spread), the
+// distinct ids are sorted (bitonic, about 230 on a grid slab instead of its
+// 1,536 entries), and each slot records its id's position, which the
+// entries then look up.
 constexpr int kUnionB = 256;
-constexpr int kUnionWindow = kUnionKeys * 16;  // ids a bitmap in the key array covers
+constexpr int kUnionSlots = kUnionKeys;  // hash slots: one per entry at most
+constexpr int kUnionPer = kUnionKeys / kUnionB;      // entries per thread
+static_assert((kUnionSlots & (kUnionSlots - 1)) == 0, "slot count must be a power of two");
+static_assert(kUnionCap < 65536, "union positions are stored in 16 bits");
+
+__device__ __forceinline__ uint32_t union_slot(const int32_t t) {
+  return ((uint32_t)t * 0x9E3779B1u) >> (32 - __builtin_ctz(kUnionSlots));  // Fibonacci hashing
+}
 
 __global__ __launch_bounds__(kUnionB) void k_patch_union(
     int32_t nv, const int32_t* __restrict__ row, const int32_t* __restrict__ ell,
     int32_t* __restrict__ union_n, const int32_t* __restrict__ union_off,
     int32_t* __restrict__ union_ids, uint16_t* __restrict__ local) {
-  __shared__ int32_t keys[kUnionKeys];
-  __shared__ int32_t part[kUnionB];
-  __shared__ int32_t total;
-  const int b = blockIdx.x, tid = threadIdx.x;
+  __shared__ int32_t tab[kUnionSlots];    // -1: empty
+  __shared__ uint16_t rank[kUnionSlots];  // union position of the id in the slot
+  __shared__ int32_t list[kUnionCap];     // the distinct ids, then sorted
+  __shared__ int32_t count;
+  const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
   const int64_t r0 = row[b];
   const int n = (row[b + 1] - (int)r0) * kPatchSlab;
   if (n > kUnionKeys) {
@@ -373,93 +384,46 @@ __global__ __launch_bounds__(kUnionB) void k_patch_union(
   }
   const int32_t* src = ell + r0 * kPatchSlab;
   const int64_t v0 = (int64_t)b * kPatchSlab;  // first vertex of the slab
-  // Window of the slab's triangle ids.  When it spans fewer than
-  // kUnionWindow ids (a few grid rows on a numbered mesh), a bitmap over it
-  // gives the ascending union and each entry's rank (word prefix plus the
-  // bits below it) without sorting; otherwise the entries are sorted.
-  __shared__ int32_t wlo, whi;
-  if (tid == 0) {
-    wlo = INT32_MAX;
-    whi = -1;
-  }
-  __syncthreads();
-  int32_t lo = INT32_MAX, hi = -1;
-  for (int k = tid; k < n; k += kUnionB) {
-    const int32_t t = v0 + (k % kPatchSlab) < nv ? src[k] : -1;
-    if (t >= 0) {
-      lo = t < lo ? t : lo;
-      hi = t > hi ? t : hi;
-    }
-  }
+  for (int k = tid; k < kUnionSlots; k += kUnionB) tab[k] = -1;
+  if (tid == 0) count = 0;
+  // entry k = tid + e * kUnionB, kept in registers (-1: padding, or a
+  // column past the last vertex)
+  int32_t ent[kUnionPer];
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const int32_t l2 = __shfl_xor(lo, o, 64), h2 = __shfl_xor(hi, o, 64);
-    lo = l2 < lo ? l2 : lo;
-    hi = h2 > hi ? h2 : hi;
-  }
-  if ((tid & 63) == 0) {
-    atomicMin(&wlo, lo);
-    atomicMax(&whi, hi);
+  for (int e = 0; e < kUnionPer; ++e) {
+    const int k = tid + e * kUnionB;
+    ent[e] = (k < n && v0 + (k % kPatchSlab) < nv) ? src[k] : -1;
   }
   __syncthreads();
-  const int32_t L = wlo, H = whi;
-  if (H >= L && H - L < kUnionWindow) {
-    uint32_t* bits = reinterpret_cast<uint32_t*>(keys);  // words [0, kUnionWindow / 32)
-    int32_t* wpre = keys + kUnionWindow / 32;             // their exclusive prefixes
-    const int nw = ((H - L) >> 5) + 1;
-    for (int w = tid; w < nw; w += kUnionB) bits[w] = 0;
-    __syncthreads();
-    for (int k = tid; k < n; k += kUnionB) {
-      const int32_t t = v0 + (k % kPatchSlab) < nv ? src[k] : -1;
-      if (t >= 0) atomicOr(&bits[(t - L) >> 5], 1u << ((t - L) & 31));
+#pragma unroll
+  for (int e = 0; e < kUnionPer; ++e) {
+    const int32_t t = ent[e];
+    if (t < 0) continue;
+    uint32_t h = union_slot(t);
+    for (;;) {  // at most kUnionKeys distinct ids in kUnionSlots slots: ends
+      const int32_t old = atomicCAS(&tab[h], -1, t);
+      if (old == -1 || old == t) break;
+      h = (h + 1) & (kUnionSlots - 1);
     }
-    __syncthreads();
-    const int Cw = (nw + kUnionB - 1) / kUnionB, w0 = tid * Cw;
-    int cnt = 0;
-    for (int w = w0; w < w0 + Cw && w < nw; ++w) cnt += __popc(bits[w]);
-    part[tid] = cnt;
-    __syncthreads();
-    if (tid == 0) {
-      int s = 0;
-      for (int w = 0; w < kUnionB; ++w) {
-        const int c = part[w];
-        part[w] = s;
-        s += c;
-      }
-      total = s;
-      union_n[b] = s <= kUnionCap ? s : -1;
-    }
-    __syncthreads();
-    if (total > kUnionCap) return;
-    int32_t* U = union_ids + union_off[b];
-    int pos = part[tid];
-    for (int w = w0; w < w0 + Cw && w < nw; ++w) {
-      wpre[w] = pos;
-      uint32_t word = bits[w];
-      while (word) {
-        U[pos++] = L + (w << 5) + (__ffs(word) - 1);
-        word &= word - 1;
-      }
-    }
-    __syncthreads();
-    uint16_t* dst = local + r0 * kPatchSlab;
-    for (int k = tid; k < n; k += kUnionB) {
-      const int32_t t = v0 + (k % kPatchSlab) < nv ? src[k] : -1;
-      uint16_t v = 0xFFFF;
-      if (t >= 0) {
-        const int32_t d = t - L, w = d >> 5;
-        v = (uint16_t)(wpre[w] + __popc(bits[w] & ((1u << (d & 31)) - 1u)));
-      }
-      dst[k] = v;
-    }
-    return;
   }
+  __syncthreads();
+  // the distinct ids, in any order (one LDS atomic per wave and pass)
+  for (int k0 = 0; k0 < kUnionSlots; k0 += kUnionB) {
+    const int32_t t = tab[k0 + tid];
+    const uint64_t m = __ballot(t >= 0);
+    int base = 0;
+    if (lane == 0 && m) base = atomicAdd(&count, __popcll(m));
+    base = __shfl(base, 0, 64);
+    const int at = base + __popcll(m & ((1ull << lane) - 1ull));
+    if (t >= 0 && at < kUnionCap) list[at] = t;
+  }
+  __syncthreads();
+  const int nu = count;
+  if (tid == 0) union_n[b] = nu <= kUnionCap ? nu : -1;
+  if (nu > kUnionCap) return;  // nu is the block's: every thread leaves
   int P = 1;
-  while (P < n) P <<= 1;
-  for (int k = tid; k < P; k += kUnionB) {
-    const int32_t t = (k < n && v0 + (k % kPatchSlab) < nv) ? src[k] : -1;
-    keys[k] = t >= 0 ? t : INT32_MAX;
-  }
+  while (P < nu) P <<= 1;
+  for (int k = nu + tid; k < P; k += kUnionB) list[k] = INT32_MAX;
   __syncthreads();
   for (int size = 2; size <= P; size <<= 1)
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
@@ -467,57 +431,35 @@ __global__ __launch_bounds__(kUnionB) void k_patch_union(
         const int q = k ^ stride;
         if (q > k) {
           const bool up = (k & size) == 0;
-          const int32_t x = keys[k], y = keys[q];
+          const int32_t x = list[k], y = list[q];
           if ((x > y) == up) {
-            keys[k] = y;
-            keys[q] = x;
+            list[k] = y;
+            list[q] = x;
           }
         }
       }
       __syncthreads();
     }
-  // distinct entries, counted over contiguous chunks so positions follow
-  const int C = (P + kUnionB - 1) / kUnionB, c0 = tid * C;
-  int32_t mine[kUnionKeys / kUnionB];  // a chunk holds at most this many keys
-  int nm = 0;
-  for (int k = c0; k < c0 + C && k < P; ++k)
-    if (keys[k] != INT32_MAX && (k == 0 || keys[k] != keys[k - 1])) mine[nm++] = keys[k];
-  part[tid] = nm;
-  __syncthreads();
-  if (tid == 0) {
-    int s = 0;
-    for (int w = 0; w < kUnionB; ++w) {
-      const int c = part[w];
-      part[w] = s;
-      s += c;
-    }
-    total = s;
-    union_n[b] = s <= kUnionCap ? s : -1;
-  }
-  __syncthreads();
-  const int nu = total;
-  if (nu > kUnionCap) return;
-  // compact the union into the front of keys (positions only move down;
-  // every chunk was read into registers before the barrier above)
   int32_t* U = union_ids + union_off[b];
-  const int pos = part[tid];
-  for (int j = 0; j < nm; ++j) {
-    keys[pos + j] = mine[j];
-    U[pos + j] = mine[j];
+  for (int k = tid; k < nu; k += kUnionB) {
+    const int32_t t = list[k];
+    U[k] = t;
+    uint32_t h = union_slot(t);
+    while (tab[h] != t) h = (h + 1) & (kUnionSlots - 1);  // present: ends
+    rank[h] = (uint16_t)k;
   }
   __syncthreads();
   uint16_t* dst = local + r0 * kPatchSlab;
-  for (int k = tid; k < n; k += kUnionB) {
-    const int32_t t = v0 + (k % kPatchSlab) < nv ? src[k] : -1;
+#pragma unroll
+  for (int e = 0; e < kUnionPer; ++e) {
+    const int k = tid + e * kUnionB;
+    if (k >= n) continue;
+    const int32_t t = ent[e];
     uint16_t v = 0xFFFF;
     if (t >= 0) {
-      int lo = 0, hi = nu - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (keys[mid] < t) lo = mid + 1;
-        else hi = mid;
-      }
-      v = (uint16_t)lo;
+      uint32_t h = union_slot(t);
+      while (tab[h] != t) h = (h + 1) & (kUnionSlots - 1);
+      v = rank[h];
     }
     dst[k] = v;
   }
