@@ -325,7 +325,7 @@ def summarize(r, sweeps, label=None):
                    or n == "reduce_max")
     smooth_ms = sum(kern[n]["ms_per_step"] for n in ("tensor_log", "pack_records",
                                                      "quality_cache", "class_live",
-                                                     "smooth_phase", "unpack_records",
+                                                     "smooth_phase",
                                                      "counter_reduce")
                     if n in kern)
     stages = {

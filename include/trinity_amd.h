@@ -299,9 +299,12 @@ int trn_smooth_run(trn_ctx* ctx, const trn_mesh* mesh, double* xy_d,
  * and, if recompute_qualit != 0, rebuilds the quality cache
  * (Smooth::cacheQuality, src/smoothing.cpp:145-155, :290); it needs no
  * colouring, so a caller can issue it before the colouring is done.
- * trn_smooth_run_prepared runs the sweeps on the records and writes xy_d
- * and tensor_d back.  Between the two, xy_d, tensor_d and qualit_d must not
- * change and no other smoothing call may use the mesh. */
+ * trn_smooth_run_prepared runs the sweeps on the records; every committed
+ * move also writes its vertex's coordinates and metric into xy_d and
+ * tensor_d, which must be the arrays given to trn_smooth_prepare (their
+ * other entries already hold the records' values).  Between the two, xy_d,
+ * tensor_d and qualit_d must not change and no other smoothing call may
+ * use the mesh. */
 int trn_smooth_prepare(trn_ctx* ctx, const trn_mesh* mesh, const double* xy_d,
                        const double* tensor_d, double* qualit_d,
                        int32_t recompute_qualit);

@@ -446,7 +446,7 @@ def test_timing_filter_brackets_only_the_named_kernel(ctx):
     assert set(only) == {"smooth_phase"}
     assert only["smooth_phase"]["launches"] == 2 * col.ncolors
     assert only["smooth_phase"]["bytes"] > 0
-    assert {"smooth_phase", "pack_records", "unpack_records"} <= set(every)
+    assert {"smooth_phase", "pack_records", "quality_cache"} <= set(every)
     assert every["smooth_phase"]["launches"] == only["smooth_phase"]["launches"]
 
 

@@ -73,19 +73,6 @@ __global__ __launch_bounds__(256) void k_pack_records(int32_t nv, const double2*
   logm[i] = r.logm;
 }
 
-// Coordinates and metric back from the records into the caller's arrays.
-__global__ __launch_bounds__(256) void k_unpack_records(int32_t nv,
-                                                        const VRecord* __restrict__ rec,
-                                                        double2* __restrict__ xy,
-                                                        double* __restrict__ tensor) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nv) return;
-  xy[i] = rec[i].xy;
-  tensor[3 * i] = rec[i].t[0];
-  tensor[3 * i + 1] = rec[i].t[1];
-  tensor[3 * i + 2] = rec[i].t[2];
-}
-
 // One colour phase.  count_d == nullptr: `order` lists `count` vertices
 // (the whole class); otherwise it is a live list of *count_d <= count
 // vertices written by k_class_live, the grid is sized for `count`, and the
@@ -95,7 +82,8 @@ __global__ __launch_bounds__(256) void k_unpack_records(int32_t nv,
 template <int MINW>
 __global__ __launch_bounds__(64, MINW) void k_smooth_phase(
     int32_t count, const int32_t* __restrict__ count_d, const int32_t* __restrict__ order,
-    VRecord* __restrict__ rec, double* __restrict__ qualit, uint8_t* __restrict__ stale,
+    VRecord* __restrict__ rec, double2* __restrict__ xy_out, double* __restrict__ t_out,
+    double* __restrict__ qualit, uint8_t* __restrict__ stale,
     const int32_t* __restrict__ v2t_off, const int32_t* __restrict__ v2t,
     const int32_t* __restrict__ star, const int32_t* __restrict__ v2v_off,
     const int32_t* __restrict__ v2v, int32_t* __restrict__ counters,
@@ -115,7 +103,8 @@ __global__ __launch_bounds__(64, MINW) void k_smooth_phase(
   const int64_t idx = (int64_t)lb * blockDim.x + threadIdx.x;
   int commit = 0, task = 0;
   if (idx < n)
-    commit = move_vertex_rec(order[idx], rec, qualit, stale, v2t_off, v2t, star, v2v_off, v2v,
+    commit = move_vertex_rec(order[idx], rec, xy_out, t_out, qualit, stale, v2t_off, v2t, star,
+                             v2v_off, v2v,
                              lds + threadIdx.x, blockDim.x, count_d == nullptr, &task);
   // counters are spread over kCounterSlots addresses: one hot address
   // serialises the atomics of every wave in L2
@@ -219,9 +208,10 @@ static void phase_shape(const trn_mesh* m, int* block, size_t* shmem) {
   *shmem = (size_t)b * per_thread * sizeof(double);
 }
 
-using PhaseKernel = void (*)(int32_t, const int32_t*, const int32_t*, VRecord*, double*,
-                            uint8_t*, const int32_t*, const int32_t*, const int32_t*,
-                            const int32_t*, const int32_t*, int32_t*, unsigned long long*);
+using PhaseKernel = void (*)(int32_t, const int32_t*, const int32_t*, VRecord*, double2*,
+                            double*, double*, uint8_t*, const int32_t*, const int32_t*,
+                            const int32_t*, const int32_t*, const int32_t*, int32_t*,
+                            unsigned long long*);
 
 static PhaseKernel phase_kernel(int minw) {
   return minw <= 2 ? k_smooth_phase<2> : (minw >= 4 ? k_smooth_phase<4> : k_smooth_phase<3>);
@@ -236,7 +226,8 @@ void tune_phase(trn_ctx* ctx, const trn_mesh* m, int32_t count, const int32_t* l
   phase_shape(m, &block, &shmem);
   TRN_HIP(hipEventRecord(ev[0], ctx->stream));
   launch(ctx, "smooth_phase", 0.0, phase_kernel(minw), dim3(nblk(count, block)), dim3(block),
-         shmem, count, (const int32_t*)nullptr, list, m->vrec, qualit, stale,
+         shmem, count, (const int32_t*)nullptr, list, m->vrec, (double2*)nullptr,
+         (double*)nullptr, qualit, stale,
          (const int32_t*)m->v2t_off, (const int32_t*)m->v2t, (const int32_t*)m->star,
          (const int32_t*)m->v2v_off, (const int32_t*)m->v2v, counters,
          (unsigned long long*)nullptr);
@@ -389,13 +380,12 @@ int32_t* smooth_sweeps_launch(trn_ctx* ctx, const trn_mesh* m, double* xy, doubl
           count_d ? timing_units(ctx, "smooth_phase", kBytesSweepPerV) : nullptr;
       launch(ctx, "smooth_phase", count_d ? 0.0 : kBytesSweepPerV * count,
              phase_kernel(m->tune.phase_minw), dim3(nblk(count, block)), dim3(block), shmem,
-             count, count_d, list, rec, qualit, stale, (const int32_t*)m->v2t_off,
+             count, count_d, list, rec, (double2*)xy, tensor, qualit, stale,
+             (const int32_t*)m->v2t_off,
              (const int32_t*)m->v2t, (const int32_t*)m->star, (const int32_t*)m->v2v_off,
              (const int32_t*)m->v2v, cs, units);
     }
   }
-  launch(ctx, "unpack_records", 80.0 * m->nv, k_unpack_records, dim3(nblk(m->nv, 256)),
-         dim3(256), 0, m->nv, (const VRecord*)rec, (double2*)xy, tensor);
   if (!want_totals || sweeps == 0) return nullptr;
   launch(ctx, "counter_reduce", 0.0, k_counter_reduce, dim3(1), dim3(1024), 0, sweeps,
          (const int32_t*)counters, totals);

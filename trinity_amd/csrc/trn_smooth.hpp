@@ -34,8 +34,14 @@ static_assert(offsetof(VRecord, xy) == 32 && offsetof(VRecord, t) == 48 &&
                   offsetof(VRecord, topo) == 72,
               "VRecord layout");
 
+// Commits also write the new coordinates and metric through to the
+// caller's arrays (xo, to) when given, so no copy back is needed after the
+// sweeps: those arrays start equal to the records' data and change only
+// where a move commits.
 struct RecVerts {
   VRecord* __restrict__ rec;
+  double2* __restrict__ xo;
+  double* __restrict__ to;
   __device__ __forceinline__ double2 pos(const int32_t j) const { return rec[j].xy; }
   __device__ __forceinline__ void metric(const int32_t j, double T[3]) const {
     T[0] = rec[j].t[0];
@@ -50,6 +56,12 @@ struct RecVerts {
     rec[i].t[1] = TN[1];
     rec[i].t[2] = TN[2];
     rec[i].logm = LN;
+    if (xo) {
+      xo[i] = make_double2(px, py);
+      to[3 * (int64_t)i] = TN[0];
+      to[3 * (int64_t)i + 1] = TN[1];
+      to[3 * (int64_t)i + 2] = TN[2];
+    }
   }
 };
 
@@ -330,14 +342,15 @@ __device__ __forceinline__ int move_vertex_generic(
 // registers do not weigh on the fast path.  Called only for a move the fast
 // or generic path left undecided (a quality within 16 ulps of q_min).
 static __device__ __noinline__ int move_exact_rec(const int32_t i, const double2 pini,
-                                           VRecord* __restrict__ rec, double* __restrict__ qualit,
+                                           VRecord* __restrict__ rec, double2* __restrict__ xo,
+                                           double* __restrict__ to, double* __restrict__ qualit,
                                            const int32_t* __restrict__ v2t_off,
                                            const int32_t* __restrict__ v2t,
                                            const int32_t* __restrict__ star,
                                            const int32_t* __restrict__ v2v_off,
                                            const int32_t* __restrict__ v2v, double* lds,
                                            const int stride) {
-  const RecVerts v{rec};
+  const RecVerts v{rec, xo, to};
   const double T[3] = {rec[i].t[0], rec[i].t[1], rec[i].t[2]};
   const int32_t t0 = v2t_off[i], deg = v2t_off[i + 1] - t0;
   const int32_t n0 = v2v_off[i], nb = v2v_off[i + 1] - n0;
@@ -362,6 +375,7 @@ static __device__ __noinline__ int move_exact_rec(const int32_t i, const double2
 // Within one phase only the vertex itself and vertices of other colours
 // touch stale[i], apart from concurrent commits that all store 0.
 __device__ __forceinline__ int move_vertex_rec(const int32_t i, VRecord* __restrict__ rec,
+                                               double2* __restrict__ xo, double* __restrict__ to,
                                                double* __restrict__ qualit,
                                                uint8_t* __restrict__ stale,
                                                const int32_t* __restrict__ v2t_off,
@@ -370,7 +384,7 @@ __device__ __forceinline__ int move_vertex_rec(const int32_t i, VRecord* __restr
                                                const int32_t* __restrict__ v2v_off,
                                                const int32_t* __restrict__ v2v, double* lds,
                                                const int stride, const int first, int* task) {
-  const RecVerts v{rec};
+  const RecVerts v{rec, xo, to};
   if (first) stale[i] = 0;
   const double2 pini = rec[i].xy;
   const double T[3] = {rec[i].t[0], rec[i].t[1], rec[i].t[2]};
@@ -382,7 +396,9 @@ __device__ __forceinline__ int move_vertex_rec(const int32_t i, VRecord* __restr
     *task = 1;
     const FastTopo ft = fast_topo_rec(i, topo);
     int c = move_vertex_fast(i, ft, pini, T, v, qualit, lds, stride);
-    if (c == 2) c = move_exact_rec(i, pini, rec, qualit, v2t_off, v2t, star, v2v_off, v2v, lds, stride);
+    if (c == 2)
+      c = move_exact_rec(i, pini, rec, xo, to, qualit, v2t_off, v2t, star, v2v_off, v2v, lds,
+                         stride);
     if (c) {
 #pragma unroll
       for (int k = 0; k < kVtopoDeg; ++k)
@@ -398,7 +414,9 @@ __device__ __forceinline__ int move_vertex_rec(const int32_t i, VRecord* __restr
   *task = 1;
   int c = move_vertex_generic<false>(i, t0, deg, n0, nb, pini, T, v, qualit, v2t, star, v2v,
                                      lds, stride);
-  if (c == 2) c = move_exact_rec(i, pini, rec, qualit, v2t_off, v2t, star, v2v_off, v2v, lds, stride);
+  if (c == 2)
+    c = move_exact_rec(i, pini, rec, xo, to, qualit, v2t_off, v2t, star, v2v_off, v2v, lds,
+                       stride);
   if (c) {
     for (int k = 0; k < nb; ++k) stale[v2v[n0 + k]] = 0;
   } else {
