@@ -55,22 +55,47 @@ __global__ __launch_bounds__(256) void k_quality_cache(int32_t nt, const int32_t
 // Data part of the vertex records (metric log, coordinates, metric) from
 // the caller's arrays, and the logs as a dense array for the quality
 // cache; replaces k_tensor_log for the sweeps.  Whole 128-byte records are
-// stored (the topology part read back), so no line is written in part.
-__global__ __launch_bounds__(256) void k_pack_records(int32_t nv, const double2* __restrict__ xy,
-                                                      const double* __restrict__ tensor,
-                                                      VRecord* __restrict__ rec,
-                                                      double4* __restrict__ logm) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nv) return;
-  VRecord r;
-  r.topo = rec[i].topo;
-  r.t[0] = tensor[3 * i];
-  r.t[1] = tensor[3 * i + 1];
-  r.t[2] = tensor[3 * i + 2];
-  r.logm = tensor_log4(r.t[0], r.t[1], r.t[2]);
-  r.xy = xy[i];
-  rec[i] = r;
-  logm[i] = r.logm;
+// stored, so no line is written in part: a block stages its records in LDS
+// (their second 64 bytes, t[2] and the topology, loaded with coalesced
+// reads), fills the data part, and writes the records back eight lanes to
+// a line.  A record occupies 9 LDS chunks of 16 bytes (one of padding), so
+// eight consecutive lanes touch all 32 banks.
+constexpr int kPackB = 256;
+constexpr int kPackPitch = 9;  // 16-byte LDS chunks per record
+
+__global__ __launch_bounds__(kPackB) void k_pack_records(int32_t nv, const double2* __restrict__ xy,
+                                                         const double* __restrict__ tensor,
+                                                         VRecord* __restrict__ rec,
+                                                         double4* __restrict__ logm) {
+  static_assert(sizeof(VRecord) == 8 * sizeof(double2) && offsetof(VRecord, t) == 48,
+                "a record is 8 chunks of 16 bytes; t[2] opens the fifth");
+  __shared__ double2 sh[kPackB * kPackPitch];
+  const int64_t v0 = (int64_t)blockIdx.x * kPackB;
+  const int nrec = nv - v0 < kPackB ? (int)(nv - v0) : kPackB;
+  const int tid = threadIdx.x;
+  double2* line = reinterpret_cast<double2*>(rec + v0);
+  for (int c = tid; c < 4 * nrec; c += kPackB) {  // chunks 4..7 of record c / 4
+    const int r = c >> 2, q = 4 + (c & 3);
+    sh[kPackPitch * r + q] = line[8 * r + q];
+  }
+  __syncthreads();
+  if (tid < nrec) {
+    const int64_t i = v0 + tid;
+    const double t0 = tensor[3 * i], t1 = tensor[3 * i + 1], t2 = tensor[3 * i + 2];
+    const double4 L = tensor_log4(t0, t1, t2);
+    double2* r = sh + kPackPitch * tid;
+    r[0] = make_double2(L.x, L.y);
+    r[1] = make_double2(L.z, L.w);
+    r[2] = xy[i];
+    r[3] = make_double2(t0, t1);
+    r[4].x = t2;
+    logm[i] = L;
+  }
+  __syncthreads();
+  for (int c = tid; c < 8 * nrec; c += kPackB) {
+    const int r = c >> 3, q = c & 7;
+    line[8 * r + q] = sh[kPackPitch * r + q];
+  }
 }
 
 // One colour phase.  count_d == nullptr: `order` lists `count` vertices
@@ -330,8 +355,8 @@ __global__ __launch_bounds__(kStatB) void k_quality_final(int32_t nparts,
 void smooth_prepare_launch(trn_ctx* ctx, const trn_mesh* m, const double* xy,
                            const double* tensor, double* qualit, bool recompute_qualit,
                            double4* logm) {
-  launch(ctx, "pack_records", 256.0 * m->nv, k_pack_records, dim3(nblk(m->nv, 256)), dim3(256),
-         0, m->nv, (const double2*)xy, tensor, m->vrec, logm);
+  launch(ctx, "pack_records", 256.0 * m->nv, k_pack_records, dim3(nblk(m->nv, kPackB)),
+         dim3(kPackB), 0, m->nv, (const double2*)xy, tensor, m->vrec, logm);
   if (recompute_qualit) quality_cache_logs(ctx, m, xy, logm, qualit);
 }
 

@@ -96,8 +96,8 @@ static_assert(sizeof(VTopo) == 56, "VTopo is 56 bytes");
 // Vertex record of the smoothing sweeps: one 128-byte line per vertex with
 // what a move reads of a vertex (metric log, coordinates, metric) and its
 // compact topology.  The topology is written when the mesh is created; each
-// smoothing call fills the first 72 bytes from the caller's arrays and
-// copies coordinates and metric back at its end.
+// smoothing call fills the first 72 bytes from the caller's arrays, and a
+// committed move writes its new data to the record and to those arrays.
 struct alignas(128) VRecord {
   double4 logm;  // matrix log of the metric (numeric.cpp:115-154)
   double2 xy;
