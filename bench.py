@@ -712,8 +712,12 @@ def run_batched(T, mg, count=512, p=2, sweeps=2, steps=5, warmup=2, loop_steps=2
     bpv = BYTES_METRIC_PER_V + BYTES_COLOR_PER_V + BYTES_SWEEP_PER_V * sweeps
     kern = {n: {"ms": round(v["ms"], 4), "launches": v["launches"]}
             for n, v in sorted(kstats.items(), key=lambda kv: -kv[1]["ms"])[:10]}
+    dominant = max(kstats, key=lambda n: kstats[n]["ms"])
+    roof = roofline_for(dominant, kstats[dominant], f"batch{count}", kstats)
+    roof["note"] = ("dominant kernel of one eager batched pass (HIP events on every launch); " +
+                    roof["note"])
     return {"meshes": count, "vertices": nv, "triangles": nt, "lp_norm": p, "sweeps": sweeps,
-            "colours": ncol,
+            "colours": ncol, "roofline": roof,
             "batched": {"ms_per_pass": round(bms, 3),
                         "Mvertices_per_s": round(nv / (bms * 1e-3) / 1e6, 2),
                         "alg_GBps": round(bpv * nv / (bms * 1e-3) / 1e9, 1),
