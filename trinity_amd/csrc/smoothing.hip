@@ -52,50 +52,25 @@ __global__ __launch_bounds__(256) void k_quality_cache(int32_t nt, const int32_t
   qualit[t] = quality_logs(t, tri, xy, logm);
 }
 
-// Data part of the vertex records (metric log, coordinates, metric) from
-// the caller's arrays, and the logs as a dense array for the quality
-// cache; replaces k_tensor_log for the sweeps.  Whole 128-byte records are
-// stored, so no line is written in part: a block stages its records in LDS
-// (their second 64 bytes, t[2] and the topology, loaded with coalesced
-// reads), fills the data part, and writes the records back eight lanes to
-// a line.  A record occupies 9 LDS chunks of 16 bytes (one of padding), so
-// eight consecutive lanes touch all 32 banks.
-constexpr int kPackB = 256;
-constexpr int kPackPitch = 9;  // 16-byte LDS chunks per record
-
-__global__ __launch_bounds__(kPackB) void k_pack_records(int32_t nv, const double2* __restrict__ xy,
-                                                         const double* __restrict__ tensor,
-                                                         VRecord* __restrict__ rec,
-                                                         double4* __restrict__ logm) {
-  static_assert(sizeof(VRecord) == 8 * sizeof(double2) && offsetof(VRecord, t) == 48,
-                "a record is 8 chunks of 16 bytes; t[2] opens the fifth");
-  __shared__ double2 sh[kPackB * kPackPitch];
-  const int64_t v0 = (int64_t)blockIdx.x * kPackB;
-  const int nrec = nv - v0 < kPackB ? (int)(nv - v0) : kPackB;
-  const int tid = threadIdx.x;
-  double2* line = reinterpret_cast<double2*>(rec + v0);
-  for (int c = tid; c < 4 * nrec; c += kPackB) {  // chunks 4..7 of record c / 4
-    const int r = c >> 2, q = 4 + (c & 3);
-    sh[kPackPitch * r + q] = line[8 * r + q];
-  }
-  __syncthreads();
-  if (tid < nrec) {
-    const int64_t i = v0 + tid;
-    const double t0 = tensor[3 * i], t1 = tensor[3 * i + 1], t2 = tensor[3 * i + 2];
-    const double4 L = tensor_log4(t0, t1, t2);
-    double2* r = sh + kPackPitch * tid;
-    r[0] = make_double2(L.x, L.y);
-    r[1] = make_double2(L.z, L.w);
-    r[2] = xy[i];
-    r[3] = make_double2(t0, t1);
-    r[4].x = t2;
-    logm[i] = L;
-  }
-  __syncthreads();
-  for (int c = tid; c < 8 * nrec; c += kPackB) {
-    const int r = c >> 3, q = c & 7;
-    line[8 * r + q] = sh[kPackPitch * r + q];
-  }
+// The sweeps' working copy (metric log, coordinates, metric) from the
+// caller's arrays, and the logs as a dense array for the quality cache;
+// replaces k_tensor_log for the sweeps.
+__global__ __launch_bounds__(256) void k_pack_records(int32_t nv, const double2* __restrict__ xy,
+                                                      const double* __restrict__ tensor,
+                                                      VData* __restrict__ vd,
+                                                      double* __restrict__ t2,
+                                                      double4* __restrict__ logm) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const double t0 = tensor[3 * i], t1 = tensor[3 * i + 1], t22 = tensor[3 * i + 2];
+  VData d;
+  d.logm = tensor_log4(t0, t1, t22);
+  d.xy = xy[i];
+  d.t01[0] = t0;
+  d.t01[1] = t1;
+  vd[i] = d;
+  t2[i] = t22;
+  logm[i] = d.logm;
 }
 
 // One colour phase.  count_d == nullptr: `order` lists `count` vertices
@@ -107,7 +82,8 @@ __global__ __launch_bounds__(kPackB) void k_pack_records(int32_t nv, const doubl
 template <int MINW>
 __global__ __launch_bounds__(64, MINW) void k_smooth_phase(
     int32_t count, const int32_t* __restrict__ count_d, const int32_t* __restrict__ order,
-    VRecord* __restrict__ rec, double2* __restrict__ xy_out, double* __restrict__ t_out,
+    VData* __restrict__ vd, double* __restrict__ t2, const VTopo* __restrict__ vtopo,
+    double2* __restrict__ xy_out, double* __restrict__ t_out,
     double* __restrict__ qualit, uint8_t* __restrict__ stale,
     const int32_t* __restrict__ v2t_off, const int32_t* __restrict__ v2t,
     const int32_t* __restrict__ star, const int32_t* __restrict__ v2v_off,
@@ -128,8 +104,8 @@ __global__ __launch_bounds__(64, MINW) void k_smooth_phase(
   const int64_t idx = (int64_t)lb * blockDim.x + threadIdx.x;
   int commit = 0, task = 0;
   if (idx < n)
-    commit = move_vertex_rec(order[idx], rec, xy_out, t_out, qualit, stale, v2t_off, v2t, star,
-                             v2v_off, v2v,
+    commit = move_vertex_rec(order[idx], vd, t2, vtopo, xy_out, t_out, qualit, stale, v2t_off,
+                             v2t, star, v2v_off, v2v,
                              lds + threadIdx.x, blockDim.x, count_d == nullptr, &task);
   // counters are spread over kCounterSlots addresses: one hot address
   // serialises the atomics of every wave in L2
@@ -233,10 +209,10 @@ static void phase_shape(const trn_mesh* m, int* block, size_t* shmem) {
   *shmem = (size_t)b * per_thread * sizeof(double);
 }
 
-using PhaseKernel = void (*)(int32_t, const int32_t*, const int32_t*, VRecord*, double2*,
-                            double*, double*, uint8_t*, const int32_t*, const int32_t*,
-                            const int32_t*, const int32_t*, const int32_t*, int32_t*,
-                            unsigned long long*);
+using PhaseKernel = void (*)(int32_t, const int32_t*, const int32_t*, VData*, double*,
+                            const VTopo*, double2*, double*, double*, uint8_t*, const int32_t*,
+                            const int32_t*, const int32_t*, const int32_t*, const int32_t*,
+                            int32_t*, unsigned long long*);
 
 static PhaseKernel phase_kernel(int minw) {
   return minw <= 2 ? k_smooth_phase<2> : (minw >= 4 ? k_smooth_phase<4> : k_smooth_phase<3>);
@@ -251,8 +227,8 @@ void tune_phase(trn_ctx* ctx, const trn_mesh* m, int32_t count, const int32_t* l
   phase_shape(m, &block, &shmem);
   TRN_HIP(hipEventRecord(ev[0], ctx->stream));
   launch(ctx, "smooth_phase", 0.0, phase_kernel(minw), dim3(nblk(count, block)), dim3(block),
-         shmem, count, (const int32_t*)nullptr, list, m->vrec, (double2*)nullptr,
-         (double*)nullptr, qualit, stale,
+         shmem, count, (const int32_t*)nullptr, list, m->vdata, m->vt2,
+         (const VTopo*)m->vtopo, (double2*)nullptr, (double*)nullptr, qualit, stale,
          (const int32_t*)m->v2t_off, (const int32_t*)m->v2t, (const int32_t*)m->star,
          (const int32_t*)m->v2v_off, (const int32_t*)m->v2v, counters,
          (unsigned long long*)nullptr);
@@ -355,8 +331,8 @@ __global__ __launch_bounds__(kStatB) void k_quality_final(int32_t nparts,
 void smooth_prepare_launch(trn_ctx* ctx, const trn_mesh* m, const double* xy,
                            const double* tensor, double* qualit, bool recompute_qualit,
                            double4* logm) {
-  launch(ctx, "pack_records", 256.0 * m->nv, k_pack_records, dim3(nblk(m->nv, kPackB)),
-         dim3(kPackB), 0, m->nv, (const double2*)xy, tensor, m->vrec, logm);
+  launch(ctx, "pack_records", 144.0 * m->nv, k_pack_records, dim3(nblk(m->nv, 256)), dim3(256),
+         0, m->nv, (const double2*)xy, tensor, m->vdata, m->vt2, logm);
   if (recompute_qualit) quality_cache_logs(ctx, m, xy, logm, qualit);
 }
 
@@ -380,7 +356,6 @@ int32_t* smooth_sweeps_launch(trn_ctx* ctx, const trn_mesh* m, double* xy, doubl
   int32_t* totals = sc.carve<int32_t>(2 * (size_t)(sweeps + 1));
   uint8_t* stale = sc.carve<uint8_t>(m->nv);
   int32_t* live = sc.carve<int32_t>((size_t)(max_class > 0 ? max_class : 1));
-  VRecord* rec = m->vrec;
   memset_async(ctx, counters, 0, sizeof(int32_t) * (ncount + nlive));
   int block;
   size_t shmem;
@@ -405,7 +380,8 @@ int32_t* smooth_sweeps_launch(trn_ctx* ctx, const trn_mesh* m, double* xy, doubl
           count_d ? timing_units(ctx, "smooth_phase", kBytesSweepPerV) : nullptr;
       launch(ctx, "smooth_phase", count_d ? 0.0 : kBytesSweepPerV * count,
              phase_kernel(m->tune.phase_minw), dim3(nblk(count, block)), dim3(block), shmem,
-             count, count_d, list, rec, (double2*)xy, tensor, qualit, stale,
+             count, count_d, list, m->vdata, m->vt2, (const VTopo*)m->vtopo, (double2*)xy,
+             tensor, qualit, stale,
              (const int32_t*)m->v2t_off,
              (const int32_t*)m->v2t, (const int32_t*)m->star, (const int32_t*)m->v2v_off,
              (const int32_t*)m->v2v, cs, units);

@@ -496,20 +496,20 @@ __global__ void k_star(int32_t nv, const int32_t* __restrict__ tri,
   }
 }
 
-// Compact topology (VTopo) of the vertex records from the CSR arrays and
-// star codes.
+// Compact topology (VTopo) of every vertex for the smoothing sweeps, from
+// the CSR arrays and star codes.
 __global__ __launch_bounds__(256) void k_vtopo(int32_t nv, const uint8_t* __restrict__ tags,
                                                const int32_t* __restrict__ v2t_off,
                                                const int32_t* __restrict__ v2t,
                                                const int32_t* __restrict__ star,
                                                const int32_t* __restrict__ v2v_off,
                                                const int32_t* __restrict__ v2v,
-                                               VRecord* __restrict__ out) {
+                                               VTopo* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nv) return;
   const int32_t t0 = v2t_off[i], deg = v2t_off[i + 1] - t0;
   const int32_t n0 = v2v_off[i], nb = v2v_off[i + 1] - n0;
-  VTopo r;
+  VTopo r = {};
   r.tag = tags[i];
   r.meta = (deg <= kVtopoDeg && nb <= kVtopoDeg) ? (uint8_t)(deg | nb << 4) : kVtopoGeneric;
   const bool fast = r.meta != kVtopoGeneric;
@@ -520,7 +520,7 @@ __global__ __launch_bounds__(256) void k_vtopo(int32_t nv, const uint8_t* __rest
     const int32_t c = (fast && k < deg) ? star[t0 + k] : 0;
     r.star[k] = (uint8_t)((c & 3) | (((c >> 2) & 7) << 2) | (((c >> 10) & 7) << 5));
   }
-  out[i].topo = r;
+  out[i] = r;
 }
 
 static unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -700,11 +700,13 @@ int trn_mesh_create(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
       launch(ctx, "topo_star", 0.0, k_star, dim3(blocks_for(nv, 256)), dim3(256), 0, nv,
              (const int32_t*)m->tri, (const int32_t*)m->v2t_off, (const int32_t*)m->v2t,
              (const int32_t*)m->v2v_off, (const int32_t*)m->v2v, m->star);
-      TRN_HIP(hipMalloc(&m->vrec, sizeof(VRecord) * (size_t)nv));
+      TRN_HIP(hipMalloc(&m->vdata, sizeof(VData) * (size_t)nv));
+      TRN_HIP(hipMalloc(&m->vt2, sizeof(double) * (size_t)nv));
+      TRN_HIP(hipMalloc(&m->vtopo, sizeof(VTopo) * (size_t)nv));
       launch(ctx, "topo_vtopo", 0.0, k_vtopo, dim3(blocks_for(nv, 256)), dim3(256), 0, nv,
              (const uint8_t*)m->tags, (const int32_t*)m->v2t_off, (const int32_t*)m->v2t,
              (const int32_t*)m->star, (const int32_t*)m->v2v_off, (const int32_t*)m->v2v,
-             m->vrec);
+             m->vtopo);
 
     } catch (...) {
       cleanup();
@@ -727,7 +729,9 @@ int trn_mesh_destroy(trn_mesh* m) {
     if (m->v2v) (void)hipFree(m->v2v);
     if (m->star) (void)hipFree(m->star);
     if (m->zorder) (void)hipFree(m->zorder);
-    if (m->vrec) (void)hipFree(m->vrec);
+    if (m->vdata) (void)hipFree(m->vdata);
+    if (m->vt2) (void)hipFree(m->vt2);
+    if (m->vtopo) (void)hipFree(m->vtopo);
     if (m->patch_n) (void)hipFree(m->patch_n);
     if (m->patch_row) (void)hipFree(m->patch_row);
     if (m->patch_ell) (void)hipFree(m->patch_ell);

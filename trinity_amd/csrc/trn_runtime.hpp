@@ -84,27 +84,28 @@ namespace trn {
 // kVtopoDeg incident triangles or neighbours (then the CSR arrays are used).
 constexpr int kVtopoDeg = 6;
 constexpr uint8_t kVtopoGeneric = 0xFF;
-struct VTopo {
+// Padded to 64 bytes, so an entry never straddles two 128-byte lines.
+struct alignas(64) VTopo {
   int32_t ring[kVtopoDeg];
   int32_t tris[kVtopoDeg];
   uint8_t star[kVtopoDeg];
   uint8_t meta;
   uint8_t tag;
 };
-static_assert(sizeof(VTopo) == 56, "VTopo is 56 bytes");
+static_assert(sizeof(VTopo) == 64, "VTopo is 64 bytes");
 
-// Vertex record of the smoothing sweeps: one 128-byte line per vertex with
-// what a move reads of a vertex (metric log, coordinates, metric) and its
-// compact topology.  The topology is written when the mesh is created; each
-// smoothing call fills the first 72 bytes from the caller's arrays, and a
-// committed move writes its new data to the record and to those arrays.
-struct alignas(128) VRecord {
-  double4 logm;  // matrix log of the metric (numeric.cpp:115-154)
+// What a move reads of a neighbour (metric log, coordinates, metric) in
+// the smoothing sweeps' working copy: 64 bytes per vertex, two vertices per
+// 128-byte line (a miss in L2 brings a whole line), with the metric's
+// third entry in a dense array of its own (eight vertices per 64 bytes).
+// Filled by each smoothing call from the caller's arrays; a committed move
+// writes its new data there and to those arrays.
+struct alignas(64) VData {
+  double4 logm;   // matrix log of the metric (numeric.cpp:115-154)
   double2 xy;
-  double t[3];   // metric
-  VTopo topo;
+  double t01[2];  // metric entries m11, m12
 };
-static_assert(sizeof(VRecord) == 128, "VRecord is one 128-byte line");
+static_assert(sizeof(VData) == 64, "VData is 64 bytes");
 
 constexpr int kPatchSlab = 64;  // vertices per slab of the stored patches
 constexpr int kUnionKeys = 4096;  // largest slab (rows x kPatchSlab) given a union
@@ -140,9 +141,12 @@ struct trn_mesh {
   // Visiting order of the vertices of a colour class: NULL (ascending, the
   // default) or zorder (trn_mesh_set_class_order).  Not owned.
   const int32_t* class_v = nullptr;
-  // [nv] working copy of the smoothing sweeps (topology filled at
-  // creation); smoothing calls on one mesh must not overlap.
-  trn::VRecord* vrec = nullptr;
+  // [nv] working copy of the smoothing sweeps (vdata, vt2: metric m22)
+  // and the vertices' compact topology (filled at creation); smoothing
+  // calls on one mesh must not overlap.
+  trn::VData* vdata = nullptr;
+  double* vt2 = nullptr;
+  trn::VTopo* vtopo = nullptr;
   // 2-ring element patches (Mesh::getVicinity(i, 2), ascending) in sliced
   // ELL form: vertices in slabs of kPatchSlab; entry e of the patch of
   // vertex i = slab_v[b * kPatchSlab + l] is patch_ell[(patch_row[b] + e) *

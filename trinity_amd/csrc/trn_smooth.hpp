@@ -3,10 +3,11 @@
 // kernels of smoothing.hip.
 //
 // The move reads, for the vertex and its 1-ring, coordinates, metric and
-// the cached matrix log of the metric, through an accessor over the vertex
-// records (RecVerts: one 128-byte VRecord per vertex holding all three and
-// the vertex's compact topology, so a neighbour costs one cache line
-// instead of three or four).
+// the cached matrix log of the metric, through an accessor over the
+// sweeps' working copy (RecVerts: a 64-byte VData per vertex holding the
+// log, the coordinates and two metric entries, two vertices per cache
+// line, and a dense array of the third metric entry), and the vertex's own
+// compact topology (VTopo, 64 bytes).
 #pragma once
 
 #include <cfloat>
@@ -30,32 +31,32 @@ __device__ __forceinline__ double quality_logs(const int64_t t, const int32_t* _
   return quality_of_mean(A, B, C, j);
 }
 
-static_assert(offsetof(VRecord, xy) == 32 && offsetof(VRecord, t) == 48 &&
-                  offsetof(VRecord, topo) == 72,
-              "VRecord layout");
-
-// Commits also write the new coordinates and metric through to the
-// caller's arrays (xo, to) when given, so no copy back is needed after the
-// sweeps: those arrays start equal to the records' data and change only
-// where a move commits.
+// The smoothing sweeps' working copy: vd[j] (log, xy, m11, m12) and t2[j]
+// (m22) of every vertex.  Commits also write the new coordinates and metric
+// through to the caller's arrays (xo, to) when given, so no copy back is
+// needed after the sweeps: those arrays start equal to the working copy
+// and change only where a move commits.
 struct RecVerts {
-  VRecord* __restrict__ rec;
+  VData* __restrict__ vd;
+  double* __restrict__ t2;
   double2* __restrict__ xo;
   double* __restrict__ to;
-  __device__ __forceinline__ double2 pos(const int32_t j) const { return rec[j].xy; }
+  __device__ __forceinline__ double2 pos(const int32_t j) const { return vd[j].xy; }
   __device__ __forceinline__ void metric(const int32_t j, double T[3]) const {
-    T[0] = rec[j].t[0];
-    T[1] = rec[j].t[1];
-    T[2] = rec[j].t[2];
+    T[0] = vd[j].t01[0];
+    T[1] = vd[j].t01[1];
+    T[2] = t2[j];
   }
-  __device__ __forceinline__ double4 log(const int32_t j) const { return rec[j].logm; }
+  __device__ __forceinline__ double4 log(const int32_t j) const { return vd[j].logm; }
   __device__ __forceinline__ void commit(const int32_t i, const double px, const double py,
                                          const double TN[3], const double4 LN) const {
-    rec[i].xy = make_double2(px, py);
-    rec[i].t[0] = TN[0];
-    rec[i].t[1] = TN[1];
-    rec[i].t[2] = TN[2];
-    rec[i].logm = LN;
+    VData d;
+    d.logm = LN;
+    d.xy = make_double2(px, py);
+    d.t01[0] = TN[0];
+    d.t01[1] = TN[1];
+    vd[i] = d;
+    t2[i] = TN[2];
     if (xo) {
       xo[i] = make_double2(px, py);
       to[3 * (int64_t)i] = TN[0];
@@ -342,16 +343,17 @@ __device__ __forceinline__ int move_vertex_generic(
 // registers do not weigh on the fast path.  Called only for a move the fast
 // or generic path left undecided (a quality within 16 ulps of q_min).
 static __device__ __noinline__ int move_exact_rec(const int32_t i, const double2 pini,
-                                           VRecord* __restrict__ rec, double2* __restrict__ xo,
-                                           double* __restrict__ to, double* __restrict__ qualit,
+                                           VData* __restrict__ vd, double* __restrict__ t2,
+                                           double2* __restrict__ xo, double* __restrict__ to,
+                                           double* __restrict__ qualit,
                                            const int32_t* __restrict__ v2t_off,
                                            const int32_t* __restrict__ v2t,
                                            const int32_t* __restrict__ star,
                                            const int32_t* __restrict__ v2v_off,
                                            const int32_t* __restrict__ v2v, double* lds,
                                            const int stride) {
-  const RecVerts v{rec, xo, to};
-  const double T[3] = {rec[i].t[0], rec[i].t[1], rec[i].t[2]};
+  const RecVerts v{vd, t2, xo, to};
+  const double T[3] = {vd[i].t01[0], vd[i].t01[1], t2[i]};
   const int32_t t0 = v2t_off[i], deg = v2t_off[i + 1] - t0;
   const int32_t n0 = v2v_off[i], nb = v2v_off[i + 1] - n0;
   return move_vertex_generic<true>(i, t0, deg, n0, nb, pini, T, v, qualit, v2t, star, v2v,
@@ -374,7 +376,9 @@ static __device__ __noinline__ int move_exact_rec(const int32_t i, const double2
 // before its move, so no flag is read before it was written in the call.
 // Within one phase only the vertex itself and vertices of other colours
 // touch stale[i], apart from concurrent commits that all store 0.
-__device__ __forceinline__ int move_vertex_rec(const int32_t i, VRecord* __restrict__ rec,
+__device__ __forceinline__ int move_vertex_rec(const int32_t i, VData* __restrict__ vd,
+                                               double* __restrict__ t2,
+                                               const VTopo* __restrict__ vtopo,
                                                double2* __restrict__ xo, double* __restrict__ to,
                                                double* __restrict__ qualit,
                                                uint8_t* __restrict__ stale,
@@ -384,11 +388,12 @@ __device__ __forceinline__ int move_vertex_rec(const int32_t i, VRecord* __restr
                                                const int32_t* __restrict__ v2v_off,
                                                const int32_t* __restrict__ v2v, double* lds,
                                                const int stride, const int first, int* task) {
-  const RecVerts v{rec, xo, to};
+  const RecVerts v{vd, t2, xo, to};
   if (first) stale[i] = 0;
-  const double2 pini = rec[i].xy;
-  const double T[3] = {rec[i].t[0], rec[i].t[1], rec[i].t[2]};
-  const VTopo topo = rec[i].topo;
+  const VData own = vd[i];
+  const double2 pini = own.xy;
+  const double T[3] = {own.t01[0], own.t01[1], t2[i]};
+  const VTopo topo = vtopo[i];
   *task = 0;
   if (topo.tag & 4) return 0;
   if (topo.meta != kVtopoGeneric) {
@@ -397,8 +402,8 @@ __device__ __forceinline__ int move_vertex_rec(const int32_t i, VRecord* __restr
     const FastTopo ft = fast_topo_rec(i, topo);
     int c = move_vertex_fast(i, ft, pini, T, v, qualit, lds, stride);
     if (c == 2)
-      c = move_exact_rec(i, pini, rec, xo, to, qualit, v2t_off, v2t, star, v2v_off, v2v, lds,
-                         stride);
+      c = move_exact_rec(i, pini, vd, t2, xo, to, qualit, v2t_off, v2t, star, v2v_off, v2v,
+                         lds, stride);
     if (c) {
 #pragma unroll
       for (int k = 0; k < kVtopoDeg; ++k)
@@ -415,8 +420,8 @@ __device__ __forceinline__ int move_vertex_rec(const int32_t i, VRecord* __restr
   int c = move_vertex_generic<false>(i, t0, deg, n0, nb, pini, T, v, qualit, v2t, star, v2v,
                                      lds, stride);
   if (c == 2)
-    c = move_exact_rec(i, pini, rec, xo, to, qualit, v2t_off, v2t, star, v2v_off, v2v, lds,
-                       stride);
+    c = move_exact_rec(i, pini, vd, t2, xo, to, qualit, v2t_off, v2t, star, v2v_off, v2v,
+                       lds, stride);
   if (c) {
     for (int k = 0; k < nb; ++k) stale[v2v[n0 + k]] = 0;
   } else {

@@ -96,10 +96,15 @@ int trn_mesh_autotune(trn_ctx* ctx, trn_mesh* mesh, const double* xy_d, const do
     int32_t* order = buf.alloc<int32_t>((size_t)m->nv);
     uint8_t* stale = buf.alloc<uint8_t>((size_t)m->nv);
     int32_t* counters = buf.alloc<int32_t>(256);
-    VRecord* rec_caller = buf.alloc<VRecord>((size_t)m->nv);    // the records as found
-    VRecord* rec_prepared = buf.alloc<VRecord>((size_t)m->nv);  // each run's starting point
+    // the working copy (vdata, vt2) as found, and each run's starting point
+    const size_t nvd = sizeof(VData) * (size_t)m->nv, nt2 = sizeof(double) * (size_t)m->nv;
+    VData* vd_caller = buf.alloc<VData>((size_t)m->nv);
+    double* t2_caller = buf.alloc<double>((size_t)m->nv);
+    VData* vd_prepared = buf.alloc<VData>((size_t)m->nv);
+    double* t2_prepared = buf.alloc<double>((size_t)m->nv);
     double* q_prepared = buf.alloc<double>((size_t)m->nt);
-    copy_async(ctx, rec_caller, m->vrec, sizeof(VRecord) * (size_t)m->nv);
+    copy_async(ctx, vd_caller, m->vdata, nvd);
+    copy_async(ctx, t2_caller, m->vt2, nt2);
     copy_async(ctx, xy, xy_d, sizeof(double) * 2 * (size_t)m->nv);
     int rc = trn_metric_run(ctx, m, xy, solut_d, 0.8, lp_norm, 1e-9, 0.4, tensor, nullptr,
                             nullptr, nullptr);
@@ -114,7 +119,8 @@ int trn_mesh_autotune(trn_ctx* ctx, trn_mesh* mesh, const double* xy_d, const do
     for (int c = 1; c < nc; ++c)
       if (off[c + 1] - off[c] > off[big + 1] - off[big]) big = c;
     const int32_t count = nc ? off[big + 1] - off[big] : 0;
-    copy_async(ctx, rec_prepared, m->vrec, sizeof(VRecord) * (size_t)m->nv);
+    copy_async(ctx, vd_prepared, m->vdata, nvd);
+    copy_async(ctx, t2_prepared, m->vt2, nt2);
     copy_async(ctx, q_prepared, qualit, sizeof(double) * (size_t)m->nt);
     float phase_t[3] = {0.f, 0.f, 0.f};
     const int minw[3] = {2, 3, 4};
@@ -122,7 +128,8 @@ int trn_mesh_autotune(trn_ctx* ctx, trn_mesh* mesh, const double* xy_d, const do
       for (int c = 0; c < 3; ++c) {
         phase_t[c] = 1e30f;
         for (int r = 0; r < kReps; ++r) {
-          copy_async(ctx, m->vrec, rec_prepared, sizeof(VRecord) * (size_t)m->nv);
+          copy_async(ctx, m->vdata, vd_prepared, nvd);
+          copy_async(ctx, m->vt2, t2_prepared, nt2);
           copy_async(ctx, qualit, q_prepared, sizeof(double) * (size_t)m->nt);
           memset_async(ctx, stale, 0, (size_t)m->nv);
           memset_async(ctx, counters, 0, sizeof(int32_t) * 256);
@@ -133,7 +140,8 @@ int trn_mesh_autotune(trn_ctx* ctx, trn_mesh* mesh, const double* xy_d, const do
         }
       }
     }
-    copy_async(ctx, m->vrec, rec_caller, sizeof(VRecord) * (size_t)m->nv);
+    copy_async(ctx, m->vdata, vd_caller, nvd);
+    copy_async(ctx, m->vt2, t2_caller, nt2);
     sync(ctx);
     const int pb = count > 0 ? (int)(std::min_element(phase_t, phase_t + 3) - phase_t) : 1;
 
