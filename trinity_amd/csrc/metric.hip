@@ -77,32 +77,27 @@ __device__ __forceinline__ void stage_union(const int lane, const int32_t* __res
 }
 
 // Fused staging: the union's per-triangle terms computed in place from the
-// triangles and their corners (the same arithmetic as k_tri_grad /
-// k_tri_hess, so the sums are bit-identical), kFusedBatch entries per lane
-// in flight.  A union never holds a deleted triangle; lanes past the end
-// read triangle 0's corners and store nothing.  (4096^2 grid, ms per
-// gradient / Hessian gather: batches of 3: 0.98 / 1.51, 4: 1.01 / 1.54,
-// 6: 1.13 / 1.60, 8: 1.14 / 1.60.)
+// corners of its triangles (C: three vertex ids per union entry, stored with
+// the mesh) with the same arithmetic as k_tri_grad / k_tri_hess, so the
+// sums are bit-identical; kFusedBatch entries per lane in flight.  Lanes
+// past the end read vertex 0 and store nothing.  (4096^2 grid, ms per
+// gradient / Hessian gather, reading the corners through the triangle
+// array: batches of 3: 0.98 / 1.51, 4: 1.01 / 1.54, 6: 1.13 / 1.60, 8:
+// 1.14 / 1.60.)
 constexpr int kFusedBatch = 3;
 
-__device__ __forceinline__ void stage_grad_terms(const int lane, const int32_t* __restrict__ U,
-                                                 const int nu, const int32_t* __restrict__ tri,
-                                                 const double2* __restrict__ xy,
+__device__ __forceinline__ void stage_grad_terms(const int lane, const int32_t* __restrict__ C,
+                                                 const int nu, const double2* __restrict__ xy,
                                                  const double* __restrict__ f, double* lds_a,
                                                  double2* lds_g) {
   for (int k0 = lane; k0 < nu; k0 += kFusedBatch * kGatherB) {
-    int32_t t[kFusedBatch];
-#pragma unroll
-    for (int c = 0; c < kFusedBatch; ++c) {
-      const int k = k0 + c * kGatherB;
-      t[c] = k < nu ? U[k] : 0;
-    }
     int32_t n[kFusedBatch][3];
 #pragma unroll
     for (int c = 0; c < kFusedBatch; ++c) {
-      const bool in = k0 + c * kGatherB < nu;
+      const int k = k0 + c * kGatherB;
+      const bool in = k < nu;
 #pragma unroll
-      for (int j = 0; j < 3; ++j) n[c][j] = in ? tri[3 * (int64_t)t[c] + j] : 0;
+      for (int j = 0; j < 3; ++j) n[c][j] = in ? C[3 * k + j] : 0;
     }
 #pragma unroll
     for (int c = 0; c < kFusedBatch; ++c) {
@@ -119,24 +114,18 @@ __device__ __forceinline__ void stage_grad_terms(const int lane, const int32_t* 
   }
 }
 
-__device__ __forceinline__ void stage_hess_terms(const int lane, const int32_t* __restrict__ U,
-                                                 const int nu, const int32_t* __restrict__ tri,
-                                                 const double2* __restrict__ xy,
+__device__ __forceinline__ void stage_hess_terms(const int lane, const int32_t* __restrict__ C,
+                                                 const int nu, const double2* __restrict__ xy,
                                                  const double2* __restrict__ grad,
                                                  double4* lds_h) {
   for (int k0 = lane; k0 < nu; k0 += kFusedBatch * kGatherB) {
-    int32_t t[kFusedBatch];
-#pragma unroll
-    for (int c = 0; c < kFusedBatch; ++c) {
-      const int k = k0 + c * kGatherB;
-      t[c] = k < nu ? U[k] : 0;
-    }
     int32_t n[kFusedBatch][3];
 #pragma unroll
     for (int c = 0; c < kFusedBatch; ++c) {
-      const bool in = k0 + c * kGatherB < nu;
+      const int k = k0 + c * kGatherB;
+      const bool in = k < nu;
 #pragma unroll
-      for (int j = 0; j < 3; ++j) n[c][j] = in ? tri[3 * (int64_t)t[c] + j] : 0;
+      for (int j = 0; j < 3; ++j) n[c][j] = in ? C[3 * k + j] : 0;
     }
 #pragma unroll
     for (int c = 0; c < kFusedBatch; ++c) {
@@ -187,8 +176,9 @@ __global__ __launch_bounds__(kGatherB * kGatherWaves) void k_vtx_grad(
     const int32_t* __restrict__ patch_ell, const int32_t* __restrict__ union_n,
     const int32_t* __restrict__ union_off, const int32_t* __restrict__ union_ids,
     const uint16_t* __restrict__ patch_local, int max_union, int wave_bytes,
-    const int32_t* __restrict__ tri, const double2* __restrict__ xy,
-    const double* __restrict__ f, const double* __restrict__ area_t,
+    const int32_t* __restrict__ union_cu, const int32_t* __restrict__ union_tri,
+    const double2* __restrict__ xy, const double* __restrict__ f,
+    const double* __restrict__ area_t,
     const double2* __restrict__ ag_t, double2* __restrict__ grad, double* __restrict__ tarea) {
   extern __shared__ double2 lds_all[];
   const GatherSlot g = gather_slot();
@@ -205,7 +195,7 @@ __global__ __launch_bounds__(kGatherB * kGatherWaves) void k_vtx_grad(
     rows = patch_row[g.b + 1] - r0;
     if (rows <= kRegRows) load_column(patch_local + (int64_t)r0 * kPatchSlab + g.lane, rows, loc);
     if (FUSED) {
-      stage_grad_terms(g.lane, U, nu, tri, xy, f, lds_a, lds_g);
+      stage_grad_terms(g.lane, union_tri + 3 * (int64_t)union_cu[g.b], nu, xy, f, lds_a, lds_g);
     } else {
       stage_union(g.lane, U, nu, area_t, lds_a);
       if (WITH_GRAD) stage_union(g.lane, U, nu, ag_t, lds_g);
@@ -246,7 +236,8 @@ __global__ __launch_bounds__(kGatherB * kGatherWaves) void k_vtx_hess(
     const int32_t* __restrict__ patch_n, const int32_t* __restrict__ patch_row,
     const int32_t* __restrict__ patch_ell, const int32_t* __restrict__ union_n,
     const int32_t* __restrict__ union_off, const int32_t* __restrict__ union_ids,
-    const uint16_t* __restrict__ patch_local, int wave_bytes, const int32_t* __restrict__ tri,
+    const uint16_t* __restrict__ patch_local, int wave_bytes,
+    const int32_t* __restrict__ union_cu, const int32_t* __restrict__ union_tri,
     const double2* __restrict__ xy, const double2* __restrict__ grad,
     const double4* __restrict__ ah_t, const double* __restrict__ tarea,
     double* __restrict__ hess, double* __restrict__ out, double ex) {
@@ -259,11 +250,10 @@ __global__ __launch_bounds__(kGatherB * kGatherWaves) void k_vtx_hess(
   int32_t r0 = 0, rows = 0;
   int loc[kRegRows];
   if (nu >= 0) {
-    const int32_t* U = union_ids + union_off[g.b];
     r0 = patch_row[g.b];
     rows = patch_row[g.b + 1] - r0;
     if (rows <= kRegRows) load_column(patch_local + (int64_t)r0 * kPatchSlab + g.lane, rows, loc);
-    stage_hess_terms(g.lane, U, nu, tri, xy, grad, lds_h);
+    stage_hess_terms(g.lane, union_tri + 3 * (int64_t)union_cu[g.b], nu, xy, grad, lds_h);
   }
   __syncthreads();
   if (!live || pos >= nv) return;
@@ -402,8 +392,9 @@ static void launch_vtx_grad(trn_ctx* ctx, const char* name, double bytes, bool w
          (const int32_t*)m->patch_n, (const int32_t*)m->patch_row,
          (const int32_t*)m->patch_ell, (const int32_t*)m->union_n,
          (const int32_t*)m->union_off, (const int32_t*)m->union_ids,
-         (const uint16_t*)m->patch_local, m->max_union, g.wave_bytes, (const int32_t*)m->tri,
-         (const double2*)xy, f, area_t, ag_t, grad, tarea);
+         (const uint16_t*)m->patch_local, m->max_union, g.wave_bytes,
+         (const int32_t*)m->union_cu, (const int32_t*)m->union_tri, (const double2*)xy, f,
+         area_t, ag_t, grad, tarea);
 }
 
 void patch_area_sums(trn_ctx* ctx, const trn_mesh* m, const double* area_t, double* tarea) {
@@ -432,8 +423,8 @@ static void launch_vtx_hess(trn_ctx* ctx, const trn_mesh* m, const double* xy,
          (const int32_t*)m->patch_row, (const int32_t*)m->patch_ell,
          (const int32_t*)m->union_n, (const int32_t*)m->union_off,
          (const int32_t*)m->union_ids, (const uint16_t*)m->patch_local, g.wave_bytes,
-         (const int32_t*)m->tri, (const double2*)xy, (const double2*)grad, ah_t, tarea, hess,
-         out, ex);
+         (const int32_t*)m->union_cu, (const int32_t*)m->union_tri, (const double2*)xy,
+         (const double2*)grad, ah_t, tarea, hess, out, ex);
 }
 
 static void run_hessian(trn_ctx* ctx, const trn_mesh* m, const double* xy, const double* grad,

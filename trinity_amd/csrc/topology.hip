@@ -465,6 +465,30 @@ __global__ __launch_bounds__(kUnionB) void k_patch_union(
   }
 }
 
+// Packed offsets of the slab unions: cnt[b] = union size (0 without one).
+__global__ void k_union_sizes(int64_t nslab, const int32_t* __restrict__ union_n,
+                              int32_t* __restrict__ cnt) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nslab) cnt[b] = union_n[b] > 0 ? union_n[b] : 0;
+}
+
+// The corners of every union entry's triangle, slab b's from union_cu[b]
+// (one block per slab).
+__global__ __launch_bounds__(256) void k_union_tri(const int32_t* __restrict__ union_n,
+                                                   const int32_t* __restrict__ union_off,
+                                                   const int32_t* __restrict__ union_ids,
+                                                   const int32_t* __restrict__ union_cu,
+                                                   const int32_t* __restrict__ tri,
+                                                   int32_t* __restrict__ union_tri) {
+  const int b = blockIdx.x;
+  const int nu = union_n[b];
+  if (nu <= 0) return;
+  const int32_t* U = union_ids + union_off[b];
+  int32_t* C = union_tri + 3 * (int64_t)union_cu[b];
+  for (int k = threadIdx.x; k < 3 * nu; k += blockDim.x)
+    C[k] = tri[3 * (int64_t)U[k / 3] + k % 3];
+}
+
 // Star codes (see trn_mesh::star).  Corners are located in the sorted 1-ring
 // by binary search; every corner of an incident triangle is in the 1-ring
 // by construction of v2v.
@@ -688,10 +712,20 @@ int trn_mesh_create(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
       memset_async(ctx, flags + 8, 0, sizeof(int32_t));
       launch(ctx, "topo_union_count", 0.0, k_count_negative, dim3(blocks_for(nslab, 256)),
              dim3(256), 0, nslab, (const int32_t*)m->union_n, flags + 8);
+      TRN_HIP(hipMalloc(&m->union_cu, sizeof(int32_t) * (size_t)(nslab + 1)));
+      launch(ctx, "topo_union_sizes", 0.0, k_union_sizes, dim3(blocks_for(nslab, 256)),
+             dim3(256), 0, nslab, (const int32_t*)m->union_n, cnt2);
+      exclusive_scan_i32(ctx, cnt2, m->union_cu, nslab, true, scan_tmp);
       copy_async(ctx, h + 1, flags + 7, 2 * sizeof(int32_t));
+      copy_async(ctx, h + 3, m->union_cu + nslab, sizeof(int32_t));
       sync(ctx);
       m->max_union = h[1] > 0 ? h[1] : 0;
       m->union_all = h[2] == 0;
+      TRN_HIP(hipMalloc(&m->union_tri, sizeof(int32_t) * 3 * (size_t)(h[3] > 0 ? h[3] : 1)));
+      launch(ctx, "topo_union_tri", 0.0, k_union_tri, dim3((unsigned)nslab), dim3(256), 0,
+             (const int32_t*)m->union_n, (const int32_t*)m->union_off,
+             (const int32_t*)m->union_ids, (const int32_t*)m->union_cu, (const int32_t*)m->tri,
+             m->union_tri);
 
       // 5. star codes of the incident triangles (smoothing reads corners
       //    through the 1-ring instead of the triangle array)
@@ -738,6 +772,8 @@ int trn_mesh_destroy(trn_mesh* m) {
     if (m->union_n) (void)hipFree(m->union_n);
     if (m->union_off) (void)hipFree(m->union_off);
     if (m->union_ids) (void)hipFree(m->union_ids);
+    if (m->union_cu) (void)hipFree(m->union_cu);
+    if (m->union_tri) (void)hipFree(m->union_tri);
     if (m->patch_local) (void)hipFree(m->patch_local);
     delete m;
   });

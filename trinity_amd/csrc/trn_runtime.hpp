@@ -165,6 +165,11 @@ struct trn_mesh {
   int32_t* union_off = nullptr;      // [nslab + 1]
   int32_t* union_ids = nullptr;      // [patch_row[nslab] * kPatchSlab]
   uint16_t* patch_local = nullptr;   // [patch_row[nslab] * kPatchSlab]
+  // The same unions packed one after the other (slab b's entries from
+  // union_cu[b]), as the three corners of each entry's triangle: the
+  // gathers stage a union entry's terms from its corners directly.
+  int32_t* union_cu = nullptr;       // [nslab + 1]
+  int32_t* union_tri = nullptr;      // [3 * union_cu[nslab]]
   int32_t max_union = 0;             // largest union_n
   bool union_all = false;            // every slab has a union
   // Launch shapes of the plain kernels (trn_mesh_autotune measures them on
