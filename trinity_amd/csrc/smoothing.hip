@@ -54,23 +54,41 @@ __global__ __launch_bounds__(256) void k_quality_cache(int32_t nt, const int32_t
 
 // The sweeps' working copy (metric log, coordinates, metric) from the
 // caller's arrays, and the logs as a dense array for the quality cache;
-// replaces k_tensor_log for the sweeps.
-__global__ __launch_bounds__(256) void k_pack_records(int32_t nv, const double2* __restrict__ xy,
-                                                      const double* __restrict__ tensor,
-                                                      VData* __restrict__ vd,
-                                                      double* __restrict__ t2,
-                                                      double4* __restrict__ logm) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nv) return;
-  const double t0 = tensor[3 * i], t1 = tensor[3 * i + 1], t22 = tensor[3 * i + 2];
-  VData d;
-  d.logm = tensor_log4(t0, t1, t22);
-  d.xy = xy[i];
-  d.t01[0] = t0;
-  d.t01[1] = t1;
-  vd[i] = d;
-  t2[i] = t22;
-  logm[i] = d.logm;
+// replaces k_tensor_log for the sweeps.  A block stages its 256 records and
+// logs in LDS and stores them with consecutive lanes on consecutive 16-byte
+// chunks, so every store instruction writes whole lines (a record is 5 LDS
+// chunks apart, a log 3, so the lanes writing them hit distinct banks).
+constexpr int kPackB = 256;
+
+__global__ __launch_bounds__(kPackB) void k_pack_records(int32_t nv, const double2* __restrict__ xy,
+                                                         const double* __restrict__ tensor,
+                                                         VData* __restrict__ vd,
+                                                         double* __restrict__ t2,
+                                                         double4* __restrict__ logm) {
+  static_assert(sizeof(VData) == 4 * sizeof(double2), "a record is 4 chunks of 16 bytes");
+  __shared__ double2 rec[kPackB * 5];
+  __shared__ double2 lg[kPackB * 3];
+  const int64_t v0 = (int64_t)blockIdx.x * kPackB;
+  const int n = nv - v0 < kPackB ? (int)(nv - v0) : kPackB;
+  const int tid = threadIdx.x;
+  if (tid < n) {
+    const int64_t i = v0 + tid;
+    const double t0 = tensor[3 * i], t1 = tensor[3 * i + 1], t22 = tensor[3 * i + 2];
+    const double4 L = tensor_log4(t0, t1, t22);
+    double2* r = rec + 5 * tid;
+    r[0] = make_double2(L.x, L.y);
+    r[1] = make_double2(L.z, L.w);
+    r[2] = xy[i];
+    r[3] = make_double2(t0, t1);
+    lg[3 * tid] = r[0];
+    lg[3 * tid + 1] = r[1];
+    t2[i] = t22;
+  }
+  __syncthreads();
+  double2* vdc = reinterpret_cast<double2*>(vd + v0);
+  for (int c = tid; c < 4 * n; c += kPackB) vdc[c] = rec[5 * (c >> 2) + (c & 3)];
+  double2* lgc = reinterpret_cast<double2*>(logm + v0);
+  for (int c = tid; c < 2 * n; c += kPackB) lgc[c] = lg[3 * (c >> 1) + (c & 1)];
 }
 
 // One colour phase.  count_d == nullptr: `order` lists `count` vertices
@@ -331,8 +349,8 @@ __global__ __launch_bounds__(kStatB) void k_quality_final(int32_t nparts,
 void smooth_prepare_launch(trn_ctx* ctx, const trn_mesh* m, const double* xy,
                            const double* tensor, double* qualit, bool recompute_qualit,
                            double4* logm) {
-  launch(ctx, "pack_records", 144.0 * m->nv, k_pack_records, dim3(nblk(m->nv, 256)), dim3(256),
-         0, m->nv, (const double2*)xy, tensor, m->vdata, m->vt2, logm);
+  launch(ctx, "pack_records", 144.0 * m->nv, k_pack_records, dim3(nblk(m->nv, kPackB)),
+         dim3(kPackB), 0, m->nv, (const double2*)xy, tensor, m->vdata, m->vt2, logm);
   if (recompute_qualit) quality_cache_logs(ctx, m, xy, logm, qualit);
 }
 
