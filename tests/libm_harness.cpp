@@ -2,7 +2,7 @@
 // Built by tests/test_libm.py with g++ (no floating-point contraction):
 // the same source the device compiles, evaluated on the host.
 //
-//   libm_harness eval <op> <file_in> <file_out>   op: exp log pow log1p_small expm1
+//   libm_harness eval <op> <file_in> <file_out>   op: exp log pow log1p_small expm1 cube
 //       evaluates trn::libm on the raw doubles of file_in (pairs for pow)
 //   libm_harness sweep <op> <n> <seed>
 //       draws n inputs per range and prints "range mismatches n" lines,
@@ -52,6 +52,7 @@ int main(int argc, char** argv) {
         if (!strcmp(op, "exp")) out.push_back(trn::libm::exp(x));
         else if (!strcmp(op, "log")) out.push_back(trn::libm::log(x));
         else if (!strcmp(op, "log1p_small")) out.push_back(trn::libm::log1p_small(x));
+        else if (!strcmp(op, "cube")) out.push_back(trn::libm::cube(x));
         else out.push_back(trn::libm::expm1(x));
       }
     }
@@ -93,6 +94,23 @@ int main(int argc, char** argv) {
         if (!same(g, trn::libm::pow(x, y))) ++bad;
       }
       printf("pow%d %ld %ld\n", r, bad, n);
+    }
+  } else if (!strcmp(op, "cube")) {
+    // the quality's m (2 - m) in [0, 1], general magnitudes of both signs,
+    // arguments whose cubes are subnormal or overflow; also counts how
+    // often the fast form defers to pow (cube_pow lines)
+    for (int r = 0; r < 4; ++r) {
+      long bad = 0;
+      for (long i = 0; i < n; ++i) {
+        double x;
+        if (r == 0) x = urand();
+        else if (r == 1) x = (urand() * 2 - 1) * exp((urand() * 2 - 1) * 200);
+        else if (r == 2) x = exp(-urand() * 260);
+        else x = exp(230 + urand() * 20) * (urand() < 0.5 ? -1 : 1);
+        volatile double g = pow(x, 3.0);
+        if (!same(g, trn::libm::cube(x))) ++bad;
+      }
+      printf("cube%d %ld %ld\n", r, bad, n);
     }
   } else {
     // eigenvalues of metrics (wide), the Riemann-distance ratio (near 1),

@@ -4,7 +4,7 @@ host C library the reference links, on the CPU.
 The header is compiled with g++ into tests/libm_harness.cpp; the same source
 is what the GPU kernels inline.  Bars:
   exp, pow   bit-identical to the host libm on every input drawn (millions
-             over the ranges the path uses, the scaled and subnormal ranges,
+  and cube   over the ranges the path uses, the scaled and subnormal ranges,
              special values);
   log        correctly rounded (checked against 300-bit mpmath values), so
              equal to the host's log wherever the host's is correctly
@@ -88,6 +88,14 @@ def test_exp_bit_identical_to_host_libm(harness):
 
 def test_pow_bit_identical_to_host_libm(harness):
     for name, (bad, tot) in sweep(harness, "pow", 500_000, 12).items():
+        assert bad == 0, f"{name}: {bad} of {tot} differ"
+
+
+def test_cube_bit_identical_to_host_pow(harness):
+    """libm::cube (the quality's pow(x, 3), mostly from the double-double
+    cube) equals the host's pow(x, 3.0) bit for bit, on the quality's
+    arguments in [0, 1] and on general, tiny and huge ones."""
+    for name, (bad, tot) in sweep(harness, "cube", 1_000_000, 14).items():
         assert bad == 0, f"{name}: {bad} of {tot} differ"
 
 

@@ -121,7 +121,7 @@ __device__ __forceinline__ void interpolate3(const double* ta, const double* tb,
 
 // pow(x, 3) as numeric.cpp:304 evaluates it: the C library's pow (which is
 // not x*x*x, nor always the correctly rounded cube).
-__device__ __forceinline__ double cube_pow(const double x) { return libm::pow(x, 3.); }
+__device__ __forceinline__ double cube_pow(const double x) { return libm::cube(x); }
 
 // x^3 rounded once: x^2 = p + e exactly (FMA error term), p*x = q + f
 // exactly, x^3 = q + f + e*x with e*x below 2^-52 of q, so the final sum
