@@ -890,7 +890,7 @@ def main():
                                                     color_seed=args.color_seed)
         extras["refine_filter"] = run_refine(T, mg, ctx, wl2, 2, seed=args.seed)
         extras["indep_set"] = run_indep_set(T, mg, ctx, wl2, seed=args.seed)
-        extras["adjoint"] = run_adjoint(T, mg, ctx, "jitter1024:fronts+quad", 5)
+        extras["adjoint"] = run_adjoint(T, mg, ctx, "jitter1024:tanh", 5)
         extras["adjoint_headline_mesh"] = run_adjoint(T, mg, ctx, args.workload, 2, p=args.p)
         extras["smoothing_modes"] = compare_smoothing(T, mg, ctx, "jitter256:fronts+quad", 5)
 
