@@ -128,6 +128,19 @@ Scratch::Scratch(trn_ctx* ctx, size_t total) : ctx_(ctx) {
 
 Scratch::~Scratch() { ctx_->ws.live = 0; }
 
+void* aux_workspace(trn_ctx* ctx, size_t bytes) {
+  if (bytes > ctx->aux.size) {
+    TRN_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->aux.base) TRN_HIP(hipFree(ctx->aux.base));
+    ctx->aux.base = nullptr;
+    ctx->aux.size = 0;
+    const size_t sz = bytes + bytes / 4;
+    TRN_HIP(hipMalloc(&ctx->aux.base, sz));
+    ctx->aux.size = sz;
+  }
+  return ctx->aux.base;
+}
+
 void sync(trn_ctx* ctx) { TRN_HIP(hipStreamSynchronize(ctx->stream)); }
 
 // Byte fill as a kernel rather than hipMemsetAsync, so that every sequence
@@ -233,6 +246,7 @@ int trn_ctx_destroy(trn_ctx* ctx) {
     }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     if (ctx->ws.base) (void)hipFree(ctx->ws.base);
+    if (ctx->aux.base) (void)hipFree(ctx->aux.base);
     if (ctx->pinned_i32) (void)hipHostFree(ctx->pinned_i32);
     if (ctx->pinned_f64) (void)hipHostFree(ctx->pinned_f64);
     if (ctx->unit_dev) (void)hipFree(ctx->unit_dev);

@@ -269,16 +269,21 @@ __global__ void k_patch_bound(int32_t nv, const int32_t* __restrict__ v2t_off,
   bound[i] = s;
 }
 
-// Exact 2-ring element patch size (getVicinity(i,2), mesh.cpp:390-434).
-__global__ __launch_bounds__(64) void k_patch_size(int32_t nv, const int32_t* __restrict__ v2t_off,
+// Exact 2-ring element patch (getVicinity(i,2), mesh.cpp:390-434) of the
+// vertex at slab position pos (vertex slab_v[pos], or pos): its size, and,
+// when tmp is given, the ascending patch itself in tmp[e * nv + pos]
+// (e < size <= cap), for k_patch_copy.
+__global__ __launch_bounds__(64) void k_patch_size(int32_t nv, const int32_t* __restrict__ slab_v,
+                                                   const int32_t* __restrict__ v2t_off,
                                                    const int32_t* __restrict__ v2t,
                                                    const int32_t* __restrict__ v2v_off,
                                                    const int32_t* __restrict__ v2v, int cap,
                                                    int32_t* __restrict__ size,
-                                                   int32_t* overflow) {
+                                                   int32_t* overflow, int32_t* __restrict__ tmp) {
   extern __shared__ int lds[];
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nv) return;
+  const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= nv) return;
+  const int64_t i = slab_v ? slab_v[pos] : pos;
   int* s = lds + threadIdx.x;
   const int stride = blockDim.x;
   int m = 0;
@@ -292,6 +297,8 @@ __global__ __launch_bounds__(64) void k_patch_size(int32_t nv, const int32_t* __
   }
   if (!ok) atomicAdd(overflow, 1);
   size[i] = m;
+  if (tmp && ok)
+    for (int e = 0; e < m; ++e) tmp[(int64_t)e * nv + pos] = s[e * stride];
 }
 
 // Rows of each patch slab: the largest patch among its kPatchSlab vertices
@@ -338,6 +345,25 @@ __global__ __launch_bounds__(kPatchSlab) void k_patch_fill(
   const int32_t r0 = row[blockIdx.x], w = row[blockIdx.x + 1] - r0;
   int32_t* col = ell + (int64_t)r0 * kPatchSlab + threadIdx.x;
   for (int e = 0; e < w; ++e) col[(int64_t)e * kPatchSlab] = e < m ? s[e * kPatchSlab] : -1;
+}
+
+// Largest temporary patch buffer mesh creation allocates (beyond it the
+// patches are built a second time for the ELL fill instead of copied).
+constexpr size_t kPatchTmpMax = size_t(8) << 30;
+
+// The stored patches from the ones k_patch_size kept in tmp (column-major
+// by slab position, tmp[e * nv + pos]), written column-wise into their
+// slabs as k_patch_fill writes them.
+__global__ __launch_bounds__(kPatchSlab) void k_patch_copy(
+    int32_t nv, const int32_t* __restrict__ slab_v, const int32_t* __restrict__ size,
+    const int32_t* __restrict__ tmp, const int32_t* __restrict__ row, int32_t* __restrict__ ell) {
+  const int64_t pos = (int64_t)blockIdx.x * kPatchSlab + threadIdx.x;
+  const int64_t i = pos < nv ? (slab_v ? slab_v[pos] : pos) : nv;
+  const int m = i < nv ? size[i] : 0;
+  const int32_t r0 = row[blockIdx.x], w = row[blockIdx.x + 1] - r0;
+  int32_t* col = ell + (int64_t)r0 * kPatchSlab + threadIdx.x;
+  for (int e = 0; e < w; ++e)
+    col[(int64_t)e * kPatchSlab] = e < m ? tmp[(int64_t)e * nv + pos] : -1;
 }
 
 // Start of each slab's union: the slab's own ELL block (patch_row[b] *
@@ -666,10 +692,15 @@ int trn_mesh_create(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
       int capp = h[0] < kMaxPatchCap ? h[0] : kMaxPatchCap;
       if (capp < 1) capp = 1;
       memset_async(ctx, flags + 5, 0, sizeof(int32_t));
+      // the patches themselves are kept for the ELL fill below when the
+      // bound makes the buffer affordable (else they are built again there)
+      int32_t* patch_tmp = nullptr;
+      if ((size_t)capp * (size_t)nv * sizeof(int32_t) <= kPatchTmpMax)
+        patch_tmp = (int32_t*)aux_workspace(ctx, (size_t)capp * (size_t)nv * sizeof(int32_t));
       launch(ctx, "topo_patch_size", 0.0, k_patch_size, dim3(blocks_for(nv, 64)), dim3(64),
-             sizeof(int) * 64 * (size_t)capp, nv, (const int32_t*)m->v2t_off,
-             (const int32_t*)m->v2t, (const int32_t*)m->v2v_off, (const int32_t*)m->v2v, capp,
-             bound, flags + 5);
+             sizeof(int) * 64 * (size_t)capp, nv, (const int32_t*)m->slab_v,
+             (const int32_t*)m->v2t_off, (const int32_t*)m->v2t, (const int32_t*)m->v2v_off,
+             (const int32_t*)m->v2v, capp, bound, flags + 5, patch_tmp);
       reduce_max_i32(ctx, bound, nv, flags + 6);
       copy_async(ctx, h, flags + 5, 2 * sizeof(int32_t));
       sync(ctx);
@@ -691,11 +722,18 @@ int trn_mesh_create(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
       sync(ctx);
       const int64_t ell_n = (int64_t)h[0] * kPatchSlab;
       TRN_HIP(hipMalloc(&m->patch_ell, sizeof(int32_t) * (size_t)(ell_n > 0 ? ell_n : 1)));
-      launch(ctx, "topo_patch_fill", 0.0, k_patch_fill, dim3((unsigned)nslab), dim3(kPatchSlab),
-             sizeof(int) * kPatchSlab * (size_t)m->patch_cap, nv, (const int32_t*)m->slab_v,
-             (const int32_t*)m->v2t_off,
-             (const int32_t*)m->v2t, (const int32_t*)m->v2v_off, (const int32_t*)m->v2v,
-             m->patch_cap, (const int32_t*)m->patch_row, m->patch_ell);
+      if (patch_tmp) {
+        launch(ctx, "topo_patch_copy", 0.0, k_patch_copy, dim3((unsigned)nslab),
+               dim3(kPatchSlab), 0, nv, (const int32_t*)m->slab_v, (const int32_t*)bound,
+               (const int32_t*)patch_tmp, (const int32_t*)m->patch_row, m->patch_ell);
+      } else {
+        launch(ctx, "topo_patch_fill", 0.0, k_patch_fill, dim3((unsigned)nslab),
+               dim3(kPatchSlab), sizeof(int) * kPatchSlab * (size_t)m->patch_cap, nv,
+               (const int32_t*)m->slab_v, (const int32_t*)m->v2t_off, (const int32_t*)m->v2t,
+               (const int32_t*)m->v2v_off, (const int32_t*)m->v2v, m->patch_cap,
+               (const int32_t*)m->patch_row, m->patch_ell);
+      }
+
 
       // 4c. per-slab unions of the patches (LDS staging of the gathers),
       //     each stored in its slab's ELL block: one sort per slab

@@ -65,6 +65,7 @@ struct trn_ctx {
   hipStream_t stream = nullptr;      // stream every launch goes to
   hipStream_t own_stream = nullptr;  // the context's own (default priority)
   trn::Workspace ws;
+  trn::Workspace aux;                // temporaries of mesh creation (grows)
   bool timing = false;
   std::string timing_only;           // non-empty: time only launches of this name
   std::vector<trn::KernelStat> stats;
@@ -223,6 +224,12 @@ inline void launch(trn_ctx* ctx, const char* name, double bytes, K kernel,
 }
 
 // Scratch slices from the context workspace.  An entry point sizes all of
+// A second growing device buffer of the context for the temporaries of mesh
+// creation (the stored patches before their ELL copy), held until the
+// context is destroyed.  Queued work reading an earlier use must have ended
+// (it synchronises before growing).
+void* aux_workspace(trn_ctx* ctx, size_t bytes);
+
 // its scratch up front (Scratch(ctx, total)), which grows the workspace if
 // needed, then carves slices in order.  Slices live until the next entry
 // point on the same context; every entry point is stream-ordered on
