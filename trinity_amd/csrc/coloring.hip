@@ -9,7 +9,7 @@
 // tile size), in three steps:
 //   tiles:    the vertices are cut into tiles of consecutive ids; one wave
 //             per tile runs the sequential first fit over its tile in
-//             index order, in LDS, seeing only neighbours inside the tile
+//             index order, seeing only neighbours inside the tile
 //             (k_color_tiles).  On a mesh numbered with locality most edges
 //             stay inside a tile, so this is the reference's one-thread
 //             colouring almost everywhere;
@@ -185,9 +185,6 @@ __global__ void k_color_check(int32_t nv, const int32_t* __restrict__ v2v_off,
 constexpr int kTileB = 64;                    // one wave per tile
 constexpr int kTileMax = 16384;               // largest tile (local ids fit 16 bits)
 
-// LDS of a tile block: the tile's colours, one byte per vertex.
-static size_t tile_shmem(int tile) { return ((size_t)tile + 15) & ~size_t(15); }
-
 __device__ __forceinline__ uint64_t lane_u64(const uint32_t lo, const uint32_t hi, const int k) {
   return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)hi, k) << 32) |
          (uint32_t)__builtin_amdgcn_readlane((int)lo, k);
@@ -198,10 +195,10 @@ __device__ __forceinline__ uint64_t lane_u64(const uint32_t lo, const uint32_t h
 // their own earlier vertices; odd tiles run second and also avoid the
 // colours of their neighbours in even tiles, which are final by then.
 // Per 64-vertex chunk every lane gathers, for its vertex, the mask of
-// colours its earlier neighbours outside the chunk hold (in-tile ones from
-// the tile's colours in LDS, even-tile ones from memory) and the mask of
-// its earlier neighbours inside the chunk; then the chunk is coloured in
-// index order, in one of two ways with the same result:
+// colours its earlier neighbours outside the chunk hold (read from `color`,
+// where the wave stores each chunk's colours as it finishes it) and the
+// mask of its earlier neighbours inside the chunk; then the chunk is
+// coloured in index order, in one of two ways with the same result:
 //   chains: when no vertex of the chunk has two earlier neighbours inside
 //           it (a mesh numbered row by row: the only one is v - 1), the
 //           colour of vertex k is a function of its in-chunk parent's
@@ -217,6 +214,13 @@ __device__ __forceinline__ uint64_t lane_u64(const uint32_t lo, const uint32_t h
 // taken is written -1 and counted in *given_up.  Edges between tiles of
 // the same parity (only where the numbering has edges longer than a tile)
 // are left to the seam check.
+// The kernel keeps no LDS: it runs beside the recovery gathers, whose
+// occupancy is bound by their LDS, and the 1-16 KB a tile would hold per
+// wave (8 waves per CU on a 4096^2 grid) took gather blocks off every CU
+// (the gradient gather beside it: 1.20 ms with the tiles in LDS, 1.03 ms
+// without).  In-tile
+// colours of earlier chunks are read back from `color`, where the block
+// barrier after each chunk makes its stores visible to the block.
 constexpr uint32_t kGaveUp = 0xFF;
 
 // The smallest colour absent from `used`, or kGaveUp past the palette.
@@ -231,7 +235,6 @@ __global__ __launch_bounds__(kTileB) void k_color_tiles(int32_t nv, int32_t tile
                                                         int max_colors,
                                                         int32_t* __restrict__ color,
                                                         int32_t* __restrict__ given_up) {
-  extern __shared__ uint8_t col[];  // [tile] colours of the tile (0: none, 0xFF: given up)
   const int64_t tb = 2 * (int64_t)blockIdx.x + parity;
   const int64_t t0 = tb * tile;
   if (t0 >= nv) return;
@@ -250,9 +253,9 @@ __global__ __launch_bounds__(kTileB) void k_color_tiles(int32_t nv, int32_t tile
       const int32_t u = v2v[a];
       if (u >= c0 && u < v) {
         near |= 1ull << (u - c0);
-      } else if (u >= t0 && u < c0) {
-        const int cu = col[u - t0];
-        if (cu != 0 && cu != 0xFF) far |= 1ull << (cu - 1);
+      } else if (u >= t0 && u < c0) {  // an earlier chunk of this tile
+        const int32_t cu = color[u];
+        if (cu > 0) far |= 1ull << (cu - 1);
       } else if (parity && (u < t0 || u >= t1) && ((u / tile) & 1) == 0) {
         const int32_t cu = color[u];  // an even tile, finished
         if (cu > 0) far |= 1ull << (cu - 1);
@@ -318,12 +321,8 @@ __global__ __launch_bounds__(kTileB) void k_color_tiles(int32_t nv, int32_t tile
         }
       }
     }
-    if (v < t1) col[v - t0] = (uint8_t)mine;
-    __syncthreads();
-  }
-  for (int64_t k = lane; k < t1 - t0; k += kTileB) {
-    const int c = col[k];
-    color[t0 + k] = c == 0xFF ? -1 : c;
+    if (v < t1) color[v] = mine == (int)kGaveUp ? -1 : mine;
+    __syncthreads();  // the chunk's colours visible to the later chunks' loads
   }
   if (lane == 0 && nfail) atomicAdd(given_up, nfail);
 }
@@ -703,7 +702,7 @@ static void color_impl(trn_ctx* ctx, const trn_mesh* mesh, uint64_t seed, int32_
     const int64_t nb = (ntiles + 1 - parity) / 2;
     if (nb > 0)
       launch(ctx, "color_tiles", 4.0 * nv + 2.0 * (double)mesh->nnz_v2v, k_color_tiles,
-             dim3((unsigned)nb), dim3(kTileB), tile_shmem(tile), nv, tile, parity,
+             dim3((unsigned)nb), dim3(kTileB), 0, nv, tile, parity,
              (const int32_t*)mesh->v2v_off, (const int32_t*)mesh->v2v, (int)max_colors,
              color_d, leave ? flags + 2 : flags);
   }
