@@ -379,10 +379,10 @@ __global__ void k_union_offsets(int32_t nslab, const int32_t* __restrict__ row,
 // entries or its union larger than kUnionCap), the ascending union at
 // union_off[b] = the slab's ELL offset, and, for every patch entry, its
 // position in the union.  The slab's entries go into an open-addressing
-// hash set in LDS (the distinct ids, whatever the numbering's spread), the
-// distinct ids are sorted (bitonic, about 230 on a grid slab instead of its
-// 1,536 entries), and each slot records its id's position, which the
-// entries then look up.
+// hash set in LDS (the distinct ids, whatever the numbering's spread), each
+// distinct id's ascending position is counted among the distinct ids
+// (about 300 on a grid slab instead of its 1,536 entries), and each slot
+// records its id's position, which the entries then look up.
 constexpr int kUnionB = 256;
 constexpr int kUnionSlots = kUnionKeys;  // hash slots: one per entry at most
 constexpr int kUnionPer = kUnionKeys / kUnionB;      // entries per thread
@@ -399,7 +399,7 @@ __global__ __launch_bounds__(kUnionB) void k_patch_union(
     int32_t* __restrict__ union_ids, uint16_t* __restrict__ local) {
   __shared__ int32_t tab[kUnionSlots];    // -1: empty
   __shared__ uint16_t rank[kUnionSlots];  // union position of the id in the slot
-  __shared__ int32_t list[kUnionCap];     // the distinct ids, then sorted
+  __shared__ int32_t list[kUnionCap];     // the distinct ids
   __shared__ int32_t count;
   const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
   const int64_t r0 = row[b];
@@ -447,32 +447,19 @@ __global__ __launch_bounds__(kUnionB) void k_patch_union(
   const int nu = count;
   if (tid == 0) union_n[b] = nu <= kUnionCap ? nu : -1;
   if (nu > kUnionCap) return;  // nu is the block's: every thread leaves
-  int P = 1;
-  while (P < nu) P <<= 1;
-  for (int k = nu + tid; k < P; k += kUnionB) list[k] = INT32_MAX;
-  __syncthreads();
-  for (int size = 2; size <= P; size <<= 1)
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int k = tid; k < P; k += kUnionB) {
-        const int q = k ^ stride;
-        if (q > k) {
-          const bool up = (k & size) == 0;
-          const int32_t x = list[k], y = list[q];
-          if ((x > y) == up) {
-            list[k] = y;
-            list[q] = x;
-          }
-        }
-      }
-      __syncthreads();
-    }
+  // ascending position of each distinct id: the number of smaller ones
+  // (every thread reads the same list entry at a time, an LDS broadcast;
+  // about 300 ids per slab on a grid, where a sorting network would take
+  // 45 barrier-separated stages)
   int32_t* U = union_ids + union_off[b];
   for (int k = tid; k < nu; k += kUnionB) {
     const int32_t t = list[k];
-    U[k] = t;
+    int r = 0;
+    for (int j = 0; j < nu; ++j) r += list[j] < t;
+    U[r] = t;
     uint32_t h = union_slot(t);
     while (tab[h] != t) h = (h + 1) & (kUnionSlots - 1);  // present: ends
-    rank[h] = (uint16_t)k;
+    rank[h] = (uint16_t)r;
   }
   __syncthreads();
   uint16_t* dst = local + r0 * kPatchSlab;
