@@ -421,27 +421,29 @@ __global__ __launch_bounds__(kUnionB) void k_patch_union(
     ent[e] = (k < n && v0 + (k % kPatchSlab) < nv) ? src[k] : -1;
   }
   __syncthreads();
+  // insertion; the entry that claims a slot also appends its id to the
+  // list of distinct ids, in any order (one LDS atomic per wave and pass)
 #pragma unroll
   for (int e = 0; e < kUnionPer; ++e) {
     const int32_t t = ent[e];
-    if (t < 0) continue;
-    uint32_t h = union_slot(t);
-    for (;;) {  // at most kUnionKeys distinct ids in kUnionSlots slots: ends
-      const int32_t old = atomicCAS(&tab[h], -1, t);
-      if (old == -1 || old == t) break;
-      h = (h + 1) & (kUnionSlots - 1);
+    bool fresh = false;
+    if (t >= 0) {
+      uint32_t h = union_slot(t);
+      for (;;) {  // at most kUnionKeys distinct ids in kUnionSlots slots: ends
+        const int32_t old = atomicCAS(&tab[h], -1, t);
+        if (old == -1) fresh = true;
+        if (old == -1 || old == t) break;
+        h = (h + 1) & (kUnionSlots - 1);
+      }
     }
-  }
-  __syncthreads();
-  // the distinct ids, in any order (one LDS atomic per wave and pass)
-  for (int k0 = 0; k0 < kUnionSlots; k0 += kUnionB) {
-    const int32_t t = tab[k0 + tid];
-    const uint64_t m = __ballot(t >= 0);
-    int base = 0;
-    if (lane == 0 && m) base = atomicAdd(&count, __popcll(m));
-    base = __shfl(base, 0, 64);
-    const int at = base + __popcll(m & ((1ull << lane) - 1ull));
-    if (t >= 0 && at < kUnionCap) list[at] = t;
+    const uint64_t m = __ballot(fresh);
+    if (m) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(&count, __popcll(m));
+      base = __shfl(base, 0, 64);
+      const int at = base + __popcll(m & ((1ull << lane) - 1ull));
+      if (fresh && at < kUnionCap) list[at] = t;
+    }
   }
   __syncthreads();
   const int nu = count;
