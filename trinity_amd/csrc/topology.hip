@@ -255,6 +255,10 @@ __global__ void k_count_negative(int64_t n, const int32_t* __restrict__ a, int32
   if ((threadIdx.x & 63) == 0 && m) atomicAdd(count, (int32_t)__popcll(m));
 }
 
+// Neighbours and triangles per neighbour whose loads k_patch_size issues
+// as batches.
+constexpr int kPatchBatch = 8;
+
 // Upper bound of the 2-ring candidates: deg(i) + sum_{j in N(i)} deg(j).
 __global__ void k_patch_bound(int32_t nv, const int32_t* __restrict__ v2t_off,
                               const int32_t* __restrict__ v2v_off,
@@ -288,9 +292,41 @@ __global__ __launch_bounds__(64) void k_patch_size(int32_t nv, const int32_t* __
   const int stride = blockDim.x;
   int m = 0;
   bool ok = true;
-  for (int32_t k = v2t_off[i]; k < v2t_off[i + 1]; ++k)
+  // the loads go out in three independent batches (own triangles and
+  // 1-ring; the neighbours' CSR ranges; their first kPatchBatch triangles)
+  // instead of one dependent chain per neighbour; lists longer than the
+  // batch finish in loops
+  const int32_t o0 = v2t_off[i], o1 = v2t_off[i + 1], n0 = v2v_off[i], n1 = v2v_off[i + 1];
+  int32_t own[kPatchBatch], nbr[kPatchBatch], jb[kPatchBatch], je[kPatchBatch];
+  int32_t tt[kPatchBatch][kPatchBatch];
+#pragma unroll
+  for (int q = 0; q < kPatchBatch; ++q) {
+    own[q] = o0 + q < o1 ? v2t[o0 + q] : -1;
+    nbr[q] = n0 + q < n1 ? v2v[n0 + q] : -1;
+  }
+#pragma unroll
+  for (int q = 0; q < kPatchBatch; ++q) {
+    jb[q] = nbr[q] >= 0 ? v2t_off[nbr[q]] : 0;
+    je[q] = nbr[q] >= 0 ? v2t_off[nbr[q] + 1] : 0;
+  }
+#pragma unroll
+  for (int q = 0; q < kPatchBatch; ++q)
+#pragma unroll
+    for (int k = 0; k < kPatchBatch; ++k) tt[q][k] = jb[q] + k < je[q] ? v2t[jb[q] + k] : -1;
+#pragma unroll
+  for (int q = 0; q < kPatchBatch; ++q)
+    if (own[q] >= 0) ok = ok && lds_sorted_insert(s, stride, m, cap, own[q]);
+  for (int32_t k = o0 + kPatchBatch; k < o1; ++k)
     ok = ok && lds_sorted_insert(s, stride, m, cap, v2t[k]);
-  for (int32_t a = v2v_off[i]; a < v2v_off[i + 1] && ok; ++a) {
+#pragma unroll
+  for (int q = 0; q < kPatchBatch; ++q) {
+#pragma unroll
+    for (int k = 0; k < kPatchBatch; ++k)
+      if (tt[q][k] >= 0) ok = ok && lds_sorted_insert(s, stride, m, cap, tt[q][k]);
+    for (int32_t k = jb[q] + kPatchBatch; k < je[q]; ++k)
+      ok = ok && lds_sorted_insert(s, stride, m, cap, v2t[k]);
+  }
+  for (int32_t a = n0 + kPatchBatch; a < n1 && ok; ++a) {
     const int32_t j = v2v[a];
     for (int32_t k = v2t_off[j]; k < v2t_off[j + 1]; ++k)
       ok = ok && lds_sorted_insert(s, stride, m, cap, v2t[k]);
