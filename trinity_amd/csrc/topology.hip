@@ -172,16 +172,6 @@ __global__ void k_vertex_span(int32_t nv, const int32_t* __restrict__ v2v_off,
   span[v] = s;
 }
 
-__global__ void k_count_v2t(int32_t nt, const int32_t* __restrict__ tri, int32_t* deg) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nt) return;
-  const int32_t a = tri[3 * t];
-  if (a < 0) return;  // deleted triangle (mesh.cpp:200)
-  atomicAdd(&deg[a], 1);
-  atomicAdd(&deg[tri[3 * t + 1]], 1);
-  atomicAdd(&deg[tri[3 * t + 2]], 1);
-}
-
 __global__ void k_fill_v2t(int32_t nt, const int32_t* __restrict__ tri, int32_t* cursor,
                            int32_t* __restrict__ v2t) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -193,11 +183,81 @@ __global__ void k_fill_v2t(int32_t nt, const int32_t* __restrict__ tri, int32_t*
   v2t[atomicAdd(&cursor[tri[3 * t + 2]], 1)] = (int32_t)t;
 }
 
-// Ascending order of each vertex's incident list (mesh.cpp:244).
+// Incident triangles without a separate counting pass: every corner claims
+// the next slot of its vertex's row of kV2tRow entries (cnt[v] ends as the
+// degree; slots past the row are only counted).
+constexpr int kV2tRow = 16;
+
+__global__ void k_fill_v2t_rows(int32_t nt, const int32_t* __restrict__ tri,
+                                int32_t* cnt, int32_t* __restrict__ rows) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const int32_t a = tri[3 * t];
+  if (a < 0) return;  // deleted triangle (mesh.cpp:200)
+  const int32_t c[3] = {a, tri[3 * t + 1], tri[3 * t + 2]};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int32_t slot = atomicAdd(&cnt[c[k]], 1);
+    if (slot < kV2tRow) rows[(int64_t)c[k] * kV2tRow + slot] = (int32_t)t;
+  }
+}
+
+// The CSR from the rows when every degree fits a row: each row sorted
+// ascending in registers (odd-even transposition network over constant
+// indices, padding INT32_MAX; mesh.cpp:244) and written at off[v].
+__global__ void k_v2t_from_rows(int32_t nv, const int32_t* __restrict__ cnt,
+                                const int32_t* __restrict__ off,
+                                const int32_t* __restrict__ rows, int32_t* __restrict__ v2t) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  const int32_t n = cnt[v];
+  const int32_t* row = rows + v * kV2tRow;
+  int32_t r[kV2tRow];
+#pragma unroll
+  for (int k = 0; k < kV2tRow; ++k) r[k] = k < n ? row[k] : INT32_MAX;
+#pragma unroll
+  for (int round = 0; round < kV2tRow; ++round)
+#pragma unroll
+    for (int k = round & 1; k + 1 < kV2tRow; k += 2) {
+      const int32_t lo = r[k] < r[k + 1] ? r[k] : r[k + 1];
+      const int32_t hi = r[k] < r[k + 1] ? r[k + 1] : r[k];
+      r[k] = lo;
+      r[k + 1] = hi;
+    }
+  int32_t* out = v2t + off[v];
+#pragma unroll
+  for (int k = 0; k < kV2tRow; ++k)
+    if (k < n) out[k] = r[k];
+}
+
+// Ascending order of each vertex's incident list (mesh.cpp:244).  Lists
+// of at most kSortRegs entries are sorted in registers (an odd-even
+// transposition network over constant indices, padding INT32_MAX) and
+// written back once; longer ones by insertion in memory.
+constexpr int kSortRegs = 8;
+
 __global__ void k_sort_lists(int32_t nv, const int32_t* __restrict__ off, int32_t* list) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nv) return;
   const int32_t b = off[i], e = off[i + 1];
+  if (e - b <= kSortRegs) {
+    int32_t r[kSortRegs];
+#pragma unroll
+    for (int k = 0; k < kSortRegs; ++k) r[k] = b + k < e ? list[b + k] : INT32_MAX;
+#pragma unroll
+    for (int round = 0; round < kSortRegs; ++round)
+#pragma unroll
+      for (int k = round & 1; k + 1 < kSortRegs; k += 2) {
+        const int32_t lo = r[k] < r[k + 1] ? r[k] : r[k + 1];
+        const int32_t hi = r[k] < r[k + 1] ? r[k + 1] : r[k];
+        r[k] = lo;
+        r[k + 1] = hi;
+      }
+#pragma unroll
+    for (int k = 0; k < kSortRegs; ++k)
+      if (b + k < e) list[b + k] = r[k];
+    return;
+  }
   for (int32_t k = b + 1; k < e; ++k) {
     const int32_t x = list[k];
     int32_t j = k - 1;
@@ -659,20 +719,29 @@ int trn_mesh_create(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
       TRN_REQUIRE(h[0] == 0, TRN_ERR_ARG,
                   std::to_string(h[0]) + " triangles have an index outside [0, nv) or repeat a vertex");
 
-      // 2. v2t CSR
+      // 2. v2t CSR: rows of kV2tRow slots filled and counted in one pass;
+      //    when a degree exceeds the row, the counts seed the two-pass fill
+      //    (positions, then sorting)
       memset_async(ctx, cnt, 0, sizeof(int32_t) * (size_t)nv);
-      launch(ctx, "topo_count_v2t", 24.0 * nt, k_count_v2t, dim3(blocks_for(nt, 256)), dim3(256),
-             0, nt, (const int32_t*)m->tri, cnt);
-      exclusive_scan_i32(ctx, cnt, m->v2t_off, nv, true, scan_tmp);
-      copy_async(ctx, cursor, m->v2t_off, sizeof(int32_t) * (size_t)nv);
-      launch(ctx, "topo_fill_v2t", 36.0 * nt, k_fill_v2t, dim3(blocks_for(nt, 256)), dim3(256), 0,
-             nt, (const int32_t*)m->tri, cursor, m->v2t);
-      launch(ctx, "topo_sort_v2t", 8.0 * nv + 24.0 * nt, k_sort_lists, dim3(blocks_for(nv, 256)),
-             dim3(256), 0, nv, (const int32_t*)m->v2t_off, m->v2t);
+      int32_t* rows = (int32_t*)aux_workspace(ctx, sizeof(int32_t) * kV2tRow * (size_t)nv);
+      launch(ctx, "topo_fill_v2t", 36.0 * nt, k_fill_v2t_rows, dim3(blocks_for(nt, 256)),
+             dim3(256), 0, nt, (const int32_t*)m->tri, cnt, rows);
       reduce_max_i32(ctx, cnt, nv, flags + 1);
       copy_async(ctx, h, flags + 1, sizeof(int32_t));
+      exclusive_scan_i32(ctx, cnt, m->v2t_off, nv, true, scan_tmp);
       sync(ctx);
       m->max_deg_t = h[0];
+      if (m->max_deg_t <= kV2tRow) {
+        launch(ctx, "topo_sort_v2t", 8.0 * nv + 24.0 * nt, k_v2t_from_rows,
+               dim3(blocks_for(nv, 256)), dim3(256), 0, nv, (const int32_t*)cnt,
+               (const int32_t*)m->v2t_off, (const int32_t*)rows, m->v2t);
+      } else {
+        copy_async(ctx, cursor, m->v2t_off, sizeof(int32_t) * (size_t)nv);
+        launch(ctx, "topo_fill_v2t", 36.0 * nt, k_fill_v2t, dim3(blocks_for(nt, 256)),
+               dim3(256), 0, nt, (const int32_t*)m->tri, cursor, m->v2t);
+        launch(ctx, "topo_sort_v2t", 8.0 * nv + 24.0 * nt, k_sort_lists,
+               dim3(blocks_for(nv, 256)), dim3(256), 0, nv, (const int32_t*)m->v2t_off, m->v2t);
+      }
       TRN_REQUIRE(m->max_deg_t <= kMaxDegree, TRN_ERR_CAPACITY,
                   "a vertex has " + std::to_string(m->max_deg_t) +
                       " incident triangles; at most " + std::to_string(kMaxDegree) +
