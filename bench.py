@@ -588,8 +588,11 @@ def run_adjoint(T, mg, ctx, workload, steps, p=2):
 
 
 def compare_smoothing(T, mg, ctx, workload, iters):
-    """Gradient-based smoothing (descent on the energy) against the
-    Laplacian sweeps, from the same mesh, scored by the same energy."""
+    """Gradient-based smoothing (preconditioned descent on the energy)
+    against the Laplacian sweeps, from the same mesh, scored by the same
+    energy.  The sweeps keep the metric of the starting positions while the
+    energy rebuilds it from the moved ones, so their energy is not monotone
+    in the number of sweeps; the descent's is."""
     import numpy as np
     import torch
     m = mg.workload(workload)
@@ -893,6 +896,7 @@ def main():
         extras["adjoint"] = run_adjoint(T, mg, ctx, "jitter1024:tanh", 5)
         extras["adjoint_headline_mesh"] = run_adjoint(T, mg, ctx, args.workload, 2, p=args.p)
         extras["smoothing_modes"] = compare_smoothing(T, mg, ctx, "jitter256:fronts+quad", 5)
+        extras["smoothing_modes_20"] = compare_smoothing(T, mg, ctx, "jitter256:fronts+quad", 20)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -407,10 +407,12 @@ int trn_adjoint_forward(trn_ctx* ctx, trn_adjoint* adjoint, const double* xy_d,
  * either may be NULL.  Synchronises. */
 int trn_adjoint_backward(trn_ctx* ctx, trn_adjoint* adjoint, double* dxy_d,
                          double* df_d);
-/* Gradient-based smoothing: iters steps of steepest descent on E over the
- * vertices Smooth::run may move (src/smoothing.cpp:134-138: not boundary),
- * each with a largest displacement of `step` halved until E decreases and no
- * triangle inverts.  Moves xy_d in place; energies (host, iters + 1, may be
+/* Gradient-based smoothing: iters steps of diagonally preconditioned
+ * steepest descent on E over the vertices Smooth::run may move
+ * (src/smoothing.cpp:134-138: not boundary), the direction at vertex i being
+ * -h_i^2 dE/dx_i with h_i the mean length of its edges, each step with a
+ * largest displacement of `step` halved until E decreases and no triangle
+ * inverts.  Moves xy_d in place; energies (host, iters + 1, may be
  * NULL) receives E before and after each step; accepted (host) the number
  * of steps taken.  Stops early when no step decreases E. */
 int trn_adjoint_smooth(trn_ctx* ctx, trn_adjoint* adjoint, double* xy_d,

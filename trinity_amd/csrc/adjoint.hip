@@ -386,6 +386,33 @@ __global__ __launch_bounds__(256) void k_adj_gather(
 
 // ---- gradient-based smoothing ---------------------------------------------------
 
+// Preconditioned gradient of the gradient-based smoothing, in place: on
+// the vertices that may move, g_i becomes h_i^2 g_i with h_i the mean
+// length of the edges at i (a diagonal preconditioner: the gradient of an
+// element quality scales as 1/h and its curvature as 1/h^2, so a plain
+// gradient step is set by the smallest elements and barely moves the
+// others).
+__global__ __launch_bounds__(256) void k_adj_precond(int32_t nv, const uint8_t* __restrict__ tags,
+                                                     const int32_t* __restrict__ v2t_off,
+                                                     const int32_t* __restrict__ v2v_off,
+                                                     const int32_t* __restrict__ v2v,
+                                                     const double2* __restrict__ xy,
+                                                     double2* __restrict__ g) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const int32_t a0 = v2v_off[i], a1 = v2v_off[i + 1];
+  if ((tags[i] & 4) || v2t_off[i + 1] == v2t_off[i] || a1 == a0) return;
+  const double2 p = xy[i];
+  double s = 0.;
+  for (int32_t a = a0; a < a1; ++a) {
+    const double2 q = xy[v2v[a]];
+    s += __builtin_sqrt((q.x - p.x) * (q.x - p.x) + (q.y - p.y) * (q.y - p.y));
+  }
+  const double h = s / (a1 - a0);
+  const double2 gi = g[i];
+  g[i] = make_double2((h * h) * gi.x, (h * h) * gi.y);
+}
+
 // max |dE/dx| over the vertices that may move (block partials)
 __global__ __launch_bounds__(256) void k_adj_gmax(int32_t nv, const uint8_t* __restrict__ tags,
                                                   const int32_t* __restrict__ v2t_off,
@@ -736,6 +763,9 @@ int trn_adjoint_smooth(trn_ctx* ctx, trn_adjoint* a, double* xy_d, const double*
     const int vb = (int)nblk(m->nv, 256);
     for (int it = 0; it < iters; ++it) {
       adj_backward(ctx, a, a->gxy, nullptr);
+      launch(ctx, "adj_precond", 0.0, k_adj_precond, dim3(vb), dim3(256), 0, m->nv,
+             (const uint8_t*)m->tags, (const int32_t*)m->v2t_off, (const int32_t*)m->v2v_off,
+             (const int32_t*)m->v2v, (const double2*)xy_d, (double2*)a->gxy);
       launch(ctx, "adj_gmax", 0.0, k_adj_gmax, dim3(vb), dim3(256), 0, m->nv,
              (const uint8_t*)m->tags, (const int32_t*)m->v2t_off, (const double2*)a->gxy,
              a->partial);
