@@ -613,33 +613,24 @@ __device__ __forceinline__ int ring_index(const int32_t* __restrict__ ring, int 
   return lo;
 }
 
-__global__ void k_star(int32_t nv, const int32_t* __restrict__ tri,
-                       const int32_t* __restrict__ v2t_off, const int32_t* __restrict__ v2t,
-                       const int32_t* __restrict__ v2v_off, const int32_t* __restrict__ v2v,
-                       int32_t* __restrict__ star) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nv) return;
-  const int32_t* ring = v2v + v2v_off[i];
-  const int n = v2v_off[i + 1] - v2v_off[i];
-  for (int32_t k = v2t_off[i]; k < v2t_off[i + 1]; ++k) {
-    const int32_t t = v2t[k];
-    const int32_t c[3] = {tri[3 * t], tri[3 * t + 1], tri[3 * t + 2]};
-    const int r = c[0] == i ? 0 : (c[1] == i ? 1 : 2);
-    const int a = ring_index(ring, n, c[(r + 1) % 3]);
-    const int b = ring_index(ring, n, c[(r + 2) % 3]);
-    star[k] = r | (a << 2) | (b << 10);
-  }
-}
+// Star codes of the incident triangles (slot r of the vertex in the
+// triangle and the 1-ring indices a, b of its two other corners,
+// r | a << 2 | b << 10, one per v2t entry) and the compact topology (VTopo)
+// of every vertex, in one pass.  A vertex with at most kStarBatch incident
+// triangles and neighbours loads its 1-ring, triangles and their corners in
+// independent batches and finds a corner's 1-ring index as the count of
+// smaller ring entries (the ring is sorted and holds the corner); larger
+// stars search the CSR.
+constexpr int kStarBatch = 8;
 
-// Compact topology (VTopo) of every vertex for the smoothing sweeps, from
-// the CSR arrays and star codes.
-__global__ __launch_bounds__(256) void k_vtopo(int32_t nv, const uint8_t* __restrict__ tags,
-                                               const int32_t* __restrict__ v2t_off,
-                                               const int32_t* __restrict__ v2t,
-                                               const int32_t* __restrict__ star,
-                                               const int32_t* __restrict__ v2v_off,
-                                               const int32_t* __restrict__ v2v,
-                                               VTopo* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_star_vtopo(int32_t nv, const uint8_t* __restrict__ tags,
+                                                    const int32_t* __restrict__ tri,
+                                                    const int32_t* __restrict__ v2t_off,
+                                                    const int32_t* __restrict__ v2t,
+                                                    const int32_t* __restrict__ v2v_off,
+                                                    const int32_t* __restrict__ v2v,
+                                                    int32_t* __restrict__ star,
+                                                    VTopo* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nv) return;
   const int32_t t0 = v2t_off[i], deg = v2t_off[i + 1] - t0;
@@ -647,13 +638,59 @@ __global__ __launch_bounds__(256) void k_vtopo(int32_t nv, const uint8_t* __rest
   VTopo r = {};
   r.tag = tags[i];
   r.meta = (deg <= kVtopoDeg && nb <= kVtopoDeg) ? (uint8_t)(deg | nb << 4) : kVtopoGeneric;
-  const bool fast = r.meta != kVtopoGeneric;
+  if (deg <= kStarBatch && nb <= kStarBatch) {
+    int32_t ring[kStarBatch], tt[kStarBatch], c0[kStarBatch], c1[kStarBatch], c2[kStarBatch];
 #pragma unroll
-  for (int k = 0; k < kVtopoDeg; ++k) {
-    r.ring[k] = (fast && k < nb) ? v2v[n0 + k] : (int32_t)i;
-    r.tris[k] = (fast && k < deg) ? v2t[t0 + k] : 0;
-    const int32_t c = (fast && k < deg) ? star[t0 + k] : 0;
-    r.star[k] = (uint8_t)((c & 3) | (((c >> 2) & 7) << 2) | (((c >> 10) & 7) << 5));
+    for (int q = 0; q < kStarBatch; ++q) {
+      ring[q] = q < nb ? v2v[n0 + q] : INT32_MAX;
+      tt[q] = q < deg ? v2t[t0 + q] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < kStarBatch; ++q) {
+      c0[q] = q < deg ? tri[3 * (int64_t)tt[q]] : 0;
+      c1[q] = q < deg ? tri[3 * (int64_t)tt[q] + 1] : 0;
+      c2[q] = q < deg ? tri[3 * (int64_t)tt[q] + 2] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < kStarBatch; ++q) {
+      if (q >= deg) continue;
+      const int rr = c0[q] == i ? 0 : (c1[q] == i ? 1 : 2);
+      const int32_t x = rr == 0 ? c1[q] : (rr == 1 ? c2[q] : c0[q]);  // corner r + 1
+      const int32_t y = rr == 0 ? c2[q] : (rr == 1 ? c0[q] : c1[q]);  // corner r + 2
+      int a = 0, b = 0;
+#pragma unroll
+      for (int p = 0; p < kStarBatch; ++p) {
+        a += ring[p] < x;
+        b += ring[p] < y;
+      }
+      star[t0 + q] = rr | (a << 2) | (b << 10);
+      if (q < kVtopoDeg) r.star[q] = (uint8_t)(rr | ((a & 7) << 2) | ((b & 7) << 5));
+    }
+    if (r.meta != kVtopoGeneric) {
+#pragma unroll
+      for (int k = 0; k < kVtopoDeg; ++k) {
+        r.ring[k] = k < nb ? ring[k] : (int32_t)i;
+        r.tris[k] = k < deg ? tt[k] : 0;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kVtopoDeg; ++k) {
+        r.ring[k] = (int32_t)i;
+        r.star[k] = 0;
+      }
+    }
+  } else {
+    const int32_t* rg = v2v + n0;
+    for (int32_t k = t0; k < t0 + deg; ++k) {
+      const int32_t t = v2t[k];
+      const int32_t c[3] = {tri[3 * (int64_t)t], tri[3 * (int64_t)t + 1], tri[3 * (int64_t)t + 2]};
+      const int rr = c[0] == i ? 0 : (c[1] == i ? 1 : 2);
+      const int a = ring_index(rg, nb, c[(rr + 1) % 3]);
+      const int b = ring_index(rg, nb, c[(rr + 2) % 3]);
+      star[k] = rr | (a << 2) | (b << 10);
+    }
+#pragma unroll
+    for (int k = 0; k < kVtopoDeg; ++k) r.ring[k] = (int32_t)i;
   }
   out[i] = r;
 }
@@ -863,16 +900,13 @@ int trn_mesh_create(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
       //    through the 1-ring instead of the triangle array)
       TRN_REQUIRE(m->max_deg_v < 256, TRN_ERR_CAPACITY, "1-ring larger than 255 vertices");
       TRN_HIP(hipMalloc(&m->star, sizeof(int32_t) * 3 * (size_t)nt));
-      launch(ctx, "topo_star", 0.0, k_star, dim3(blocks_for(nv, 256)), dim3(256), 0, nv,
-             (const int32_t*)m->tri, (const int32_t*)m->v2t_off, (const int32_t*)m->v2t,
-             (const int32_t*)m->v2v_off, (const int32_t*)m->v2v, m->star);
       TRN_HIP(hipMalloc(&m->vdata, sizeof(VData) * (size_t)nv));
       TRN_HIP(hipMalloc(&m->vt2, sizeof(double) * (size_t)nv));
       TRN_HIP(hipMalloc(&m->vtopo, sizeof(VTopo) * (size_t)nv));
-      launch(ctx, "topo_vtopo", 0.0, k_vtopo, dim3(blocks_for(nv, 256)), dim3(256), 0, nv,
-             (const uint8_t*)m->tags, (const int32_t*)m->v2t_off, (const int32_t*)m->v2t,
-             (const int32_t*)m->star, (const int32_t*)m->v2v_off, (const int32_t*)m->v2v,
-             m->vtopo);
+      launch(ctx, "topo_star", 0.0, k_star_vtopo, dim3(blocks_for(nv, 256)), dim3(256), 0, nv,
+             (const uint8_t*)m->tags, (const int32_t*)m->tri, (const int32_t*)m->v2t_off,
+             (const int32_t*)m->v2t, (const int32_t*)m->v2v_off, (const int32_t*)m->v2v,
+             m->star, m->vtopo);
 
     } catch (...) {
       cleanup();
