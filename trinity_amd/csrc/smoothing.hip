@@ -34,6 +34,9 @@
 namespace trn {
 
 constexpr int kCounterSlots = 64;
+// Longest live list a phase moves cooperatively (kCoopLanes lanes per
+// move); longer ones take a lane per move.
+constexpr int32_t kCoopMaxMoves = 32768;
 
 __global__ __launch_bounds__(256) void k_tensor_log(int32_t nv, const double* __restrict__ tensor,
                                                     double4* __restrict__ logm) {
@@ -113,6 +116,29 @@ __global__ __launch_bounds__(64, MINW) void k_smooth_phase(
   if (count_d) {
     n = *count_d;
     if (units && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(units, (unsigned long long)n);
+    // a short live list: kCoopLanes lanes per move (the phase then lasts
+    // about one move's latency, which the cooperative form shortens)
+    const uint32_t vpb = blockDim.x / kCoopLanes;
+    const uint32_t coop_blocks = (uint32_t)((n + (int32_t)vpb - 1) / (int32_t)vpb);
+    if (n <= kCoopMaxMoves && coop_blocks <= gridDim.x) {
+      if (blockIdx.x >= coop_blocks) return;
+      lb = xcd_block_of(blockIdx.x, coop_blocks);
+      const int sub = (int)(threadIdx.x & (kCoopLanes - 1));
+      const int base = (int)(threadIdx.x & 63) & ~(kCoopLanes - 1);
+      const int64_t idx = (int64_t)lb * vpb + (threadIdx.x / kCoopLanes);
+      int commit = 0, task = 0;
+      if (idx < n)
+        commit = move_vertex_rec_coop(order[idx], sub, base, vd, t2, vtopo, xy_out, t_out,
+                                      qualit, stale, v2t_off, v2t, star, v2v_off, v2v,
+                                      lds + threadIdx.x, blockDim.x, 0, &task);
+      const uint64_t mc = __ballot(commit && sub == 0), mt = __ballot(task && sub == 0);
+      if ((threadIdx.x & 63) == 0) {
+        int32_t* slot = counters + 2 * (blockIdx.x & (kCounterSlots - 1));
+        if (mc) atomicAdd(&slot[0], (int32_t)__popcll(mc));
+        if (mt) atomicAdd(&slot[1], (int32_t)__popcll(mt));
+      }
+      return;
+    }
     const uint32_t live_blocks = (uint32_t)((n + (int32_t)blockDim.x - 1) / (int32_t)blockDim.x);
     if (blockIdx.x >= live_blocks) return;
     lb = xcd_block_of(blockIdx.x, live_blocks);
