@@ -34,9 +34,11 @@
 namespace trn {
 
 constexpr int kCounterSlots = 64;
-// Longest live list a phase moves cooperatively (kCoopLanes lanes per
-// move); longer ones take a lane per move.
-constexpr int32_t kCoopMaxMoves = 32768;
+// Longest live lists a phase moves with 8 and with 2 lanes per move;
+// longer ones take a lane per move (measured on the 4096^2 grid: 4 lanes
+// per move in between, or the 2-lane form up to 2^20 moves, did no better).
+constexpr int32_t kCoop8MaxMoves = 32768;
+constexpr int32_t kCoop2MaxMoves = 262144;
 
 __global__ __launch_bounds__(256) void k_tensor_log(int32_t nv, const double* __restrict__ tensor,
                                                     double4* __restrict__ logm) {
@@ -94,6 +96,38 @@ __global__ __launch_bounds__(kPackB) void k_pack_records(int32_t nv, const doubl
   for (int c = tid; c < 2 * n; c += kPackB) lgc[c] = lg[3 * (c >> 1) + (c & 1)];
 }
 
+// The phase over a live list of n vertices with G lanes per move, when the
+// grid holds the blocks that takes (true: the block has done its part).
+template <int G>
+__device__ __forceinline__ bool coop_phase(
+    const int32_t n, const int32_t* __restrict__ order, VData* __restrict__ vd,
+    double* __restrict__ t2, const VTopo* __restrict__ vtopo, double2* __restrict__ xy_out,
+    double* __restrict__ t_out, double* __restrict__ qualit, uint8_t* __restrict__ stale,
+    const int32_t* __restrict__ v2t_off, const int32_t* __restrict__ v2t,
+    const int32_t* __restrict__ star, const int32_t* __restrict__ v2v_off,
+    const int32_t* __restrict__ v2v, int32_t* __restrict__ counters, double* lds) {
+  const uint32_t vpb = blockDim.x / G;
+  const uint32_t coop_blocks = (uint32_t)((n + (int32_t)vpb - 1) / (int32_t)vpb);
+  if (coop_blocks > gridDim.x) return false;
+  if (blockIdx.x >= coop_blocks) return true;
+  const uint32_t lb = xcd_block_of(blockIdx.x, coop_blocks);
+  const int sub = (int)(threadIdx.x & (G - 1));
+  const int base = (int)(threadIdx.x & 63) & ~(G - 1);
+  const int64_t idx = (int64_t)lb * vpb + (threadIdx.x / G);
+  int commit = 0, task = 0;
+  if (idx < n)
+    commit = move_vertex_rec_coop<G>(order[idx], sub, base, vd, t2, vtopo, xy_out, t_out, qualit,
+                                     stale, v2t_off, v2t, star, v2v_off, v2v, lds + threadIdx.x,
+                                     blockDim.x, 0, &task);
+  const uint64_t mc = __ballot(commit && sub == 0), mt = __ballot(task && sub == 0);
+  if ((threadIdx.x & 63) == 0) {
+    int32_t* slot = counters + 2 * (blockIdx.x & (kCounterSlots - 1));
+    if (mc) atomicAdd(&slot[0], (int32_t)__popcll(mc));
+    if (mt) atomicAdd(&slot[1], (int32_t)__popcll(mt));
+  }
+  return true;
+}
+
 // One colour phase.  count_d == nullptr: `order` lists `count` vertices
 // (the whole class); otherwise it is a live list of *count_d <= count
 // vertices written by k_class_live, the grid is sized for `count`, and the
@@ -116,29 +150,16 @@ __global__ __launch_bounds__(64, MINW) void k_smooth_phase(
   if (count_d) {
     n = *count_d;
     if (units && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(units, (unsigned long long)n);
-    // a short live list: kCoopLanes lanes per move (the phase then lasts
+    // a short live list: several lanes per move (the phase then lasts
     // about one move's latency, which the cooperative form shortens)
-    const uint32_t vpb = blockDim.x / kCoopLanes;
-    const uint32_t coop_blocks = (uint32_t)((n + (int32_t)vpb - 1) / (int32_t)vpb);
-    if (n <= kCoopMaxMoves && coop_blocks <= gridDim.x) {
-      if (blockIdx.x >= coop_blocks) return;
-      lb = xcd_block_of(blockIdx.x, coop_blocks);
-      const int sub = (int)(threadIdx.x & (kCoopLanes - 1));
-      const int base = (int)(threadIdx.x & 63) & ~(kCoopLanes - 1);
-      const int64_t idx = (int64_t)lb * vpb + (threadIdx.x / kCoopLanes);
-      int commit = 0, task = 0;
-      if (idx < n)
-        commit = move_vertex_rec_coop(order[idx], sub, base, vd, t2, vtopo, xy_out, t_out,
-                                      qualit, stale, v2t_off, v2t, star, v2v_off, v2v,
-                                      lds + threadIdx.x, blockDim.x, 0, &task);
-      const uint64_t mc = __ballot(commit && sub == 0), mt = __ballot(task && sub == 0);
-      if ((threadIdx.x & 63) == 0) {
-        int32_t* slot = counters + 2 * (blockIdx.x & (kCounterSlots - 1));
-        if (mc) atomicAdd(&slot[0], (int32_t)__popcll(mc));
-        if (mt) atomicAdd(&slot[1], (int32_t)__popcll(mt));
-      }
+    if (n <= kCoop8MaxMoves && coop_phase<8>(n, order, vd, t2, vtopo, xy_out, t_out, qualit,
+                                              stale, v2t_off, v2t, star, v2v_off, v2v, counters,
+                                              lds))
       return;
-    }
+    if (n <= kCoop2MaxMoves && coop_phase<2>(n, order, vd, t2, vtopo, xy_out, t_out, qualit,
+                                              stale, v2t_off, v2t, star, v2v_off, v2v, counters,
+                                              lds))
+      return;
     const uint32_t live_blocks = (uint32_t)((n + (int32_t)blockDim.x - 1) / (int32_t)blockDim.x);
     if (blockIdx.x >= live_blocks) return;
     lb = xcd_block_of(blockIdx.x, live_blocks);

@@ -362,57 +362,70 @@ static __device__ __noinline__ int move_exact_rec(const int32_t i, const double2
 
 // Cooperative form of the fast move for phases with few moves (the live
 // lists of the later sweeps, where a phase lasts about one move's latency):
-// the kCoopLanes lanes of an aligned group share vertex i, lane s taking
-// 1-ring entry s and incident triangle s, so the per-neighbour and
-// per-triangle work of the setup and of every try runs side by side.
-// Arithmetic and its order are those of move_vertex_fast: sums over the
-// 1-ring are formed in ring order from the lanes' terms (every lane adds
-// the same sequence), q_min is the minimum of the cached qualities (the
-// reference's fold skips NaN, so NaN enters the group minimum as DBL_MAX),
-// and a try's outcome is that of the sequential early-exit loop: valid when
-// every triangle passes; otherwise decided by the first triangle that
-// fails, undecided (2) when that one came within 16 ulps of q_min.  The
-// caller guarantees deg, nb <= kVtopoDeg.  Every lane of the group returns
-// the same value; lane 0 commits the vertex data, each lane its triangle's
-// quality.
-constexpr int kCoopLanes = 8;
+// the G lanes of an aligned group share vertex i, lane s taking the 1-ring
+// entries and incident triangles k = s + G j (j < kCoopSlots(G)), so the
+// per-neighbour and per-triangle work of the setup and of every try runs
+// side by side.  Arithmetic and its order are those of move_vertex_fast:
+// sums over the 1-ring are formed in ring order from the lanes' terms
+// (every lane adds the same sequence), q_min is the minimum of the cached
+// qualities (the reference's fold skips NaN, so NaN enters the group
+// minimum as DBL_MAX), and a try's outcome is that of the sequential
+// early-exit loop: valid when every triangle passes; otherwise decided by
+// the first triangle that fails, undecided (2) when that one came within
+// 16 ulps of q_min.  The caller guarantees deg, nb <= kVtopoDeg.  Every
+// lane of the group returns the same value; lane 0 commits the vertex data,
+// each lane its triangles' qualities.
+template <int G>
+constexpr int kCoopSlots = (kVtopoDeg + G - 1) / G;
 
-template <class V>
+template <int G, class V>
 __device__ __forceinline__ int move_vertex_coop(const int32_t i, const int s, const int base,
-                                                const int deg, const int nb, const int32_t ring_s,
-                                                const int32_t tri_s, const int star_s,
+                                                const int deg, const int nb,
+                                                const int32_t (&ring_s)[kCoopSlots<G>],
+                                                const int32_t (&tri_s)[kCoopSlots<G>],
+                                                const int (&star_s)[kCoopSlots<G>],
                                                 const double2 pini, const double T[3],
                                                 const V& v, double* __restrict__ qualit) {
+  constexpr int S = kCoopSlots<G>;
   const double P[2] = {pini.x, pini.y};
   // q_min (smoothing.cpp:48-51)
   double qm = DBL_MAX;
-  if (s < deg) {
-    const double q = qualit[tri_s];
-    qm = q == q ? q : DBL_MAX;
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    if (s + G * j < deg) {
+      const double q = qualit[tri_s[j]];
+      qm = dmin(qm, q == q ? q : DBL_MAX);
+    }
   }
 #pragma unroll
-  for (int d = 1; d < kCoopLanes; d <<= 1) qm = dmin(qm, __shfl_xor(qm, d, 64));
+  for (int d = 1; d < G; d <<= 1) qm = dmin(qm, __shfl_xor(qm, d, 64));
   const double q_min = qm;
   // 1-ring terms (smoothing.cpp:61-79), summed in ring order
-  double ox = 0, oy = 0;
-  double4 L = make_double4(0., 0., 0., 0.);
-  if (s < nb) {
-    const double2 pb = v.pos(ring_s);
-    const double Pb[2] = {pb.x, pb.y};
-    double Tb[3];
-    v.metric(ring_s, Tb);
-    L = v.log(ring_s);
-    const double len = riemann_dist(P, Pb, T, Tb);
-    ox = P[0] + ((Pb[0] - P[0]) / len);
-    oy = P[1] + ((Pb[1] - P[1]) / len);
+  double ox[S], oy[S];
+  double4 L[S];
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    ox[j] = oy[j] = 0.;
+    L[j] = make_double4(0., 0., 0., 0.);
+    if (s + G * j < nb) {
+      const double2 pb = v.pos(ring_s[j]);
+      const double Pb[2] = {pb.x, pb.y};
+      double Tb[3];
+      v.metric(ring_s[j], Tb);
+      L[j] = v.log(ring_s[j]);
+      const double len = riemann_dist(P, Pb, T, Tb);
+      ox[j] = P[0] + ((Pb[0] - P[0]) / len);
+      oy[j] = P[1] + ((Pb[1] - P[1]) / len);
+    }
   }
   double opt0 = 0, opt1 = 0;
   double lm[4] = {0., 0., 0., 0.};
 #pragma unroll
   for (int k = 0; k < kVtopoDeg; ++k) {
-    const double a0 = __shfl(ox, base + k, 64), a1 = __shfl(oy, base + k, 64);
-    const double l0 = __shfl(L.x, base + k, 64), l1 = __shfl(L.y, base + k, 64);
-    const double l2 = __shfl(L.z, base + k, 64), l3 = __shfl(L.w, base + k, 64);
+    const int src = base + k % G, j = k / G;
+    const double a0 = __shfl(ox[j], src, 64), a1 = __shfl(oy[j], src, 64);
+    const double l0 = __shfl(L[j].x, src, 64), l1 = __shfl(L[j].y, src, 64);
+    const double l2 = __shfl(L[j].z, src, 64), l3 = __shfl(L[j].w, src, 64);
     if (k < nb) {
       opt0 += a0;
       opt1 += a1;
@@ -427,52 +440,83 @@ __device__ __forceinline__ int move_vertex_coop(const int32_t i, const int s, co
   double TN[3];
   tensor_exp_mean(lm, nb, TN);  // smoothing.cpp:84
   const double4 LN = tensor_log4(TN[0], TN[1], TN[2]);
-  // incident triangle s: corners, averaged tensor, sqrt(det), opposite edge
-  const int r = star_s & 3;
-  const int32_t ja = __shfl(ring_s, base + ((star_s >> 2) & 7), 64);
-  const int32_t jb = __shfl(ring_s, base + ((star_s >> 5) & 7), 64);
-  double2 pa = pini, pb = pini;
-  double J[3] = {0., 0., 0.}, sd = 0., el = 0.;
-  if (s < deg) {
-    pa = v.pos(ja);
-    pb = v.pos(jb);
-    const double4 La = v.log(ja), Lb = v.log(jb);
-    const double4 L0 = r == 0 ? LN : (r == 1 ? Lb : La);
-    const double4 L1 = r == 0 ? La : (r == 1 ? LN : Lb);
-    const double4 L2 = r == 0 ? Lb : (r == 1 ? La : LN);
-    mean_from_logs(L0, L1, L2, J);
-    sd = mean_sqrt_det(J);
-    el = edge_len(J, pb.x - pa.x, pb.y - pa.y);
+  // the lane's incident triangles: corners, averaged tensor, sqrt(det),
+  // opposite edge
+  double2 pa[S], pb[S];
+  double Jm[S][3], sd[S], el[S], qs[S];
+  int rot[S];
+#pragma unroll
+  for (int j = 0; j < S; ++j) {
+    rot[j] = star_s[j] & 3;
+    const int ia = (star_s[j] >> 2) & 7, ib = (star_s[j] >> 5) & 7;
+    // ring entry q lives in slot q / G of lane q % G
+    int32_t ja = i, jb = i;
+#pragma unroll
+    for (int jj = 0; jj < S; ++jj) {
+      const int32_t ca = __shfl(ring_s[jj], base + ia % G, 64);
+      const int32_t cb = __shfl(ring_s[jj], base + ib % G, 64);
+      if (ia / G == jj) ja = ca;
+      if (ib / G == jj) jb = cb;
+    }
+    pa[j] = pb[j] = pini;
+    Jm[j][0] = Jm[j][1] = Jm[j][2] = 0.;
+    sd[j] = el[j] = qs[j] = 0.;
+    if (s + G * j < deg) {
+      pa[j] = v.pos(ja);
+      pb[j] = v.pos(jb);
+      const double4 La = v.log(ja), Lb = v.log(jb);
+      const int r = rot[j];
+      const double4 L0 = r == 0 ? LN : (r == 1 ? Lb : La);
+      const double4 L1 = r == 0 ? La : (r == 1 ? LN : Lb);
+      const double4 L2 = r == 0 ? Lb : (r == 1 ? La : LN);
+      mean_from_logs(L0, L1, L2, Jm[j]);
+      sd[j] = mean_sqrt_det(Jm[j]);
+      el[j] = edge_len(Jm[j], pb[j].x - pa[j].x, pb[j].y - pa[j].y);
+    }
   }
   int valid = 0;
   bool undecided = false;
   double w = 1;
-  double px = P[0], py = P[1], qs = 0.;
+  double px = P[0], py = P[1];
   for (int retry = 0; retry < 5; ++retry) {  // smoothing.cpp:96-114
     px = (1 - w) * P[0] + w * opt0;
     py = (1 - w) * P[1] + w * opt1;
-    bool ok = true, near = false;
-    if (s < deg) {
-      const double pp[3][2] = {
-          {r == 0 ? px : (r == 1 ? pb.x : pa.x), r == 0 ? py : (r == 1 ? pb.y : pa.y)},
-          {r == 0 ? pa.x : (r == 1 ? px : pb.x), r == 0 ? pa.y : (r == 1 ? py : pb.y)},
-          {r == 0 ? pb.x : (r == 1 ? pa.x : px), r == 0 ? pb.y : (r == 1 ? pa.y : py)}};
-      ok = false;
-      if (ccw(pp[0], pp[1], pp[2])) {
-        qs = quality_with_corner<false>(pp, r, px, py, pa, pb, J, sd, el, q_min, &near);
-        ok = !near && qs > q_min;
+    uint32_t badm = 0, nearm = 0;  // bit k: triangle k fails / is near q_min
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      const int k = s + G * j;
+      if (k < deg) {
+        const int r = rot[j];
+        const double2 a = pa[j], b = pb[j];
+        const double pp[3][2] = {
+            {r == 0 ? px : (r == 1 ? b.x : a.x), r == 0 ? py : (r == 1 ? b.y : a.y)},
+            {r == 0 ? a.x : (r == 1 ? px : b.x), r == 0 ? a.y : (r == 1 ? py : b.y)},
+            {r == 0 ? b.x : (r == 1 ? a.x : px), r == 0 ? b.y : (r == 1 ? a.y : py)}};
+        bool ok = false, near = false;
+        if (ccw(pp[0], pp[1], pp[2])) {
+          qs[j] = quality_with_corner<false>(pp, r, px, py, a, b, Jm[j], sd[j], el[j], q_min,
+                                             &near);
+          ok = !near && qs[j] > q_min;
+        }
+        badm |= (uint32_t)!ok << k;
+        nearm |= (uint32_t)near << k;
       }
     }
-    const uint32_t bad = (uint32_t)(__ballot(!ok) >> base) & ((1u << kCoopLanes) - 1);
-    const uint32_t nearm = (uint32_t)(__ballot(near) >> base) & ((1u << kCoopLanes) - 1);
-    valid = bad == 0;
-    undecided = !valid && ((nearm >> __builtin_ctz(bad)) & 1);
+#pragma unroll
+    for (int d = 1; d < G; d <<= 1) {
+      badm |= (uint32_t)__shfl_xor((int)badm, d, 64);
+      nearm |= (uint32_t)__shfl_xor((int)nearm, d, 64);
+    }
+    valid = badm == 0;
+    undecided = !valid && ((nearm >> __builtin_ctz(badm)) & 1);
     w *= 0.5;
     if (valid || undecided) break;
   }
   if (undecided) return 2;
   if (valid) {  // smoothing.cpp:116-123
-    if (s < deg) qualit[tri_s] = qs;
+#pragma unroll
+    for (int j = 0; j < S; ++j)
+      if (s + G * j < deg) qualit[tri_s[j]] = qs[j];
     if (s == 0) v.commit(i, px, py, TN, LN);
   }
   return valid;
@@ -548,11 +592,12 @@ __device__ __forceinline__ int move_vertex_rec(const int32_t i, VData* __restric
   return c;
 }
 
-// move_vertex_rec for the kCoopLanes lanes of a group (lane s, group
-// starting at lane `base`), every lane holding vertex i: fast-path vertices
-// move cooperatively (move_vertex_coop); a move it leaves undecided, and
-// every vertex of a larger star, is done by lane 0 alone as
-// move_vertex_rec does it.  Every lane returns the same value and task.
+// move_vertex_rec for the G lanes of a group (lane s, group starting at
+// lane `base`), every lane holding vertex i: fast-path vertices move
+// cooperatively (move_vertex_coop); a move it leaves undecided, and every
+// vertex of a larger star, is done by lane 0 alone as move_vertex_rec does
+// it.  Every lane returns the same value and task.
+template <int G>
 __device__ __forceinline__ int move_vertex_rec_coop(
     const int32_t i, const int s, const int base, VData* __restrict__ vd,
     double* __restrict__ t2, const VTopo* __restrict__ vtopo, double2* __restrict__ xo,
@@ -561,6 +606,7 @@ __device__ __forceinline__ int move_vertex_rec_coop(
     const int32_t* __restrict__ star, const int32_t* __restrict__ v2v_off,
     const int32_t* __restrict__ v2v, double* lds, const int stride, const int first,
     int* task) {
+  constexpr int S = kCoopSlots<G>;
   const RecVerts v{vd, t2, xo, to};
   const VTopo* vt = vtopo + i;
   const uint8_t tag = vt->tag, meta = vt->meta;
@@ -575,12 +621,18 @@ __device__ __forceinline__ int move_vertex_rec_coop(
     if (first && s == 0) stale[i] = 0;
     if (deg == 0) return 0;
     *task = 1;
-    const int32_t ring_s = s < nb ? vt->ring[s] : i;
-    const int32_t tri_s = s < deg ? vt->tris[s] : 0;
-    const int star_s = s < deg ? vt->star[s] : 0;
+    int32_t ring_s[S], tri_s[S];
+    int star_s[S];
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      const int k = s + G * j;
+      ring_s[j] = k < nb ? vt->ring[k] : i;
+      tri_s[j] = k < deg ? vt->tris[k] : 0;
+      star_s[j] = k < deg ? vt->star[k] : 0;
+    }
     const VData own = vd[i];
     const double T[3] = {own.t01[0], own.t01[1], t2[i]};
-    c = move_vertex_coop(i, s, base, deg, nb, ring_s, tri_s, star_s, own.xy, T, v, qualit);
+    c = move_vertex_coop<G>(i, s, base, deg, nb, ring_s, tri_s, star_s, own.xy, T, v, qualit);
     if (c == 2) {
       if (s == 0)
         c = move_exact_rec(i, own.xy, vd, t2, xo, to, qualit, v2t_off, v2t, star, v2v_off, v2v,
@@ -588,7 +640,9 @@ __device__ __forceinline__ int move_vertex_rec_coop(
       c = __shfl(c, base, 64);
     }
     if (c) {
-      if (s < nb) stale[ring_s] = 0;
+#pragma unroll
+      for (int j = 0; j < S; ++j)
+        if (s + G * j < nb) stale[ring_s[j]] = 0;
     } else if (s == 0) {
       stale[i] = 1;
     }
