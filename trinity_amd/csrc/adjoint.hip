@@ -35,6 +35,11 @@ constexpr int kQB = 15;  // [xy adjoint 2, M adjoint 3] x 3 corners
 constexpr int kGB = 6;   // vertex-gradient adjoint 2 x 3 corners
 constexpr int kAD = 7;   // area adjoint, basis-gradient adjoints 6
 constexpr int kFB = 3;   // nodal-value adjoint x 3 corners
+// Entries of a vertex's incident list and of a triangle's ring union whose
+// loads the gathers issue as one batch before summing (in the same order);
+// longer lists finish in a loop.
+constexpr int kAdjBatchV = 8;
+constexpr int kAdjBatchT = 16;
 
 static unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
@@ -125,7 +130,27 @@ __global__ __launch_bounds__(256) void k_adj_global(
   double fb = 0.;
   if (i < nv) {
     double a0 = 0., a1 = 0., a2 = 0.;
-    for (int32_t k = v2t_off[i]; k < v2t_off[i + 1]; ++k) {
+    const int32_t k0 = v2t_off[i], k1 = v2t_off[i + 1];
+    const double* sp[kAdjBatchV];
+#pragma unroll
+    for (int q = 0; q < kAdjBatchV; ++q)
+      sp[q] = k0 + q < k1 ? QB + kQB * (int64_t)v2t[k0 + q] + 5 * (star[k0 + q] & 3) + 2 : QB;
+    double b[kAdjBatchV][3];
+#pragma unroll
+    for (int q = 0; q < kAdjBatchV; ++q) {
+      b[q][0] = sp[q][0];
+      b[q][1] = sp[q][1];
+      b[q][2] = sp[q][2];
+    }
+#pragma unroll
+    for (int q = 0; q < kAdjBatchV; ++q) {
+      if (k0 + q < k1) {
+        a0 += b[q][0];
+        a1 += b[q][1];
+        a2 += b[q][2];
+      }
+    }
+    for (int32_t k = k0 + kAdjBatchV; k < k1; ++k) {
       const double* s = QB + kQB * (int64_t)v2t[k] + 5 * (star[k] & 3) + 2;
       a0 += s[0];
       a1 += s[1];
@@ -162,7 +187,16 @@ __global__ __launch_bounds__(256) void k_adj_local(
   if (i >= nv) return;
   const double cb = scal[2];
   double ring = 0.;
-  for (int32_t k = v2t_off[i]; k < v2t_off[i + 1]; ++k) ring += area_t[v2t[k]];
+  const int32_t k0 = v2t_off[i], k1 = v2t_off[i + 1];
+  {
+    double av[kAdjBatchV];
+#pragma unroll
+    for (int q = 0; q < kAdjBatchV; ++q) av[q] = k0 + q < k1 ? area_t[v2t[k0 + q]] : 0.;
+#pragma unroll
+    for (int q = 0; q < kAdjBatchV; ++q)
+      if (k0 + q < k1) ring += av[q];
+  }
+  for (int32_t k = k0 + kAdjBatchV; k < k1; ++k) ring += area_t[v2t[k]];
   const double* m = Mloc + 3 * i;
   const double sd = __builtin_sqrt(m[0] * m[2] - m[1] * m[1]);
   const double w = cb * ring / 3 / (2 * sd);  // metric.cpp:171-179
@@ -253,7 +287,28 @@ __global__ __launch_bounds__(kAdjB) void k_adj_hess_tri(
   const int32_t* s = tr_ell + (int64_t)tr_row[blockIdx.x] * kAdjB + threadIdx.x;
   const int m = tr_n[t];
   double s0 = 0., s12 = 0., s3 = 0.;
-  for (int e = 0; e < m; ++e) {
+  {
+    int32_t iv[kAdjBatchT];
+#pragma unroll
+    for (int e = 0; e < kAdjBatchT; ++e) iv[e] = e < m ? s[e * kAdjB] : 0;
+    double bv[kAdjBatchT][3];
+#pragma unroll
+    for (int e = 0; e < kAdjBatchT; ++e) {
+      const double* b = Sb + 3 * (int64_t)iv[e];
+      bv[e][0] = e < m ? b[0] : 0.;
+      bv[e][1] = e < m ? b[1] : 0.;
+      bv[e][2] = e < m ? b[2] : 0.;
+    }
+#pragma unroll
+    for (int e = 0; e < kAdjBatchT; ++e) {
+      if (e < m) {
+        s0 += bv[e][0];
+        s12 += bv[e][1];
+        s3 += bv[e][2];
+      }
+    }
+  }
+  for (int e = kAdjBatchT; e < m; ++e) {
     const double* b = Sb + 3 * (int64_t)s[e * kAdjB];
     s0 += b[0];
     s12 += b[1];
@@ -293,7 +348,26 @@ __global__ __launch_bounds__(256) void k_adj_grad_vtx(
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nv) return;
   double gx = 0., gy = 0.;
-  for (int32_t k = v2t_off[i]; k < v2t_off[i + 1]; ++k) {
+  const int32_t k0 = v2t_off[i], k1 = v2t_off[i + 1];
+  {
+    double2 gv[kAdjBatchV];
+#pragma unroll
+    for (int q = 0; q < kAdjBatchV; ++q) {
+      gv[q] = make_double2(0., 0.);
+      if (k0 + q < k1) {
+        const double* s = GB + kGB * (int64_t)v2t[k0 + q] + 2 * (star[k0 + q] & 3);
+        gv[q] = make_double2(s[0], s[1]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kAdjBatchV; ++q) {
+      if (k0 + q < k1) {
+        gx += gv[q].x;
+        gy += gv[q].y;
+      }
+    }
+  }
+  for (int32_t k = k0 + kAdjBatchV; k < k1; ++k) {
     const double* s = GB + kGB * (int64_t)v2t[k] + 2 * (star[k] & 3);
     gx += s[0];
     gy += s[1];
@@ -320,7 +394,27 @@ __global__ __launch_bounds__(kAdjB) void k_adj_grad_tri(
   const int32_t* s = tr_ell + (int64_t)tr_row[blockIdx.x] * kAdjB + threadIdx.x;
   const int m = tr_n[t];
   double sx = 0., sy = 0., tb = 0.;
-  for (int e = 0; e < m; ++e) {
+  {
+    int32_t iv[kAdjBatchT];
+#pragma unroll
+    for (int e = 0; e < kAdjBatchT; ++e) iv[e] = e < m ? s[e * kAdjB] : 0;
+    double2 vv[kAdjBatchT];
+    double tv[kAdjBatchT];
+#pragma unroll
+    for (int e = 0; e < kAdjBatchT; ++e) {
+      vv[e] = e < m ? Sgb[iv[e]] : make_double2(0., 0.);
+      tv[e] = e < m ? TAb[iv[e]] : 0.;
+    }
+#pragma unroll
+    for (int e = 0; e < kAdjBatchT; ++e) {
+      if (e < m) {
+        sx += vv[e].x;
+        sy += vv[e].y;
+        tb += tv[e];
+      }
+    }
+  }
+  for (int e = kAdjBatchT; e < m; ++e) {
     const int i = s[e * kAdjB];
     const double2 v = Sgb[i];
     sx += v.x;
@@ -373,7 +467,30 @@ __global__ __launch_bounds__(256) void k_adj_gather(
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nv) return;
   double gx = 0., gy = 0., gf = 0.;
-  for (int32_t k = v2t_off[i]; k < v2t_off[i + 1]; ++k) {
+  const int32_t k0 = v2t_off[i], k1 = v2t_off[i + 1];
+  {
+    double vx[kAdjBatchV], vy[kAdjBatchV], vf[kAdjBatchV];
+#pragma unroll
+    for (int q = 0; q < kAdjBatchV; ++q) {
+      vx[q] = vy[q] = vf[q] = 0.;
+      if (k0 + q < k1) {
+        const int64_t t = v2t[k0 + q];
+        const int r = star[k0 + q] & 3;
+        vx[q] = QB[kQB * t + 5 * r];
+        vy[q] = QB[kQB * t + 5 * r + 1];
+        vf[q] = FB[kFB * t + r];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kAdjBatchV; ++q) {
+      if (k0 + q < k1) {
+        gx += vx[q];
+        gy += vy[q];
+        gf += vf[q];
+      }
+    }
+  }
+  for (int32_t k = k0 + kAdjBatchV; k < k1; ++k) {
     const int64_t t = v2t[k];
     const int r = star[k] & 3;
     gx += QB[kQB * t + 5 * r];
