@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void k_adj_quality(int32_t nt, const int32_t* 
                                                      const double* __restrict__ M,
                                                      const double* __restrict__ qualit,
                                                      double* __restrict__ QB) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= nt || tri[3 * t] < 0) return;
   const int32_t n0 = tri[3 * t], n1 = tri[3 * t + 1], n2 = tri[3 * t + 2];
   double g[6] = {0., 0., 0., 0., 0., 0.}, gm[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
@@ -126,7 +126,8 @@ __global__ __launch_bounds__(256) void k_adj_global(
     const double* __restrict__ Mloc, const double* __restrict__ scal, double emin, double emax,
     double* __restrict__ Mlb, double* __restrict__ partial) {
   __shared__ double red[4];
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t lb = xcd_block();
+  const int64_t i = (int64_t)lb * blockDim.x + threadIdx.x;
   double fb = 0.;
   if (i < nv) {
     double a0 = 0., a1 = 0., a2 = 0.;
@@ -171,7 +172,7 @@ __global__ __launch_bounds__(256) void k_adj_global(
     fb = (e.branch == 2) ? (b00 + b11) * dfa[1] : b00 * dfa[0] + b11 * dfa[1];
   }
   const double b = block_sum(fb, red);
-  if (threadIdx.x == 0) partial[blockIdx.x] = b;
+  if (threadIdx.x == 0) partial[lb] = b;
 }
 
 // B4: complexity (sqrt det term of the vertex) and local normalisation;
@@ -183,7 +184,7 @@ __global__ __launch_bounds__(256) void k_adj_local(
     const double* __restrict__ Hs, const double* __restrict__ TA,
     const double* __restrict__ scal, double ex, const double* __restrict__ Mlb,
     double* __restrict__ Sb, double* __restrict__ TAb) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (i >= nv) return;
   const double cb = scal[2];
   double ring = 0.;
@@ -281,10 +282,11 @@ __global__ __launch_bounds__(kAdjB) void k_adj_hess_tri(
     const double2* __restrict__ G, const double* __restrict__ Sb,
     const int32_t* __restrict__ tr_n, const int32_t* __restrict__ tr_row,
     const int32_t* __restrict__ tr_ell, double* __restrict__ AD, double* __restrict__ GB) {
-  const int64_t t = (int64_t)blockIdx.x * kAdjB + threadIdx.x;
+  const uint32_t lb = xcd_block();
+  const int64_t t = (int64_t)lb * kAdjB + threadIdx.x;
   if (t >= nt || tri[3 * t] < 0) return;
   const int32_t n[3] = {tri[3 * t], tri[3 * t + 1], tri[3 * t + 2]};
-  const int32_t* s = tr_ell + (int64_t)tr_row[blockIdx.x] * kAdjB + threadIdx.x;
+  const int32_t* s = tr_ell + (int64_t)tr_row[lb] * kAdjB + threadIdx.x;
   const int m = tr_n[t];
   double s0 = 0., s12 = 0., s3 = 0.;
   {
@@ -345,7 +347,7 @@ __global__ __launch_bounds__(256) void k_adj_grad_vtx(
     const int32_t* __restrict__ star, const double* __restrict__ GB,
     const double2* __restrict__ G, const double* __restrict__ TA, double* __restrict__ TAb,
     double2* __restrict__ Sgb) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (i >= nv) return;
   double gx = 0., gy = 0.;
   const int32_t k0 = v2t_off[i], k1 = v2t_off[i + 1];
@@ -388,10 +390,11 @@ __global__ __launch_bounds__(kAdjB) void k_adj_grad_tri(
     const int32_t* __restrict__ tr_n, const int32_t* __restrict__ tr_row,
     const int32_t* __restrict__ tr_ell, const double* __restrict__ scal,
     const double* __restrict__ AD, double* __restrict__ QB, double* __restrict__ FB) {
-  const int64_t t = (int64_t)blockIdx.x * kAdjB + threadIdx.x;
+  const uint32_t lb = xcd_block();
+  const int64_t t = (int64_t)lb * kAdjB + threadIdx.x;
   if (t >= nt || tri[3 * t] < 0) return;
   const int32_t n[3] = {tri[3 * t], tri[3 * t + 1], tri[3 * t + 2]};
-  const int32_t* s = tr_ell + (int64_t)tr_row[blockIdx.x] * kAdjB + threadIdx.x;
+  const int32_t* s = tr_ell + (int64_t)tr_row[lb] * kAdjB + threadIdx.x;
   const int m = tr_n[t];
   double sx = 0., sy = 0., tb = 0.;
   {
@@ -464,7 +467,7 @@ __global__ __launch_bounds__(256) void k_adj_gather(
     int32_t nv, const int32_t* __restrict__ v2t_off, const int32_t* __restrict__ v2t,
     const int32_t* __restrict__ star, const double* __restrict__ QB,
     const double* __restrict__ FB, double2* __restrict__ dxy, double* __restrict__ df) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (i >= nv) return;
   double gx = 0., gy = 0., gf = 0.;
   const int32_t k0 = v2t_off[i], k1 = v2t_off[i + 1];
