@@ -37,9 +37,11 @@ constexpr int kAD = 7;   // area adjoint, basis-gradient adjoints 6
 constexpr int kFB = 3;   // nodal-value adjoint x 3 corners
 // Entries of a vertex's incident list and of a triangle's ring union whose
 // loads the gathers issue as one batch before summing (in the same order);
-// longer lists finish in a loop.
+// longer lists finish in a loop.  The ring-union batch is set per kernel
+// by measurement on the 4096^2 grid.
 constexpr int kAdjBatchV = 8;
-constexpr int kAdjBatchT = 16;
+constexpr int kAdjBatchHess = 8;
+constexpr int kAdjBatchGrad = 16;
 
 static unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
@@ -290,19 +292,19 @@ __global__ __launch_bounds__(kAdjB) void k_adj_hess_tri(
   const int m = tr_n[t];
   double s0 = 0., s12 = 0., s3 = 0.;
   {
-    int32_t iv[kAdjBatchT];
+    int32_t iv[kAdjBatchHess];
 #pragma unroll
-    for (int e = 0; e < kAdjBatchT; ++e) iv[e] = e < m ? s[e * kAdjB] : 0;
-    double bv[kAdjBatchT][3];
+    for (int e = 0; e < kAdjBatchHess; ++e) iv[e] = e < m ? s[e * kAdjB] : 0;
+    double bv[kAdjBatchHess][3];
 #pragma unroll
-    for (int e = 0; e < kAdjBatchT; ++e) {
+    for (int e = 0; e < kAdjBatchHess; ++e) {
       const double* b = Sb + 3 * (int64_t)iv[e];
       bv[e][0] = e < m ? b[0] : 0.;
       bv[e][1] = e < m ? b[1] : 0.;
       bv[e][2] = e < m ? b[2] : 0.;
     }
 #pragma unroll
-    for (int e = 0; e < kAdjBatchT; ++e) {
+    for (int e = 0; e < kAdjBatchHess; ++e) {
       if (e < m) {
         s0 += bv[e][0];
         s12 += bv[e][1];
@@ -310,7 +312,7 @@ __global__ __launch_bounds__(kAdjB) void k_adj_hess_tri(
       }
     }
   }
-  for (int e = kAdjBatchT; e < m; ++e) {
+  for (int e = kAdjBatchHess; e < m; ++e) {
     const double* b = Sb + 3 * (int64_t)s[e * kAdjB];
     s0 += b[0];
     s12 += b[1];
@@ -398,18 +400,18 @@ __global__ __launch_bounds__(kAdjB) void k_adj_grad_tri(
   const int m = tr_n[t];
   double sx = 0., sy = 0., tb = 0.;
   {
-    int32_t iv[kAdjBatchT];
+    int32_t iv[kAdjBatchGrad];
 #pragma unroll
-    for (int e = 0; e < kAdjBatchT; ++e) iv[e] = e < m ? s[e * kAdjB] : 0;
-    double2 vv[kAdjBatchT];
-    double tv[kAdjBatchT];
+    for (int e = 0; e < kAdjBatchGrad; ++e) iv[e] = e < m ? s[e * kAdjB] : 0;
+    double2 vv[kAdjBatchGrad];
+    double tv[kAdjBatchGrad];
 #pragma unroll
-    for (int e = 0; e < kAdjBatchT; ++e) {
+    for (int e = 0; e < kAdjBatchGrad; ++e) {
       vv[e] = e < m ? Sgb[iv[e]] : make_double2(0., 0.);
       tv[e] = e < m ? TAb[iv[e]] : 0.;
     }
 #pragma unroll
-    for (int e = 0; e < kAdjBatchT; ++e) {
+    for (int e = 0; e < kAdjBatchGrad; ++e) {
       if (e < m) {
         sx += vv[e].x;
         sy += vv[e].y;
@@ -417,7 +419,7 @@ __global__ __launch_bounds__(kAdjB) void k_adj_grad_tri(
       }
     }
   }
-  for (int e = kAdjBatchT; e < m; ++e) {
+  for (int e = kAdjBatchGrad; e < m; ++e) {
     const int i = s[e * kAdjB];
     const double2 v = Sgb[i];
     sx += v.x;
