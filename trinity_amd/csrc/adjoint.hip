@@ -671,12 +671,11 @@ static int32_t ring_cap(const trn_mesh* m) {
 static double adj_forward(trn_ctx* ctx, trn_adjoint* a, const double* xy, const double* f,
                           double tf, int32_t p, double hmin, double hmax) {
   const trn_mesh* m = a->mesh;
-  double C = 0.;
-  int rc = trn_metric_run(ctx, m, xy, f, tf, p, hmin, hmax, a->M, &C, a->G, a->Hs);
-  if (rc != TRN_OK) fail(rc, trn_last_error());
-  copy_async(ctx, a->Mloc, a->Hs, sizeof(double) * 3 * (size_t)m->nv);
-  rc = trn_normalize_local(ctx, m->nv, a->Mloc, p);
-  if (rc != TRN_OK) fail(rc, trn_last_error());
+  TRN_REQUIRE(hmin > 0 && hmax > 0, TRN_ERR_ARG, "h_min and h_max must be positive");
+  // the metric pass keeps the gradient, the raw Hessian, the local metric
+  // and the patch areas its own kernels form
+  const double C = metric_tape(ctx, m, xy, f, tf, p, hmin, hmax, a->M, a->G, a->Hs, a->Mloc,
+                               a->TA);
   const double target = (double)(int)std::floor(m->nv * tf);  // metric.cpp:39
   a->C = C;
   a->fact = target / C;
@@ -685,7 +684,6 @@ static double adj_forward(trn_ctx* ctx, trn_adjoint* a, const double* xy, const 
   copy_async(ctx, a->scal, ctx->pinned_f64, 2 * sizeof(double));
   launch(ctx, "adj_area", 0.0, k_adj_area, dim3(nblk(m->nt, 256)), dim3(256), 0, m->nt,
          (const int32_t*)m->tri, (const double2*)xy, a->area_t);
-  patch_area_sums(ctx, m, (const double*)a->area_t, a->TA);
   launch(ctx, "adj_logm", 0.0, k_adj_logm, dim3(nblk(m->nv, 256)), dim3(256), 0, m->nv,
          (const double*)a->M, a->logm);
   const int eb = (int)std::min<int64_t>(nblk(m->nt, 256), a->npartial);

@@ -397,11 +397,6 @@ static void launch_vtx_grad(trn_ctx* ctx, const char* name, double bytes, bool w
          area_t, ag_t, grad, tarea);
 }
 
-void patch_area_sums(trn_ctx* ctx, const trn_mesh* m, const double* area_t, double* tarea) {
-  launch_vtx_grad(ctx, "vtx_area", 0.0, false, false, m, nullptr, nullptr, area_t, nullptr,
-                  nullptr, tarea, m->tune.grad_waves);
-}
-
 // The per-triangle kernels run only for meshes with slabs too large for a
 // union (the union slabs compute their terms while staging).
 static void run_gradient(trn_ctx* ctx, const trn_mesh* m, const double* xy, const double* f,
@@ -508,6 +503,26 @@ void recover_launch(trn_ctx* ctx, const trn_mesh* m, const double* xy, const dou
   double* grad = grad_out ? grad_out : (double*)c.carve<double2>(m->nv);
   run_gradient(ctx, m, xy, f, grad, b);
   run_hessian(ctx, m, xy, grad, hess_out, tensor, true, metric_exponent(lp_norm), b);
+}
+
+double metric_tape(trn_ctx* ctx, const trn_mesh* m, const double* xy, const double* f,
+                   double target_factor, int lp_norm, double h_min, double h_max, double* tensor,
+                   double* grad, double* hess, double* mloc, double* tarea) {
+  check_mesh(m);
+  Scratch sc(ctx, grad_scratch_bytes(m));
+  GradBuffers b = carve_grad(sc, m);
+  b.tarea = tarea;
+  run_gradient(ctx, m, xy, f, grad, b);
+  run_hessian(ctx, m, xy, grad, hess, mloc, true, metric_exponent(lp_norm), b);
+  copy_async(ctx, tensor, mloc, sizeof(double) * 3 * (size_t)m->nv);
+  run_complexity(ctx, m, xy, tensor, b);
+  const double target = (double)(int)std::floor(m->nv * target_factor);  // metric.cpp:39
+  launch(ctx, "vtx_global", kBytesGlobPerV * m->nv, k_vtx_global, dim3(nblk(m->nv, 256)),
+         dim3(256), 0, m->nv, tensor, (const double*)b.cplx, target, 0.0, 1 / (h_min * h_min),
+         1 / (h_max * h_max));
+  copy_async(ctx, ctx->pinned_f64, b.cplx, sizeof(double));
+  sync(ctx);
+  return ctx->pinned_f64[0];
 }
 
 }  // namespace trn

@@ -36,10 +36,13 @@ size_t exclusive_scan_scratch(int64_t n);
 // Z-curve order of the vertices from their coordinates (order.hip).
 void spatial_order(trn_ctx* ctx, int32_t nv, const double* xy, int32_t* perm);
 
-// Per-vertex sums of per-triangle areas over the stored 2-ring patches, in
-// patch order (the patch areas of Metrics::computeGradient,
-// src/hessian.cpp:53), from metric.hip's gather.
-void patch_area_sums(trn_ctx* ctx, const trn_mesh* m, const double* area_t, double* tarea);
+// The metric pass as trn_metric_run runs it, for the adjoint's tape: the
+// gradient (grad), the raw Hessian (hess), the locally normalised metric
+// (mloc) and the patch areas (tarea) of the pass are kept beside the final
+// metric (tensor); returns the complexity (synchronises).
+double metric_tape(trn_ctx* ctx, const trn_mesh* m, const double* xy, const double* f,
+                   double target_factor, int lp_norm, double h_min, double h_max, double* tensor,
+                   double* grad, double* hess, double* mloc, double* tarea);
 
 // Internal launch sequences shared by the entry points and the batched pass.
 // Each takes its device scratch from the caller (sized by the *_scratch
