@@ -649,8 +649,9 @@ def run_batched(T, mg, count=512, p=2, sweeps=2, steps=5, warmup=2, loop_steps=2
     # batched, on a context with its own stream (graph capture needs one)
     gctx = T.Context(0, use_torch_stream=False)
     gctx.use_own_stream()
-    b = T.Batch(gctx, [(m.nv, m.tri, m.tags) for m in ms])
-    xy_in = torch.from_numpy(np.concatenate([m.xy for m in ms])).cuda()
+    xy_host = np.concatenate([m.xy for m in ms])
+    b = T.Batch(gctx, [(m.nv, m.tri, m.tags) for m in ms], xy=xy_host)
+    xy_in = torch.from_numpy(xy_host).cuda()
     solut = torch.from_numpy(np.concatenate([m.solut for m in ms])).cuda()
     xy = torch.empty_like(xy_in)
     ten = torch.empty((nv, 3), dtype=torch.float64, device="cuda")
@@ -685,8 +686,9 @@ def run_batched(T, mg, count=512, p=2, sweeps=2, steps=5, warmup=2, loop_steps=2
     ctx = T.Context(0)
     dms, arrs = [], []
     for m in ms:
-        dms.append(T.DeviceMesh(ctx, m.nv, m.tri, m.tags))
-        arrs.append((torch.from_numpy(m.xy).cuda(), torch.from_numpy(m.solut).cuda(),
+        x0 = torch.from_numpy(m.xy).cuda()
+        dms.append(T.DeviceMesh(ctx, m.nv, m.tri, m.tags, xy=x0))
+        arrs.append((x0, torch.from_numpy(m.solut).cuda(),
                      torch.empty((m.nv, 2), dtype=torch.float64, device="cuda"),
                      torch.empty((m.nv, 3), dtype=torch.float64, device="cuda"),
                      torch.empty(m.nt, dtype=torch.float64, device="cuda")))

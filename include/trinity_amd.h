@@ -346,6 +346,14 @@ typedef struct trn_batch trn_batch;
 int trn_batch_create(trn_ctx* ctx, int32_t nmesh, const int32_t* nv,
                      const int32_t* nt, const int32_t* tri, const uint8_t* tags,
                      trn_batch** out);
+/* trn_batch_create with the meshes' coordinates (host or device, 2 doubles
+ * per vertex, concatenated like tags), used only for memory locality as in
+ * trn_mesh_create: the stored 2-ring patches of each mesh are grouped along
+ * the Z-curve of that mesh's own bounding box.  Results do not depend on
+ * it. */
+int trn_batch_create_xy(trn_ctx* ctx, int32_t nmesh, const int32_t* nv,
+                        const int32_t* nt, const int32_t* tri, const uint8_t* tags,
+                        const double* xy, trn_batch** out);
 int trn_batch_destroy(trn_batch* batch);
 /* Totals and the union topology (its CSR via trn_mesh_csr). */
 int trn_batch_info(const trn_batch* batch, int32_t* nv_total, int32_t* nt_total,

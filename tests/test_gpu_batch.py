@@ -157,6 +157,26 @@ def test_batch_graph_replays_equal_eager_pass(ctx):
         c2.close()
 
 
+def test_batch_with_coordinates_is_bit_identical(ctx):
+    """Coordinates given at creation (trn_batch_create_xy) only reorder the
+    stored patches along each mesh's Z-curve: the metric, the smoothed
+    coordinates and metric and the counters equal those of a batch built
+    without them, bit for bit."""
+    ms = _meshes(seed=23)
+    outs = []
+    for with_xy in (False, True):
+        xy_host = np.concatenate([m.xy for m in ms]) if with_xy else None
+        b = T().Batch(ctx, [(m.nv, m.tri, m.tags) for m in ms], xy=xy_host)
+        b.color(seed=3, max_colors=64)
+        xy_in, xy, solut, ten, q = _fields(ms)
+        cnt = b.run(xy_in, xy, solut, ten, q, TF, P, HMIN, HMAX, sweeps=2, counters=True)
+        outs.append((xy.cpu().numpy(), ten.cpu().numpy(), q.cpu().numpy(), cnt))
+        b.close()
+    (x0, t0, q0, c0), (x1, t1, q1, c1) = outs
+    assert np.array_equal(x0, x1) and np.array_equal(t0, t1) and np.array_equal(q0, q1)
+    assert [list(c) for c in c0] == [list(c) for c in c1]
+
+
 def test_batch_rejects_bad_input(ctx):
     m = mg.grid_mesh(4)
     bad = m.tri.copy()

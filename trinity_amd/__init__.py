@@ -46,7 +46,7 @@ EXPORTS = [
     "trn_smooth_prepare",
     "trn_smooth_run_prepared",
     "trn_smooth_run_prepared_noalloc",
-    "trn_batch_create", "trn_batch_destroy", "trn_batch_info", "trn_batch_color",
+    "trn_batch_create", "trn_batch_create_xy", "trn_batch_destroy", "trn_batch_info", "trn_batch_color",
     "trn_batch_coloring", "trn_batch_pass", "trn_batch_pass_graph",
     "trn_adjoint_create", "trn_adjoint_destroy", "trn_adjoint_forward", "trn_adjoint_backward",
     "trn_adjoint_smooth", "trn_debug_math",
@@ -131,6 +131,8 @@ def _load():
         "trn_smooth_run_prepared_noalloc": ([_vp, _vp, _vp, _vp, _vp, _vp, _P(_i32), _i32, _i32,
                                              _P(_i32), _P(_i32)], ctypes.c_int),
         "trn_batch_create": ([_vp, _i32, _P(_i32), _P(_i32), _vp, _vp, _P(_vp)], ctypes.c_int),
+        "trn_batch_create_xy": ([_vp, _i32, _P(_i32), _P(_i32), _vp, _vp, _vp, _P(_vp)],
+                                ctypes.c_int),
         "trn_batch_destroy": ([_vp], ctypes.c_int),
         "trn_batch_info": ([_vp, _P(_i32), _P(_i32), _P(_vp)], ctypes.c_int),
         "trn_batch_color": ([_vp, _vp, ctypes.c_uint64, _i32, _P(_i32)], ctypes.c_int),
@@ -644,9 +646,12 @@ class Batch:
 
     meshes: list of (nv, tri[nt,3] 0-based per mesh, tags[nv]).  Field
     arrays passed to `run` are the per-mesh arrays concatenated in list
-    order; `voff` / `toff` give each mesh's first vertex / triangle."""
+    order; `voff` / `toff` give each mesh's first vertex / triangle.  xy
+    (optional, the meshes' coordinates concatenated, host or device) only
+    orders the stored patches along each mesh's Z-curve
+    (trn_batch_create_xy)."""
 
-    def __init__(self, ctx: Context, meshes):
+    def __init__(self, ctx: Context, meshes, xy=None):
         self.ctx = ctx
         self.nmesh = len(meshes)
         nv = np.array([int(m[0]) for m in meshes], dtype=np.int32)
@@ -659,10 +664,25 @@ class Batch:
         self.toff = np.concatenate([[0], np.cumsum(nt, dtype=np.int64)])
         self.nv, self.nt = int(self.voff[-1]), int(self.toff[-1])
         h = _vp()
-        _check(lib().trn_batch_create(ctx.handle, self.nmesh,
-                                      nv.ctypes.data_as(_P(_i32)), nt.ctypes.data_as(_P(_i32)),
-                                      tri.ctypes.data_as(_vp), tags.ctypes.data_as(_vp),
-                                      ctypes.byref(h)))
+        if xy is None:
+            _check(lib().trn_batch_create(ctx.handle, self.nmesh,
+                                          nv.ctypes.data_as(_P(_i32)),
+                                          nt.ctypes.data_as(_P(_i32)), tri.ctypes.data_as(_vp),
+                                          tags.ctypes.data_as(_vp), ctypes.byref(h)))
+        else:
+            if isinstance(xy, np.ndarray):
+                xy_h = np.ascontiguousarray(xy, dtype=np.float64)
+                xp = xy_h.ctypes.data_as(_vp)
+            else:
+                xy_h = xy.contiguous()
+                xp = _ptr(xy_h)
+            if int(np.prod(xy_h.shape)) != 2 * self.nv:
+                raise ValueError("xy must hold 2 values per vertex of the batch")
+            _check(lib().trn_batch_create_xy(ctx.handle, self.nmesh,
+                                             nv.ctypes.data_as(_P(_i32)),
+                                             nt.ctypes.data_as(_P(_i32)),
+                                             tri.ctypes.data_as(_vp), tags.ctypes.data_as(_vp),
+                                             xp, ctypes.byref(h)))
         self.handle = h
         self.ncolors = 0
 

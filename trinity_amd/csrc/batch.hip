@@ -222,9 +222,11 @@ static int32_t* enqueue_pass(trn_ctx* ctx, trn_batch* b, const double* xy_in, do
 
 extern "C" {
 
-int trn_batch_create(trn_ctx* ctx, int32_t nmesh, const int32_t* nv, const int32_t* nt,
-                     const int32_t* tri, const uint8_t* tags, trn_batch** out) {
-  return guard_impl("trn_batch_create", [&] {
+static int batch_create(const char* name, trn_ctx* ctx, int32_t nmesh, const int32_t* nv,
+                        const int32_t* nt, const int32_t* tri, const uint8_t* tags,
+                        const double* xy, bool want_xy, trn_batch** out) {
+  return guard_impl(name, [&] {
+    TRN_REQUIRE(!want_xy || xy != nullptr, TRN_ERR_ARG, "null coordinates");
     DeviceScope ds(ctx);
     TRN_REQUIRE(out && nv && nt && tri && tags && nmesh > 0, TRN_ERR_ARG, "bad arguments");
     *out = nullptr;
@@ -260,8 +262,6 @@ int trn_batch_create(trn_ctx* ctx, int32_t nmesh, const int32_t* nv, const int32
           }
         }
       }
-      const int rc = trn_mesh_create(ctx, b->nv, b->nt, gtri.data(), tags, nullptr, &b->mesh);
-      if (rc != TRN_OK) fail(rc, trn_last_error());
       // complexity tiles: mesh m gets the plain path's grid for its size
       std::vector<int32_t> tile0(nmesh + 1, 0), tile_mesh;
       std::vector<int32_t> mesh_of(b->nv);
@@ -277,6 +277,31 @@ int trn_batch_create(trn_ctx* ctx, int32_t nmesh, const int32_t* nv, const int32
       dalloc(&b->voff_d, nmesh + 1);
       dalloc(&b->toff_d, nmesh + 1);
       dalloc(&b->mesh_of, b->nv);
+      copy_async(ctx, b->voff_d, b->voff.data(), sizeof(int32_t) * (nmesh + 1));
+      copy_async(ctx, b->mesh_of, mesh_of.data(), sizeof(int32_t) * b->nv);
+      if (xy) {
+        // patch slabs along each mesh's own Z-curve
+        double* xy_d = nullptr;
+        int32_t* perm = nullptr;
+        dalloc(&xy_d, 2 * (int64_t)b->nv);
+        dalloc(&perm, b->nv);
+        try {
+          copy_async(ctx, xy_d, xy, sizeof(double) * 2 * (size_t)b->nv);
+          spatial_order_segments(ctx, b->nv, xy_d, nmesh, (const int32_t*)b->mesh_of,
+                                 (const int32_t*)b->voff_d, perm);
+          b->mesh = mesh_create_ordered(ctx, b->nv, b->nt, gtri.data(), tags, perm);
+        } catch (...) {
+          (void)hipFree(xy_d);
+          (void)hipFree(perm);
+          throw;
+        }
+        sync(ctx);
+        TRN_HIP(hipFree(xy_d));
+        TRN_HIP(hipFree(perm));
+      } else {
+        const int rc = trn_mesh_create(ctx, b->nv, b->nt, gtri.data(), tags, nullptr, &b->mesh);
+        if (rc != TRN_OK) fail(rc, trn_last_error());
+      }
       dalloc(&b->tile_mesh, b->ntiles);
       dalloc(&b->tile0, nmesh + 1);
       dalloc(&b->partial, b->ntiles);
@@ -285,9 +310,7 @@ int trn_batch_create(trn_ctx* ctx, int32_t nmesh, const int32_t* nv, const int32
       dalloc(&b->logm, b->nv);
       dalloc(&b->color, b->nv);
       dalloc(&b->order, b->nv);
-      copy_async(ctx, b->voff_d, b->voff.data(), sizeof(int32_t) * (nmesh + 1));
       copy_async(ctx, b->toff_d, b->toff.data(), sizeof(int32_t) * (nmesh + 1));
-      copy_async(ctx, b->mesh_of, mesh_of.data(), sizeof(int32_t) * b->nv);
       copy_async(ctx, b->tile_mesh, tile_mesh.data(), sizeof(int32_t) * b->ntiles);
       copy_async(ctx, b->tile0, tile0.data(), sizeof(int32_t) * (nmesh + 1));
       sync(ctx);  // the host vectors go out of scope
@@ -298,6 +321,17 @@ int trn_batch_create(trn_ctx* ctx, int32_t nmesh, const int32_t* nv, const int32
     }
     *out = b;
   });
+}
+
+int trn_batch_create(trn_ctx* ctx, int32_t nmesh, const int32_t* nv, const int32_t* nt,
+                     const int32_t* tri, const uint8_t* tags, trn_batch** out) {
+  return batch_create("trn_batch_create", ctx, nmesh, nv, nt, tri, tags, nullptr, false, out);
+}
+
+int trn_batch_create_xy(trn_ctx* ctx, int32_t nmesh, const int32_t* nv, const int32_t* nt,
+                        const int32_t* tri, const uint8_t* tags, const double* xy,
+                        trn_batch** out) {
+  return batch_create("trn_batch_create_xy", ctx, nmesh, nv, nt, tri, tags, xy, true, out);
 }
 
 int trn_batch_destroy(trn_batch* b) {

@@ -130,4 +130,89 @@ void spatial_order(trn_ctx* ctx, int32_t nv, const double* xy, int32_t* perm) {
                                              (const int32_t*)idx, perm, nv, 0, 32, ctx->stream));
 }
 
+// The order for a disjoint union of meshes: vertices grouped by mesh (seg_of
+// [nv], mesh m owning [seg_off[m], seg_off[m + 1])), each mesh along the
+// Z-curve of its own bounding box (one block per mesh reduces it).
+__global__ __launch_bounds__(256) void k_bbox_segments(const int32_t* __restrict__ seg_off,
+                                                       const double2* __restrict__ xy,
+                                                       double* __restrict__ box) {
+  __shared__ double s[4][4];
+  const int64_t b = seg_off[blockIdx.x], e = seg_off[blockIdx.x + 1];
+  double lo_x = INFINITY, lo_y = INFINITY, hi_x = -INFINITY, hi_y = -INFINITY;
+  for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+    const double2 p = xy[i];
+    lo_x = fmin(lo_x, p.x);
+    lo_y = fmin(lo_y, p.y);
+    hi_x = fmax(hi_x, p.x);
+    hi_y = fmax(hi_y, p.y);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lo_x = fmin(lo_x, __shfl_xor(lo_x, o, 64));
+    lo_y = fmin(lo_y, __shfl_xor(lo_y, o, 64));
+    hi_x = fmax(hi_x, __shfl_xor(hi_x, o, 64));
+    hi_y = fmax(hi_y, __shfl_xor(hi_y, o, 64));
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s[w][0] = lo_x;
+    s[w][1] = lo_y;
+    s[w][2] = hi_x;
+    s[w][3] = hi_y;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+      s[0][0] = fmin(s[0][0], s[k][0]);
+      s[0][1] = fmin(s[0][1], s[k][1]);
+      s[0][2] = fmax(s[0][2], s[k][2]);
+      s[0][3] = fmax(s[0][3], s[k][3]);
+    }
+    for (int c = 0; c < 4; ++c) box[4 * blockIdx.x + c] = s[0][c];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_morton_segments(int32_t nv, const double2* __restrict__ xy,
+                                                         const int32_t* __restrict__ seg_of,
+                                                         const double* __restrict__ box,
+                                                         uint64_t* __restrict__ key,
+                                                         int32_t* __restrict__ idx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const int32_t g = seg_of[i];
+  const double* bx = box + 4 * (int64_t)g;
+  const double x0 = bx[0], y0 = bx[1];
+  const double ext = fmax(bx[2] - x0, bx[3] - y0);
+  const double scale = ext > 0. ? 65536. / ext : 0.;
+  const double2 p = xy[i];
+  const uint32_t z =
+      spread_bits(quantize(p.x, x0, scale)) | (spread_bits(quantize(p.y, y0, scale)) << 1);
+  key[i] = ((uint64_t)(uint32_t)g << 32) | z;
+  idx[i] = (int32_t)i;
+}
+
+void spatial_order_segments(trn_ctx* ctx, int32_t nv, const double* xy, int32_t nseg,
+                            const int32_t* seg_of, const int32_t* seg_off, int32_t* perm) {
+  int bits = 32;
+  while (bits < 64 && ((int64_t)1 << (bits - 32)) < (int64_t)nseg) ++bits;
+  size_t sort_bytes = 0;
+  TRN_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const uint64_t*)nullptr,
+                                             (uint64_t*)nullptr, (const int32_t*)nullptr,
+                                             (int32_t*)nullptr, nv, 0, bits, ctx->stream));
+  Scratch sc(ctx, scratch_bytes(sizeof(double) * 4 * (size_t)nseg) +
+                      2 * scratch_bytes(sizeof(uint64_t) * (size_t)nv) +
+                      scratch_bytes(sizeof(int32_t) * (size_t)nv) + scratch_bytes(sort_bytes));
+  double* box = sc.carve<double>(4 * (size_t)nseg);
+  uint64_t* key = sc.carve<uint64_t>(nv);
+  uint64_t* key2 = sc.carve<uint64_t>(nv);
+  int32_t* idx = sc.carve<int32_t>(nv);
+  void* tmp = sc.carve<char>(sort_bytes);
+  launch(ctx, "order_bbox", 16.0 * nv, k_bbox_segments, dim3((unsigned)nseg), dim3(256), 0,
+         seg_off, (const double2*)xy, box);
+  launch(ctx, "order_morton", 28.0 * nv, k_morton_segments, dim3((unsigned)((nv + 255) / 256)),
+         dim3(256), 0, nv, (const double2*)xy, seg_of, (const double*)box, key, idx);
+  TRN_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, sort_bytes, (const uint64_t*)key, key2,
+                                             (const int32_t*)idx, perm, nv, 0, bits, ctx->stream));
+}
+
 }  // namespace trn

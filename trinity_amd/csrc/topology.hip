@@ -701,6 +701,229 @@ static unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b
 
 using namespace trn;
 
+namespace trn {
+
+// trn_mesh_create's work; the patch slabs follow order_d when given, else the
+// Z-curve of xy_d when given, else index order.  Throws on error.
+static trn_mesh* mesh_create_impl(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
+                                  const uint8_t* tags, const double* xy_d,
+                                  const int32_t* order_d) {
+  TRN_REQUIRE(nv > 0 && nt > 0, TRN_ERR_ARG, "nv and nt must be positive");
+  TRN_REQUIRE(tri != nullptr && tags != nullptr, TRN_ERR_ARG, "null tri or tags");
+  TRN_REQUIRE((int64_t)nt * 3 < (int64_t)INT32_MAX, TRN_ERR_ARG, "too many triangles");
+  trn_mesh* m = new trn_mesh();
+  m->ctx = ctx;
+  m->nv = nv;
+  m->nt = nt;
+  auto cleanup = [&]() { trn_mesh_destroy(m); };
+  try {
+    TRN_HIP(hipMalloc(&m->tri, sizeof(int32_t) * 3 * (size_t)nt));
+    TRN_HIP(hipMalloc(&m->tags, (size_t)nv));
+    TRN_HIP(hipMalloc(&m->v2t_off, sizeof(int32_t) * ((size_t)nv + 1)));
+    TRN_HIP(hipMalloc(&m->v2t, sizeof(int32_t) * 3 * (size_t)nt));
+    TRN_HIP(hipMalloc(&m->v2v_off, sizeof(int32_t) * ((size_t)nv + 1)));
+    copy_async(ctx, m->tri, tri, sizeof(int32_t) * 3 * (size_t)nt);
+    copy_async(ctx, m->tags, tags, (size_t)nv);
+
+    // 0. locality order (Z-curve of the coordinates): the vertices of a
+    //    patch slab, and on request the visiting order of the colour
+    //    classes.  It takes the context workspace, so it runs before the
+    //    workspace below is carved.
+    if (xy_d || order_d) {
+      TRN_HIP(hipMalloc(&m->zorder, sizeof(int32_t) * (size_t)nv));
+      if (order_d)
+        copy_async(ctx, m->zorder, order_d, sizeof(int32_t) * (size_t)nv);
+      else
+        spatial_order(ctx, nv, xy_d, m->zorder);
+      m->slab_v = m->zorder;
+    }
+
+    Scratch sc(ctx, 4 * scratch_bytes(sizeof(int32_t) * ((size_t)nv + 1)) +
+                        exclusive_scan_scratch(nv) + 256 * 4);
+    int32_t* cnt = sc.carve<int32_t>((size_t)nv + 1);
+    int32_t* cursor = sc.carve<int32_t>((size_t)nv + 1);
+    int32_t* cnt2 = sc.carve<int32_t>((size_t)nv + 1);
+    int32_t* bound = sc.carve<int32_t>((size_t)nv + 1);
+    int32_t* flags = sc.carve<int32_t>(64);
+    int32_t* scan_tmp = sc.carve<int32_t>(exclusive_scan_scratch(nv) / 4);
+    int32_t* h = ctx->pinned_i32;
+
+    // 1. validate indices (an index out of range would make the atomics
+    //    below write outside their arrays)
+    memset_async(ctx, flags, 0, 64 * sizeof(int32_t));
+    launch(ctx, "topo_validate", 12.0 * nt, k_validate_tri, dim3(blocks_for(nt, 256)),
+           dim3(256), 0, nv, nt, (const int32_t*)m->tri, flags);
+    copy_async(ctx, h, flags, sizeof(int32_t));
+    sync(ctx);
+    TRN_REQUIRE(h[0] == 0, TRN_ERR_ARG,
+                std::to_string(h[0]) + " triangles have an index outside [0, nv) or repeat a vertex");
+
+    // 2. v2t CSR: rows of kV2tRow slots filled and counted in one pass;
+    //    when a degree exceeds the row, the counts seed the two-pass fill
+    //    (positions, then sorting)
+    memset_async(ctx, cnt, 0, sizeof(int32_t) * (size_t)nv);
+    int32_t* rows = (int32_t*)aux_workspace(ctx, sizeof(int32_t) * kV2tRow * (size_t)nv);
+    launch(ctx, "topo_fill_v2t", 36.0 * nt, k_fill_v2t_rows, dim3(blocks_for(nt, 256)),
+           dim3(256), 0, nt, (const int32_t*)m->tri, cnt, rows);
+    reduce_max_i32(ctx, cnt, nv, flags + 1);
+    copy_async(ctx, h, flags + 1, sizeof(int32_t));
+    exclusive_scan_i32(ctx, cnt, m->v2t_off, nv, true, scan_tmp);
+    sync(ctx);
+    m->max_deg_t = h[0];
+    if (m->max_deg_t <= kV2tRow) {
+      launch(ctx, "topo_sort_v2t", 8.0 * nv + 24.0 * nt, k_v2t_from_rows,
+             dim3(blocks_for(nv, 256)), dim3(256), 0, nv, (const int32_t*)cnt,
+             (const int32_t*)m->v2t_off, (const int32_t*)rows, m->v2t);
+    } else {
+      copy_async(ctx, cursor, m->v2t_off, sizeof(int32_t) * (size_t)nv);
+      launch(ctx, "topo_fill_v2t", 36.0 * nt, k_fill_v2t, dim3(blocks_for(nt, 256)),
+             dim3(256), 0, nt, (const int32_t*)m->tri, cursor, m->v2t);
+      launch(ctx, "topo_sort_v2t", 8.0 * nv + 24.0 * nt, k_sort_lists,
+             dim3(blocks_for(nv, 256)), dim3(256), 0, nv, (const int32_t*)m->v2t_off, m->v2t);
+    }
+    TRN_REQUIRE(m->max_deg_t <= kMaxDegree, TRN_ERR_CAPACITY,
+                "a vertex has " + std::to_string(m->max_deg_t) +
+                    " incident triangles; at most " + std::to_string(kMaxDegree) +
+                    " are supported");
+
+    // 3. v2v CSR (count, scan, fill)
+    const int capv = 2 * (m->max_deg_t > 0 ? m->max_deg_t : 1);
+    const size_t shv = sizeof(int) * 64 * (size_t)capv;
+    memset_async(ctx, flags + 2, 0, sizeof(int32_t));
+    launch(ctx, "topo_count_v2v", 0.0, k_v2v<0>, dim3(blocks_for(nv, 64)), dim3(64), shv, nv,
+           (const int32_t*)m->tri, (const int32_t*)m->v2t_off, (const int32_t*)m->v2t, capv,
+           cnt2, (const int32_t*)nullptr, (int32_t*)nullptr, flags + 2);
+    exclusive_scan_i32(ctx, cnt2, m->v2v_off, nv, true, scan_tmp);
+    reduce_max_i32(ctx, cnt2, nv, flags + 3);
+    copy_async(ctx, h, flags + 2, 2 * sizeof(int32_t));
+    copy_async(ctx, h + 2, m->v2v_off + nv, sizeof(int32_t));
+    sync(ctx);
+    TRN_REQUIRE(h[0] == 0, TRN_ERR_STATE, "1-ring overflow");
+    m->max_deg_v = h[1];
+    m->nnz_v2v = h[2];
+    TRN_HIP(hipMalloc(&m->v2v, sizeof(int32_t) * (size_t)(m->nnz_v2v > 0 ? m->nnz_v2v : 1)));
+    launch(ctx, "topo_fill_v2v", 0.0, k_v2v<1>, dim3(blocks_for(nv, 64)), dim3(64), shv, nv,
+           (const int32_t*)m->tri, (const int32_t*)m->v2t_off, (const int32_t*)m->v2t, capv,
+           (int32_t*)nullptr, (const int32_t*)m->v2v_off, m->v2v, flags + 2);
+
+    // 3b. bandwidth of the numbering (longest edge |u - v|)
+    launch(ctx, "topo_span", 8.0 * nv + 4.0 * (double)m->nnz_v2v, k_vertex_span,
+           dim3(blocks_for(nv, 256)), dim3(256), 0, nv, (const int32_t*)m->v2v_off,
+           (const int32_t*)m->v2v, cnt);
+    reduce_max_i32(ctx, cnt, nv, flags + 7);
+    copy_async(ctx, h, flags + 7, sizeof(int32_t));
+    sync(ctx);
+    m->bandwidth = h[0];
+
+    // 4. largest 2-ring element patch
+    launch(ctx, "topo_patch_bound", 0.0, k_patch_bound, dim3(blocks_for(nv, 256)), dim3(256),
+           0, nv, (const int32_t*)m->v2t_off, (const int32_t*)m->v2v_off,
+           (const int32_t*)m->v2v, bound);
+    reduce_max_i32(ctx, bound, nv, flags + 4);
+    copy_async(ctx, h, flags + 4, sizeof(int32_t));
+    sync(ctx);
+    int capp = h[0] < kMaxPatchCap ? h[0] : kMaxPatchCap;
+    if (capp < 1) capp = 1;
+    memset_async(ctx, flags + 5, 0, sizeof(int32_t));
+    // the patches themselves are kept for the ELL fill below when the
+    // bound makes the buffer affordable (else they are built again there)
+    int32_t* patch_tmp = nullptr;
+    if ((size_t)capp * (size_t)nv * sizeof(int32_t) <= kPatchTmpMax)
+      patch_tmp = (int32_t*)aux_workspace(ctx, (size_t)capp * (size_t)nv * sizeof(int32_t));
+    launch(ctx, "topo_patch_size", 0.0, k_patch_size, dim3(blocks_for(nv, 64)), dim3(64),
+           sizeof(int) * 64 * (size_t)capp, nv, (const int32_t*)m->slab_v,
+           (const int32_t*)m->v2t_off, (const int32_t*)m->v2t, (const int32_t*)m->v2v_off,
+           (const int32_t*)m->v2v, capp, bound, flags + 5, patch_tmp);
+    reduce_max_i32(ctx, bound, nv, flags + 6);
+    copy_async(ctx, h, flags + 5, 2 * sizeof(int32_t));
+    sync(ctx);
+    TRN_REQUIRE(h[0] == 0, TRN_ERR_CAPACITY,
+                "a 2-ring element patch exceeds " + std::to_string(kMaxPatchCap) + " triangles");
+    m->max_patch = h[1];
+    m->patch_cap = m->max_patch > 0 ? m->max_patch : 1;
+
+    // 4b. stored patches (sliced ELL) for the recovery gathers
+    const int64_t nslab = (nv + kPatchSlab - 1) / kPatchSlab;
+    TRN_HIP(hipMalloc(&m->patch_n, sizeof(int32_t) * (size_t)nv));
+    TRN_HIP(hipMalloc(&m->patch_row, sizeof(int32_t) * (size_t)(nslab + 1)));
+    copy_async(ctx, m->patch_n, bound, sizeof(int32_t) * (size_t)nv);
+    launch(ctx, "topo_patch_rows", 0.0, k_patch_slab_rows, dim3((unsigned)nslab),
+           dim3(kPatchSlab), 0, nv, (const int32_t*)m->slab_v, (const int32_t*)bound,
+           m->patch_row);
+    exclusive_scan_i32(ctx, m->patch_row, m->patch_row, nslab, true, scan_tmp);
+    copy_async(ctx, h, m->patch_row + nslab, sizeof(int32_t));
+    sync(ctx);
+    const int64_t ell_n = (int64_t)h[0] * kPatchSlab;
+    TRN_HIP(hipMalloc(&m->patch_ell, sizeof(int32_t) * (size_t)(ell_n > 0 ? ell_n : 1)));
+    if (patch_tmp) {
+      launch(ctx, "topo_patch_copy", 0.0, k_patch_copy, dim3((unsigned)nslab),
+             dim3(kPatchSlab), 0, nv, (const int32_t*)m->slab_v, (const int32_t*)bound,
+             (const int32_t*)patch_tmp, (const int32_t*)m->patch_row, m->patch_ell);
+    } else {
+      launch(ctx, "topo_patch_fill", 0.0, k_patch_fill, dim3((unsigned)nslab),
+             dim3(kPatchSlab), sizeof(int) * kPatchSlab * (size_t)m->patch_cap, nv,
+             (const int32_t*)m->slab_v, (const int32_t*)m->v2t_off, (const int32_t*)m->v2t,
+             (const int32_t*)m->v2v_off, (const int32_t*)m->v2v, m->patch_cap,
+             (const int32_t*)m->patch_row, m->patch_ell);
+    }
+
+
+    // 4c. per-slab unions of the patches (LDS staging of the gathers),
+    //     each stored in its slab's ELL block: one sort per slab
+    TRN_HIP(hipMalloc(&m->union_n, sizeof(int32_t) * (size_t)nslab));
+    TRN_HIP(hipMalloc(&m->union_off, sizeof(int32_t) * (size_t)(nslab + 1)));
+    TRN_HIP(hipMalloc(&m->union_ids, sizeof(int32_t) * (size_t)(ell_n > 0 ? ell_n : 1)));
+    TRN_HIP(hipMalloc(&m->patch_local, sizeof(uint16_t) * (size_t)(ell_n > 0 ? ell_n : 1)));
+    launch(ctx, "topo_union_offsets", 0.0, k_union_offsets, dim3(blocks_for(nslab + 1, 256)),
+           dim3(256), 0, (int32_t)nslab, (const int32_t*)m->patch_row, m->union_off);
+    launch(ctx, "topo_patch_union", 0.0, k_patch_union, dim3((unsigned)nslab), dim3(kUnionB),
+           0, nv, (const int32_t*)m->patch_row, (const int32_t*)m->patch_ell, m->union_n,
+           (const int32_t*)m->union_off, m->union_ids, m->patch_local);
+    reduce_max_i32(ctx, m->union_n, nslab, flags + 7);
+    memset_async(ctx, flags + 8, 0, sizeof(int32_t));
+    launch(ctx, "topo_union_count", 0.0, k_count_negative, dim3(blocks_for(nslab, 256)),
+           dim3(256), 0, nslab, (const int32_t*)m->union_n, flags + 8);
+    TRN_HIP(hipMalloc(&m->union_cu, sizeof(int32_t) * (size_t)(nslab + 1)));
+    launch(ctx, "topo_union_sizes", 0.0, k_union_sizes, dim3(blocks_for(nslab, 256)),
+           dim3(256), 0, nslab, (const int32_t*)m->union_n, cnt2);
+    exclusive_scan_i32(ctx, cnt2, m->union_cu, nslab, true, scan_tmp);
+    copy_async(ctx, h + 1, flags + 7, 2 * sizeof(int32_t));
+    copy_async(ctx, h + 3, m->union_cu + nslab, sizeof(int32_t));
+    sync(ctx);
+    m->max_union = h[1] > 0 ? h[1] : 0;
+    m->union_all = h[2] == 0;
+    TRN_HIP(hipMalloc(&m->union_tri, sizeof(int32_t) * 3 * (size_t)(h[3] > 0 ? h[3] : 1)));
+    launch(ctx, "topo_union_tri", 0.0, k_union_tri, dim3((unsigned)nslab), dim3(256), 0,
+           (const int32_t*)m->union_n, (const int32_t*)m->union_off,
+           (const int32_t*)m->union_ids, (const int32_t*)m->union_cu, (const int32_t*)m->tri,
+           m->union_tri);
+
+    // 5. star codes of the incident triangles (smoothing reads corners
+    //    through the 1-ring instead of the triangle array)
+    TRN_REQUIRE(m->max_deg_v < 256, TRN_ERR_CAPACITY, "1-ring larger than 255 vertices");
+    TRN_HIP(hipMalloc(&m->star, sizeof(int32_t) * 3 * (size_t)nt));
+    TRN_HIP(hipMalloc(&m->vdata, sizeof(VData) * (size_t)nv));
+    TRN_HIP(hipMalloc(&m->vt2, sizeof(double) * (size_t)nv));
+    TRN_HIP(hipMalloc(&m->vtopo, sizeof(VTopo) * (size_t)nv));
+    launch(ctx, "topo_star", 0.0, k_star_vtopo, dim3(blocks_for(nv, 256)), dim3(256), 0, nv,
+           (const uint8_t*)m->tags, (const int32_t*)m->tri, (const int32_t*)m->v2t_off,
+           (const int32_t*)m->v2t, (const int32_t*)m->v2v_off, (const int32_t*)m->v2v,
+           m->star, m->vtopo);
+
+  } catch (...) {
+    cleanup();
+    throw;
+  }
+  return m;
+}
+
+trn_mesh* mesh_create_ordered(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
+                              const uint8_t* tags, const int32_t* order_d) {
+  return mesh_create_impl(ctx, nv, nt, tri, tags, nullptr, order_d);
+}
+
+}  // namespace trn
+
 extern "C" {
 
 int trn_mesh_create(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
@@ -709,210 +932,7 @@ int trn_mesh_create(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
     DeviceScope ds(ctx);
     TRN_REQUIRE(out != nullptr, TRN_ERR_ARG, "null output");
     *out = nullptr;
-    TRN_REQUIRE(nv > 0 && nt > 0, TRN_ERR_ARG, "nv and nt must be positive");
-    TRN_REQUIRE(tri != nullptr && tags != nullptr, TRN_ERR_ARG, "null tri or tags");
-    TRN_REQUIRE((int64_t)nt * 3 < (int64_t)INT32_MAX, TRN_ERR_ARG, "too many triangles");
-    trn_mesh* m = new trn_mesh();
-    m->ctx = ctx;
-    m->nv = nv;
-    m->nt = nt;
-    auto cleanup = [&]() { trn_mesh_destroy(m); };
-    try {
-      TRN_HIP(hipMalloc(&m->tri, sizeof(int32_t) * 3 * (size_t)nt));
-      TRN_HIP(hipMalloc(&m->tags, (size_t)nv));
-      TRN_HIP(hipMalloc(&m->v2t_off, sizeof(int32_t) * ((size_t)nv + 1)));
-      TRN_HIP(hipMalloc(&m->v2t, sizeof(int32_t) * 3 * (size_t)nt));
-      TRN_HIP(hipMalloc(&m->v2v_off, sizeof(int32_t) * ((size_t)nv + 1)));
-      copy_async(ctx, m->tri, tri, sizeof(int32_t) * 3 * (size_t)nt);
-      copy_async(ctx, m->tags, tags, (size_t)nv);
-
-      // 0. locality order (Z-curve of the coordinates): the vertices of a
-      //    patch slab, and on request the visiting order of the colour
-      //    classes.  It takes the context workspace, so it runs before the
-      //    workspace below is carved.
-      if (xy_d) {
-        TRN_HIP(hipMalloc(&m->zorder, sizeof(int32_t) * (size_t)nv));
-        spatial_order(ctx, nv, xy_d, m->zorder);
-        m->slab_v = m->zorder;
-      }
-
-      Scratch sc(ctx, 4 * scratch_bytes(sizeof(int32_t) * ((size_t)nv + 1)) +
-                          exclusive_scan_scratch(nv) + 256 * 4);
-      int32_t* cnt = sc.carve<int32_t>((size_t)nv + 1);
-      int32_t* cursor = sc.carve<int32_t>((size_t)nv + 1);
-      int32_t* cnt2 = sc.carve<int32_t>((size_t)nv + 1);
-      int32_t* bound = sc.carve<int32_t>((size_t)nv + 1);
-      int32_t* flags = sc.carve<int32_t>(64);
-      int32_t* scan_tmp = sc.carve<int32_t>(exclusive_scan_scratch(nv) / 4);
-      int32_t* h = ctx->pinned_i32;
-
-      // 1. validate indices (an index out of range would make the atomics
-      //    below write outside their arrays)
-      memset_async(ctx, flags, 0, 64 * sizeof(int32_t));
-      launch(ctx, "topo_validate", 12.0 * nt, k_validate_tri, dim3(blocks_for(nt, 256)),
-             dim3(256), 0, nv, nt, (const int32_t*)m->tri, flags);
-      copy_async(ctx, h, flags, sizeof(int32_t));
-      sync(ctx);
-      TRN_REQUIRE(h[0] == 0, TRN_ERR_ARG,
-                  std::to_string(h[0]) + " triangles have an index outside [0, nv) or repeat a vertex");
-
-      // 2. v2t CSR: rows of kV2tRow slots filled and counted in one pass;
-      //    when a degree exceeds the row, the counts seed the two-pass fill
-      //    (positions, then sorting)
-      memset_async(ctx, cnt, 0, sizeof(int32_t) * (size_t)nv);
-      int32_t* rows = (int32_t*)aux_workspace(ctx, sizeof(int32_t) * kV2tRow * (size_t)nv);
-      launch(ctx, "topo_fill_v2t", 36.0 * nt, k_fill_v2t_rows, dim3(blocks_for(nt, 256)),
-             dim3(256), 0, nt, (const int32_t*)m->tri, cnt, rows);
-      reduce_max_i32(ctx, cnt, nv, flags + 1);
-      copy_async(ctx, h, flags + 1, sizeof(int32_t));
-      exclusive_scan_i32(ctx, cnt, m->v2t_off, nv, true, scan_tmp);
-      sync(ctx);
-      m->max_deg_t = h[0];
-      if (m->max_deg_t <= kV2tRow) {
-        launch(ctx, "topo_sort_v2t", 8.0 * nv + 24.0 * nt, k_v2t_from_rows,
-               dim3(blocks_for(nv, 256)), dim3(256), 0, nv, (const int32_t*)cnt,
-               (const int32_t*)m->v2t_off, (const int32_t*)rows, m->v2t);
-      } else {
-        copy_async(ctx, cursor, m->v2t_off, sizeof(int32_t) * (size_t)nv);
-        launch(ctx, "topo_fill_v2t", 36.0 * nt, k_fill_v2t, dim3(blocks_for(nt, 256)),
-               dim3(256), 0, nt, (const int32_t*)m->tri, cursor, m->v2t);
-        launch(ctx, "topo_sort_v2t", 8.0 * nv + 24.0 * nt, k_sort_lists,
-               dim3(blocks_for(nv, 256)), dim3(256), 0, nv, (const int32_t*)m->v2t_off, m->v2t);
-      }
-      TRN_REQUIRE(m->max_deg_t <= kMaxDegree, TRN_ERR_CAPACITY,
-                  "a vertex has " + std::to_string(m->max_deg_t) +
-                      " incident triangles; at most " + std::to_string(kMaxDegree) +
-                      " are supported");
-
-      // 3. v2v CSR (count, scan, fill)
-      const int capv = 2 * (m->max_deg_t > 0 ? m->max_deg_t : 1);
-      const size_t shv = sizeof(int) * 64 * (size_t)capv;
-      memset_async(ctx, flags + 2, 0, sizeof(int32_t));
-      launch(ctx, "topo_count_v2v", 0.0, k_v2v<0>, dim3(blocks_for(nv, 64)), dim3(64), shv, nv,
-             (const int32_t*)m->tri, (const int32_t*)m->v2t_off, (const int32_t*)m->v2t, capv,
-             cnt2, (const int32_t*)nullptr, (int32_t*)nullptr, flags + 2);
-      exclusive_scan_i32(ctx, cnt2, m->v2v_off, nv, true, scan_tmp);
-      reduce_max_i32(ctx, cnt2, nv, flags + 3);
-      copy_async(ctx, h, flags + 2, 2 * sizeof(int32_t));
-      copy_async(ctx, h + 2, m->v2v_off + nv, sizeof(int32_t));
-      sync(ctx);
-      TRN_REQUIRE(h[0] == 0, TRN_ERR_STATE, "1-ring overflow");
-      m->max_deg_v = h[1];
-      m->nnz_v2v = h[2];
-      TRN_HIP(hipMalloc(&m->v2v, sizeof(int32_t) * (size_t)(m->nnz_v2v > 0 ? m->nnz_v2v : 1)));
-      launch(ctx, "topo_fill_v2v", 0.0, k_v2v<1>, dim3(blocks_for(nv, 64)), dim3(64), shv, nv,
-             (const int32_t*)m->tri, (const int32_t*)m->v2t_off, (const int32_t*)m->v2t, capv,
-             (int32_t*)nullptr, (const int32_t*)m->v2v_off, m->v2v, flags + 2);
-
-      // 3b. bandwidth of the numbering (longest edge |u - v|)
-      launch(ctx, "topo_span", 8.0 * nv + 4.0 * (double)m->nnz_v2v, k_vertex_span,
-             dim3(blocks_for(nv, 256)), dim3(256), 0, nv, (const int32_t*)m->v2v_off,
-             (const int32_t*)m->v2v, cnt);
-      reduce_max_i32(ctx, cnt, nv, flags + 7);
-      copy_async(ctx, h, flags + 7, sizeof(int32_t));
-      sync(ctx);
-      m->bandwidth = h[0];
-
-      // 4. largest 2-ring element patch
-      launch(ctx, "topo_patch_bound", 0.0, k_patch_bound, dim3(blocks_for(nv, 256)), dim3(256),
-             0, nv, (const int32_t*)m->v2t_off, (const int32_t*)m->v2v_off,
-             (const int32_t*)m->v2v, bound);
-      reduce_max_i32(ctx, bound, nv, flags + 4);
-      copy_async(ctx, h, flags + 4, sizeof(int32_t));
-      sync(ctx);
-      int capp = h[0] < kMaxPatchCap ? h[0] : kMaxPatchCap;
-      if (capp < 1) capp = 1;
-      memset_async(ctx, flags + 5, 0, sizeof(int32_t));
-      // the patches themselves are kept for the ELL fill below when the
-      // bound makes the buffer affordable (else they are built again there)
-      int32_t* patch_tmp = nullptr;
-      if ((size_t)capp * (size_t)nv * sizeof(int32_t) <= kPatchTmpMax)
-        patch_tmp = (int32_t*)aux_workspace(ctx, (size_t)capp * (size_t)nv * sizeof(int32_t));
-      launch(ctx, "topo_patch_size", 0.0, k_patch_size, dim3(blocks_for(nv, 64)), dim3(64),
-             sizeof(int) * 64 * (size_t)capp, nv, (const int32_t*)m->slab_v,
-             (const int32_t*)m->v2t_off, (const int32_t*)m->v2t, (const int32_t*)m->v2v_off,
-             (const int32_t*)m->v2v, capp, bound, flags + 5, patch_tmp);
-      reduce_max_i32(ctx, bound, nv, flags + 6);
-      copy_async(ctx, h, flags + 5, 2 * sizeof(int32_t));
-      sync(ctx);
-      TRN_REQUIRE(h[0] == 0, TRN_ERR_CAPACITY,
-                  "a 2-ring element patch exceeds " + std::to_string(kMaxPatchCap) + " triangles");
-      m->max_patch = h[1];
-      m->patch_cap = m->max_patch > 0 ? m->max_patch : 1;
-
-      // 4b. stored patches (sliced ELL) for the recovery gathers
-      const int64_t nslab = (nv + kPatchSlab - 1) / kPatchSlab;
-      TRN_HIP(hipMalloc(&m->patch_n, sizeof(int32_t) * (size_t)nv));
-      TRN_HIP(hipMalloc(&m->patch_row, sizeof(int32_t) * (size_t)(nslab + 1)));
-      copy_async(ctx, m->patch_n, bound, sizeof(int32_t) * (size_t)nv);
-      launch(ctx, "topo_patch_rows", 0.0, k_patch_slab_rows, dim3((unsigned)nslab),
-             dim3(kPatchSlab), 0, nv, (const int32_t*)m->slab_v, (const int32_t*)bound,
-             m->patch_row);
-      exclusive_scan_i32(ctx, m->patch_row, m->patch_row, nslab, true, scan_tmp);
-      copy_async(ctx, h, m->patch_row + nslab, sizeof(int32_t));
-      sync(ctx);
-      const int64_t ell_n = (int64_t)h[0] * kPatchSlab;
-      TRN_HIP(hipMalloc(&m->patch_ell, sizeof(int32_t) * (size_t)(ell_n > 0 ? ell_n : 1)));
-      if (patch_tmp) {
-        launch(ctx, "topo_patch_copy", 0.0, k_patch_copy, dim3((unsigned)nslab),
-               dim3(kPatchSlab), 0, nv, (const int32_t*)m->slab_v, (const int32_t*)bound,
-               (const int32_t*)patch_tmp, (const int32_t*)m->patch_row, m->patch_ell);
-      } else {
-        launch(ctx, "topo_patch_fill", 0.0, k_patch_fill, dim3((unsigned)nslab),
-               dim3(kPatchSlab), sizeof(int) * kPatchSlab * (size_t)m->patch_cap, nv,
-               (const int32_t*)m->slab_v, (const int32_t*)m->v2t_off, (const int32_t*)m->v2t,
-               (const int32_t*)m->v2v_off, (const int32_t*)m->v2v, m->patch_cap,
-               (const int32_t*)m->patch_row, m->patch_ell);
-      }
-
-
-      // 4c. per-slab unions of the patches (LDS staging of the gathers),
-      //     each stored in its slab's ELL block: one sort per slab
-      TRN_HIP(hipMalloc(&m->union_n, sizeof(int32_t) * (size_t)nslab));
-      TRN_HIP(hipMalloc(&m->union_off, sizeof(int32_t) * (size_t)(nslab + 1)));
-      TRN_HIP(hipMalloc(&m->union_ids, sizeof(int32_t) * (size_t)(ell_n > 0 ? ell_n : 1)));
-      TRN_HIP(hipMalloc(&m->patch_local, sizeof(uint16_t) * (size_t)(ell_n > 0 ? ell_n : 1)));
-      launch(ctx, "topo_union_offsets", 0.0, k_union_offsets, dim3(blocks_for(nslab + 1, 256)),
-             dim3(256), 0, (int32_t)nslab, (const int32_t*)m->patch_row, m->union_off);
-      launch(ctx, "topo_patch_union", 0.0, k_patch_union, dim3((unsigned)nslab), dim3(kUnionB),
-             0, nv, (const int32_t*)m->patch_row, (const int32_t*)m->patch_ell, m->union_n,
-             (const int32_t*)m->union_off, m->union_ids, m->patch_local);
-      reduce_max_i32(ctx, m->union_n, nslab, flags + 7);
-      memset_async(ctx, flags + 8, 0, sizeof(int32_t));
-      launch(ctx, "topo_union_count", 0.0, k_count_negative, dim3(blocks_for(nslab, 256)),
-             dim3(256), 0, nslab, (const int32_t*)m->union_n, flags + 8);
-      TRN_HIP(hipMalloc(&m->union_cu, sizeof(int32_t) * (size_t)(nslab + 1)));
-      launch(ctx, "topo_union_sizes", 0.0, k_union_sizes, dim3(blocks_for(nslab, 256)),
-             dim3(256), 0, nslab, (const int32_t*)m->union_n, cnt2);
-      exclusive_scan_i32(ctx, cnt2, m->union_cu, nslab, true, scan_tmp);
-      copy_async(ctx, h + 1, flags + 7, 2 * sizeof(int32_t));
-      copy_async(ctx, h + 3, m->union_cu + nslab, sizeof(int32_t));
-      sync(ctx);
-      m->max_union = h[1] > 0 ? h[1] : 0;
-      m->union_all = h[2] == 0;
-      TRN_HIP(hipMalloc(&m->union_tri, sizeof(int32_t) * 3 * (size_t)(h[3] > 0 ? h[3] : 1)));
-      launch(ctx, "topo_union_tri", 0.0, k_union_tri, dim3((unsigned)nslab), dim3(256), 0,
-             (const int32_t*)m->union_n, (const int32_t*)m->union_off,
-             (const int32_t*)m->union_ids, (const int32_t*)m->union_cu, (const int32_t*)m->tri,
-             m->union_tri);
-
-      // 5. star codes of the incident triangles (smoothing reads corners
-      //    through the 1-ring instead of the triangle array)
-      TRN_REQUIRE(m->max_deg_v < 256, TRN_ERR_CAPACITY, "1-ring larger than 255 vertices");
-      TRN_HIP(hipMalloc(&m->star, sizeof(int32_t) * 3 * (size_t)nt));
-      TRN_HIP(hipMalloc(&m->vdata, sizeof(VData) * (size_t)nv));
-      TRN_HIP(hipMalloc(&m->vt2, sizeof(double) * (size_t)nv));
-      TRN_HIP(hipMalloc(&m->vtopo, sizeof(VTopo) * (size_t)nv));
-      launch(ctx, "topo_star", 0.0, k_star_vtopo, dim3(blocks_for(nv, 256)), dim3(256), 0, nv,
-             (const uint8_t*)m->tags, (const int32_t*)m->tri, (const int32_t*)m->v2t_off,
-             (const int32_t*)m->v2t, (const int32_t*)m->v2v_off, (const int32_t*)m->v2v,
-             m->star, m->vtopo);
-
-    } catch (...) {
-      cleanup();
-      throw;
-    }
-    *out = m;
+    *out = mesh_create_impl(ctx, nv, nt, tri, tags, xy_d, nullptr);
   });
 }
 

@@ -35,6 +35,15 @@ size_t exclusive_scan_scratch(int64_t n);
 
 // Z-curve order of the vertices from their coordinates (order.hip).
 void spatial_order(trn_ctx* ctx, int32_t nv, const double* xy, int32_t* perm);
+// The same for a disjoint union of meshes: grouped by mesh (seg_of[nv] its
+// mesh, seg_off[nseg + 1] the meshes' first vertices, device arrays), each
+// mesh along the Z-curve of its own bounding box.
+void spatial_order_segments(trn_ctx* ctx, int32_t nv, const double* xy, int32_t nseg,
+                            const int32_t* seg_of, const int32_t* seg_off, int32_t* perm);
+// trn_mesh_create with the vertex order of the patch slabs given (device,
+// a permutation of 0..nv-1) instead of derived from coordinates.
+trn_mesh* mesh_create_ordered(trn_ctx* ctx, int32_t nv, int32_t nt, const int32_t* tri,
+                              const uint8_t* tags, const int32_t* order_d);
 
 // The metric pass as trn_metric_run runs it, for the adjoint's tape: the
 // gradient (grad), the raw Hessian (hess), the locally normalised metric
