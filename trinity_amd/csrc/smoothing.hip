@@ -188,10 +188,14 @@ __global__ __launch_bounds__(64, MINW) void k_smooth_phase(
 // larger blocks finish sooner).  A block's vertices keep their class order,
 // so each phase block still covers a compact region; the order of the
 // blocks does not change the result, a class being an independent set.
-// Stale vertices are counted as tasks.
+// Stale vertices are counted as tasks.  The block's offsets are an
+// exclusive scan over its (entry round, wave) groups in class order, done
+// by the first kLiveGroups / 64 waves with shuffles.
 constexpr int kLiveB = 1024;
-constexpr int kLivePer = 4;
-constexpr int kLiveGroups = kLivePer * (kLiveB / 64);  // (entry round, wave) groups
+constexpr int kLivePer = 8;
+constexpr int kLiveWaves = kLiveB / 64;
+constexpr int kLiveGroups = kLivePer * kLiveWaves;  // (entry round, wave) groups
+static_assert(kLiveGroups % 64 == 0 && kLiveGroups <= kLiveB, "one scan lane per group");
 
 __global__ __launch_bounds__(kLiveB) void k_class_live(int32_t count,
                                                        const int32_t* __restrict__ order,
@@ -199,7 +203,7 @@ __global__ __launch_bounds__(kLiveB) void k_class_live(int32_t count,
                                                        int32_t* __restrict__ live,
                                                        int32_t* __restrict__ live_count,
                                                        int32_t* __restrict__ counters) {
-  __shared__ int32_t grp_live[kLiveGroups], wave_skip[kLiveB / 64], base;
+  __shared__ int32_t grp_live[kLiveGroups], wave_skip[kLiveWaves], part[kLiveGroups / 64], base;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t first = (int64_t)xcd_block() * (kLiveB * kLivePer) + threadIdx.x;
   const uint64_t below = (1ull << lane) - 1;
@@ -209,35 +213,51 @@ __global__ __launch_bounds__(kLiveB) void k_class_live(int32_t count,
 #pragma unroll
   for (int j = 0; j < kLivePer; ++j) {  // entry first + j * kLiveB: class order is (j, thread)
     const int64_t idx = first + (int64_t)j * kLiveB;
+    v[j] = idx < count ? order[idx] : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < kLivePer; ++j) {
+    const int64_t idx = first + (int64_t)j * kLiveB;
     bool is_live = false, skipped = false;
-    v[j] = 0;
     if (idx < count) {
-      v[j] = order[idx];
       skipped = stale[v[j]] != 0;
       is_live = !skipped;
     }
     ml[j] = __ballot(is_live);
     nskip += (int)__popcll(__ballot(skipped));
-    if (lane == 0) grp_live[j * (kLiveB / 64) + wave] = (int32_t)__popcll(ml[j]);
+    if (lane == 0) grp_live[j * kLiveWaves + wave] = (int32_t)__popcll(ml[j]);
   }
   if (lane == 0) wave_skip[wave] = nskip;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    int32_t nl = 0, nsk = 0;
-    for (int g = 0; g < kLiveGroups; ++g) {
-      const int32_t c = grp_live[g];
-      grp_live[g] = nl;  // exclusive prefix in class order
-      nl += c;
+  int32_t own = 0, incl = 0;
+  if (threadIdx.x < kLiveGroups) {
+    own = grp_live[threadIdx.x];
+    incl = own;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t y = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += y;
     }
-    for (int w = 0; w < kLiveB / 64; ++w) nsk += wave_skip[w];
-    base = nl ? atomicAdd(live_count, nl) : 0;
-    if (nsk) atomicAdd(&counters[2 * (blockIdx.x & (kCounterSlots - 1)) + 1], nsk);
+    if (lane == 63) part[wave] = incl;
+  }
+  __syncthreads();
+  if (threadIdx.x < kLiveGroups) {
+    int32_t before = 0;
+    for (int w = 0; w < wave; ++w) before += part[w];
+    grp_live[threadIdx.x] = before + incl - own;  // exclusive prefix in class order
+    if (threadIdx.x == kLiveGroups - 1) {
+      const int32_t nl = before + incl;
+      int32_t nsk = 0;
+      for (int w = 0; w < kLiveWaves; ++w) nsk += wave_skip[w];
+      base = nl ? atomicAdd(live_count, nl) : 0;
+      if (nsk) atomicAdd(&counters[2 * (blockIdx.x & (kCounterSlots - 1)) + 1], nsk);
+    }
   }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < kLivePer; ++j)
     if ((ml[j] >> lane) & 1)
-      live[base + grp_live[j * (kLiveB / 64) + wave] + (int32_t)__popcll(ml[j] & below)] = v[j];
+      live[base + grp_live[j * kLiveWaves + wave] + (int32_t)__popcll(ml[j] & below)] = v[j];
 }
 
 // counters[s][slot][2] -> out[s][2]
