@@ -79,28 +79,28 @@ __device__ __forceinline__ void stage_union(const int lane, const int32_t* __res
 // Fused staging: the union's per-triangle terms computed in place from the
 // corners of its triangles (C: three vertex ids per union entry, stored with
 // the mesh) with the same arithmetic as k_tri_grad / k_tri_hess, so the
-// sums are bit-identical; kFusedBatch entries per lane in flight.  Lanes
-// past the end read vertex 0 and store nothing.  (4096^2 grid, ms per
-// gradient / Hessian gather, reading the corners through the triangle
-// array: batches of 3: 0.98 / 1.51, 4: 1.01 / 1.54, 6: 1.13 / 1.60, 8:
-// 1.14 / 1.60.)
-constexpr int kFusedBatch = 3;
+// sums are bit-identical; kFusedBatch* entries per lane in flight.  Lanes
+// past the end read vertex 0 and store nothing.  (4096^2 jittered grid on
+// its Z-curve slabs, ms per pass for the whole stage: batches of 3: 9.54,
+// 2: 9.37, 4: 9.49; gradient gather alone 0.77 ms at 3, 0.69 at 2.)
+constexpr int kFusedBatchGrad = 2;
+constexpr int kFusedBatchHess = 2;
 
 __device__ __forceinline__ void stage_grad_terms(const int lane, const int32_t* __restrict__ C,
                                                  const int nu, const double2* __restrict__ xy,
                                                  const double* __restrict__ f, double* lds_a,
                                                  double2* lds_g) {
-  for (int k0 = lane; k0 < nu; k0 += kFusedBatch * kGatherB) {
-    int32_t n[kFusedBatch][3];
+  for (int k0 = lane; k0 < nu; k0 += kFusedBatchGrad * kGatherB) {
+    int32_t n[kFusedBatchGrad][3];
 #pragma unroll
-    for (int c = 0; c < kFusedBatch; ++c) {
+    for (int c = 0; c < kFusedBatchGrad; ++c) {
       const int k = k0 + c * kGatherB;
       const bool in = k < nu;
 #pragma unroll
       for (int j = 0; j < 3; ++j) n[c][j] = in ? C[3 * k + j] : 0;
     }
 #pragma unroll
-    for (int c = 0; c < kFusedBatch; ++c) {
+    for (int c = 0; c < kFusedBatchGrad; ++c) {
       double a;
       double2 ag;
       grad_terms(xy[n[c][0]], xy[n[c][1]], xy[n[c][2]], f[n[c][0]], f[n[c][1]], f[n[c][2]], a,
@@ -118,17 +118,17 @@ __device__ __forceinline__ void stage_hess_terms(const int lane, const int32_t* 
                                                  const int nu, const double2* __restrict__ xy,
                                                  const double2* __restrict__ grad,
                                                  double4* lds_h) {
-  for (int k0 = lane; k0 < nu; k0 += kFusedBatch * kGatherB) {
-    int32_t n[kFusedBatch][3];
+  for (int k0 = lane; k0 < nu; k0 += kFusedBatchHess * kGatherB) {
+    int32_t n[kFusedBatchHess][3];
 #pragma unroll
-    for (int c = 0; c < kFusedBatch; ++c) {
+    for (int c = 0; c < kFusedBatchHess; ++c) {
       const int k = k0 + c * kGatherB;
       const bool in = k < nu;
 #pragma unroll
       for (int j = 0; j < 3; ++j) n[c][j] = in ? C[3 * k + j] : 0;
     }
 #pragma unroll
-    for (int c = 0; c < kFusedBatch; ++c) {
+    for (int c = 0; c < kFusedBatchHess; ++c) {
       const double4 r = hess_terms(xy[n[c][0]], xy[n[c][1]], xy[n[c][2]], grad[n[c][0]],
                                    grad[n[c][1]], grad[n[c][2]]);
       const int k = k0 + c * kGatherB;
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(kGatherB * kGatherWaves) void k_vtx_hess(
     const int32_t* __restrict__ union_cu, const int32_t* __restrict__ union_tri,
     const double2* __restrict__ xy, const double2* __restrict__ grad,
     const double4* __restrict__ ah_t, const double* __restrict__ tarea,
-    double* __restrict__ hess, double* __restrict__ out, double ex) {
+    double* __restrict__ hess, double* __restrict__ out, double* __restrict__ sdet, double ex) {
   extern __shared__ double4 lds_all4[];
   const GatherSlot g = gather_slot();
   const bool live = g.b < (uint32_t)nslab;
@@ -260,14 +260,15 @@ __global__ __launch_bounds__(kGatherB * kGatherWaves) void k_vtx_hess(
   const int i = slab_v ? slab_v[pos] : pos;
   if (nu >= 0) {
     if (rows <= kRegRows)
-      vtx_hess_reg<NORMALIZE>(i, loc, patch_n[i], (const double4*)lds_h, tarea, hess, out, ex);
+      vtx_hess_reg<NORMALIZE>(i, loc, patch_n[i], (const double4*)lds_h, tarea, hess, out, sdet,
+                              ex);
     else
       vtx_hess_one<NORMALIZE>(i, patch_local + (int64_t)r0 * kPatchSlab + g.lane, kPatchSlab,
-                              patch_n[i], (const double4*)lds_h, tarea, hess, out, ex);
+                              patch_n[i], (const double4*)lds_h, tarea, hess, out, sdet, ex);
     return;
   }
   vtx_hess_one<NORMALIZE>(i, patch_ell + (int64_t)patch_row[g.b] * kPatchSlab + g.lane,
-                          kPatchSlab, patch_n[i], ah_t, tarea, hess, out, ex);
+                          kPatchSlab, patch_n[i], ah_t, tarea, hess, out, sdet, ex);
 }
 
 __global__ __launch_bounds__(256) void k_normalize_local(int32_t nv, double* __restrict__ tensor,
@@ -283,17 +284,27 @@ __global__ __launch_bounds__(256) void k_normalize_local(int32_t nv, double* __r
 
 // Fixed-shape deterministic sum: thread g of the fixed grid adds triangles
 // g, g+G, ... in order; waves reduce with a fixed butterfly; blocks write
-// partials that one block adds in a fixed tree.
+// partials that one block adds in a fixed tree.  sdet: the vertices' terms
+// from the normalising Hessian gather (8 bytes per corner read instead of
+// the 24 of its tensor; the same values), else the tensor.
 __global__ __launch_bounds__(kCplxB) void k_tri_cplx(int32_t nt, const int32_t* __restrict__ tri,
                                                      const double2* __restrict__ xy,
                                                      const double* __restrict__ tensor,
+                                                     const double* __restrict__ sdet,
                                                      double* __restrict__ partial) {
   __shared__ double lds[kCplxB / 64];
   double phi = 0.;
+  if (sdet) {
 #pragma unroll 4
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nt;
-       t += (int64_t)gridDim.x * blockDim.x)
-    phi += cplx_one(t, tri, xy, tensor);
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nt;
+         t += (int64_t)gridDim.x * blockDim.x)
+      phi += cplx_one_sdet(t, tri, xy, sdet);
+  } else {
+#pragma unroll 4
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nt;
+         t += (int64_t)gridDim.x * blockDim.x)
+      phi += cplx_one(t, tri, xy, tensor);
+  }
   const double b = block_sum(phi, lds);
   if (threadIdx.x == 0) partial[blockIdx.x] = b;
 }
@@ -333,18 +344,21 @@ constexpr double kBytesCplxPerV = 64.0;    // M24 xy16 tri24
 constexpr double kBytesGlobPerV = 48.0;    // M24 | M24
 
 struct GradBuffers {
-  double* area_t;
-  double2* ag_t;
-  double4* ah_t;
-  double* tarea;
-  double* partial;
-  double* cplx;
+  double* area_t = nullptr;
+  double2* ag_t = nullptr;
+  double4* ah_t = nullptr;
+  double* tarea = nullptr;
+  double* partial = nullptr;
+  double* cplx = nullptr;
+  double* sdet = nullptr;   // per-vertex complexity terms, when carved
+  bool sdet_ready = false;  // written by the last normalising Hessian gather
 };
 
 static size_t grad_scratch_bytes(const trn_mesh* m) {
   return scratch_bytes(sizeof(double) * m->nt) + scratch_bytes(sizeof(double2) * m->nt) +
          scratch_bytes(sizeof(double4) * m->nt) + scratch_bytes(sizeof(double) * m->nv) +
-         scratch_bytes(sizeof(double) * kCplxBlocks) + scratch_bytes(sizeof(double) * 4);
+         scratch_bytes(sizeof(double) * kCplxBlocks) + scratch_bytes(sizeof(double) * 4) +
+         scratch_bytes(sizeof(double) * m->nv);
 }
 
 static GradBuffers carve_grad(Scratch& sc, const trn_mesh* m) {
@@ -355,6 +369,7 @@ static GradBuffers carve_grad(Scratch& sc, const trn_mesh* m) {
   b.tarea = sc.carve<double>(m->nv);
   b.partial = sc.carve<double>(kCplxBlocks);
   b.cplx = sc.carve<double>(4);
+  b.sdet = sc.carve<double>(m->nv);
   return b;
 }
 
@@ -410,7 +425,8 @@ static void run_gradient(trn_ctx* ctx, const trn_mesh* m, const double* xy, cons
 
 static void launch_vtx_hess(trn_ctx* ctx, const trn_mesh* m, const double* xy,
                             const double* grad, const double4* ah_t, const double* tarea,
-                            double* hess, double* out, bool normalize, double ex, int waves) {
+                            double* hess, double* out, double* sdet, bool normalize, double ex,
+                            int waves) {
   const GatherShape g = gather_shape(m, sizeof(double4), waves);
   auto kern = normalize ? k_vtx_hess<true> : k_vtx_hess<false>;
   launch(ctx, normalize ? "vtx_hess_norm" : "vtx_hess", kBytesHessPerV * m->nv, kern, g.grid,
@@ -419,7 +435,7 @@ static void launch_vtx_hess(trn_ctx* ctx, const trn_mesh* m, const double* xy,
          (const int32_t*)m->union_n, (const int32_t*)m->union_off,
          (const int32_t*)m->union_ids, (const uint16_t*)m->patch_local, g.wave_bytes,
          (const int32_t*)m->union_cu, (const int32_t*)m->union_tri, (const double2*)xy,
-         (const double2*)grad, ah_t, tarea, hess, out, ex);
+         (const double2*)grad, ah_t, tarea, hess, out, normalize ? sdet : nullptr, ex);
 }
 
 static void run_hessian(trn_ctx* ctx, const trn_mesh* m, const double* xy, const double* grad,
@@ -427,8 +443,9 @@ static void run_hessian(trn_ctx* ctx, const trn_mesh* m, const double* xy, const
   if (!m->union_all)
     launch(ctx, "tri_hess", 0.0, k_tri_hess, dim3(nblk(m->nt, 256)), dim3(256), 0, m->nt,
            (const int32_t*)m->tri, (const double2*)xy, (const double2*)grad, b.ah_t);
-  launch_vtx_hess(ctx, m, xy, grad, b.ah_t, b.tarea, hess, out, normalize, ex,
+  launch_vtx_hess(ctx, m, xy, grad, b.ah_t, b.tarea, hess, out, b.sdet, normalize, ex,
                   m->tune.hess_waves);
+  b.sdet_ready = normalize && b.sdet;
 }
 
 // ---- shapes the autotuner measures ----------------------------------------
@@ -464,8 +481,8 @@ void tune_gathers(trn_ctx* ctx, const trn_mesh* m, const double* xy, const doubl
     launch(ctx, "tri_hess", 0.0, k_tri_hess, dim3(nblk(m->nt, 256)), dim3(256), 0, m->nt,
            (const int32_t*)m->tri, (const double2*)xy, (const double2*)grad, b.ah_t);
   TRN_HIP(hipEventRecord(ev[2], ctx->stream));
-  launch_vtx_hess(ctx, m, xy, (const double*)grad, b.ah_t, b.tarea, nullptr, tensor, true,
-                  metric_exponent(lp_norm), hess_waves);
+  launch_vtx_hess(ctx, m, xy, (const double*)grad, b.ah_t, b.tarea, nullptr, tensor, nullptr,
+                  true, metric_exponent(lp_norm), hess_waves);
   TRN_HIP(hipEventRecord(ev[3], ctx->stream));
 }
 
@@ -474,7 +491,8 @@ static void run_complexity(trn_ctx* ctx, const trn_mesh* m, const double* xy,
   int blocks = (int)nblk(m->nt, kCplxB);
   if (blocks > kCplxBlocks) blocks = kCplxBlocks;
   launch(ctx, "tri_cplx", kBytesCplxPerV * m->nv, k_tri_cplx, dim3(blocks), dim3(kCplxB), 0,
-         m->nt, (const int32_t*)m->tri, (const double2*)xy, tensor, b.partial);
+         m->nt, (const int32_t*)m->tri, (const double2*)xy, tensor,
+         b.sdet_ready ? (const double*)b.sdet : nullptr, b.partial);
   launch(ctx, "cplx_final", 8.0 * blocks, k_cplx_final, dim3(1), dim3(kCplxB), 0,
          (const double*)b.partial, blocks, b.cplx);
 }

@@ -185,11 +185,19 @@ __device__ __forceinline__ void normalize_local(double& t0, double& t1, double& 
 
 // Patch sums -> symmetric Hessian (src/hessian.cpp:141-152), optionally
 // followed by the local normalisation, stored at out (and hess if given).
+// sdet (when given): sqrt(det) of the normalised tensor, the vertex's term of
+// the complexity (cplx_term), computed once per vertex instead of once per
+// incident triangle.
+__device__ __forceinline__ double cplx_term(const double m0, const double m1, const double m2) {
+  const double det = m0 * m2 - m1 * m1;  // metric.cpp:171-179
+  return __builtin_sqrt(det);
+}
+
 template <bool NORMALIZE>
 __device__ __forceinline__ void hess_finish(const int i, double h0, double h1, double h2,
                                             double h3, const double* __restrict__ tarea,
                                             double* __restrict__ hess, double* __restrict__ out,
-                                            const double ex) {
+                                            double* __restrict__ sdet, const double ex) {
   const double ta = tarea[i];  // same sum as the gradient's (hessian.cpp:120)
   h0 /= ta;
   h1 /= ta;
@@ -205,6 +213,7 @@ __device__ __forceinline__ void hess_finish(const int i, double h0, double h1, d
   out[3 * (int64_t)i] = t0;
   out[3 * (int64_t)i + 1] = t1;
   out[3 * (int64_t)i + 2] = t2;
+  if (NORMALIZE && sdet) sdet[i] = cplx_term(t0, t1, t2);
 }
 
 // Per-vertex Hessian (src/hessian.cpp:141-152) from the patch in the LDS set,
@@ -214,7 +223,7 @@ __device__ __forceinline__ void vtx_hess_one(const int i, const Idx* s, const in
                                              const int m, const double4* __restrict__ ah_t,
                                              const double* __restrict__ tarea,
                                              double* __restrict__ hess, double* __restrict__ out,
-                                             const double ex) {
+                                             double* __restrict__ sdet, const double ex) {
   double h0 = 0., h1 = 0., h2 = 0., h3 = 0.;  // hessian.cpp:93
   for (int e0 = 0; e0 < m; e0 += kPatchBatch) {
     int t[kPatchBatch];
@@ -232,7 +241,7 @@ __device__ __forceinline__ void vtx_hess_one(const int i, const Idx* s, const in
         h3 += r[c].w;
       }
   }
-  hess_finish<NORMALIZE>(i, h0, h1, h2, h3, tarea, hess, out, ex);
+  hess_finish<NORMALIZE>(i, h0, h1, h2, h3, tarea, hess, out, sdet, ex);
 }
 
 template <bool NORMALIZE>
@@ -240,7 +249,7 @@ __device__ __forceinline__ void vtx_hess_reg(const int i, const int (&loc)[kRegR
                                              const int m, const double4* ah,
                                              const double* __restrict__ tarea,
                                              double* __restrict__ hess, double* __restrict__ out,
-                                             const double ex) {
+                                             double* __restrict__ sdet, const double ex) {
   double h0 = 0., h1 = 0., h2 = 0., h3 = 0.;  // hessian.cpp:93
 #pragma unroll
   for (int e = 0; e < kRegRows; ++e)
@@ -251,7 +260,7 @@ __device__ __forceinline__ void vtx_hess_reg(const int i, const int (&loc)[kRegR
       h2 += r.z;
       h3 += r.w;
     }
-  hess_finish<NORMALIZE>(i, h0, h1, h2, h3, tarea, hess, out, ex);
+  hess_finish<NORMALIZE>(i, h0, h1, h2, h3, tarea, hess, out, sdet, ex);
 }
 
 // One triangle's term of Metrics::computeComplexity (src/metric.cpp:161-180).
@@ -271,9 +280,30 @@ __device__ __forceinline__ double cplx_one(const int64_t t, const int32_t* __res
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
     const double* m = tensor + 3 * (int64_t)vs[j];
-    const double det = m[0] * m[2] - m[1] * m[1];
-    rho += __builtin_sqrt(det);
+    rho += cplx_term(m[0], m[1], m[2]);
   }
+  rho /= 3;
+  return rho * area;
+}
+
+// cplx_one with the vertices' terms read from sdet (hess_finish): the same
+// values added in the same order.
+__device__ __forceinline__ double cplx_one_sdet(const int64_t t, const int32_t* __restrict__ tri,
+                                                const double2* __restrict__ xy,
+                                                const double* __restrict__ sdet) {
+  const int32_t v0 = tri[3 * t], v1 = tri[3 * t + 1], v2 = tri[3 * t + 2];
+  if (v0 < 0) return 0.;
+  const double2 p0 = xy[v0], p1 = xy[v1], p2 = xy[v2];
+  const double r0 = sdet[v0], r1 = sdet[v1], r2 = sdet[v2];
+  const double s0 = p1.x - p0.x;  // metric.cpp:163-167
+  const double s1 = p2.x - p0.x;
+  const double s2 = p1.y - p0.y;
+  const double s3 = p2.y - p0.y;
+  const double area = 0.5 * (s0 * s3 - s1 * s2);
+  double rho = 0.;
+  rho += r0;
+  rho += r1;
+  rho += r2;
   rho /= 3;
   return rho * area;
 }
