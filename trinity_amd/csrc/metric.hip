@@ -81,9 +81,10 @@ __device__ __forceinline__ void stage_union(const int lane, const int32_t* __res
 // the mesh) with the same arithmetic as k_tri_grad / k_tri_hess, so the
 // sums are bit-identical; kFusedBatch* entries per lane in flight.  Lanes
 // past the end read vertex 0 and store nothing.  (4096^2 jittered grid on
-// its Z-curve slabs, ms per pass for the whole stage: batches of 3: 9.54,
-// 2: 9.37, 4: 9.49; gradient gather alone 0.77 ms at 3, 0.69 at 2.)
-constexpr int kFusedBatchGrad = 2;
+// its Z-curve slabs, ms per pass with both batches at 3: 9.54, at 2: 9.37,
+// at 4: 9.49; gradient / Hessian batches 1 / 2: 9.32, 2 / 1: 9.37, 2 / 3:
+// 9.42; gradient gather alone 0.77 ms at 3, 0.69 at 2, 0.64 at 1.)
+constexpr int kFusedBatchGrad = 1;
 constexpr int kFusedBatchHess = 2;
 
 __device__ __forceinline__ void stage_grad_terms(const int lane, const int32_t* __restrict__ C,
