@@ -183,16 +183,16 @@ __device__ __forceinline__ void normalize_local(double& t0, double& t1, double& 
   t2 = h[3];
 }
 
-// Patch sums -> symmetric Hessian (src/hessian.cpp:141-152), optionally
-// followed by the local normalisation, stored at out (and hess if given).
-// sdet (when given): sqrt(det) of the normalised tensor, the vertex's term of
-// the complexity (cplx_term), computed once per vertex instead of once per
-// incident triangle.
+// A vertex's term of Metrics::computeComplexity: sqrt(det) of its tensor.
 __device__ __forceinline__ double cplx_term(const double m0, const double m1, const double m2) {
   const double det = m0 * m2 - m1 * m1;  // metric.cpp:171-179
   return __builtin_sqrt(det);
 }
 
+// Patch sums -> symmetric Hessian (src/hessian.cpp:141-152), optionally
+// followed by the local normalisation, stored at out (and hess if given).
+// sdet (when given): the normalised tensor's cplx_term, computed once per
+// vertex instead of once per incident triangle by the complexity kernel.
 template <bool NORMALIZE>
 __device__ __forceinline__ void hess_finish(const int i, double h0, double h1, double h2,
                                             double h3, const double* __restrict__ tarea,
